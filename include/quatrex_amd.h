@@ -1,0 +1,153 @@
+/*
+ * quatrex_amd — C ABI of the MI355X (gfx950) batched RGF + open-boundary path.
+ *
+ * Plain C: opaque handle, plain device pointers, sizes and strides.  Every
+ * array argument is DEVICE memory unless marked "host".  Complex numbers are
+ * interleaved (re, im) doubles (complex128) in all arrays that cross the
+ * boundary; the `precision` argument selects the arithmetic used inside.
+ * Calls are asynchronous on the given HIP stream (hipStream_t passed as
+ * void*, NULL = default stream) and never throw.  Return codes: 0 ok,
+ * >0 warning, <0 error; qx_last_error() gives the message.
+ *
+ * Block-tridiagonal (BTD) stacks use the storage of the reference's DSDBCOO
+ * container (src/qttools/datastructures/dsdbcoo.py:23-1008): one array of
+ * shape (n_stack, stack_stride) whose fully populated blocks are contiguous
+ * row-major slices.  qx_btd_layout gives the element offset of each block
+ * inside one stack entry; an offset of -1 marks a block that is not stored
+ * (read as zero, never written).
+ */
+#ifndef QUATREX_AMD_H
+#define QUATREX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QX_ABI_VERSION 1
+
+/* status codes */
+#define QX_OK 0
+#define QX_NOT_CONVERGED 1   /* warning: iteration limit reached            */
+#define QX_EINVAL (-1)       /* invalid argument                            */
+#define QX_ENOMEM (-2)       /* device allocation failed                    */
+#define QX_EHIP (-3)         /* HIP runtime error                           */
+#define QX_ESINGULAR (-4)    /* exactly singular block met by an inversion  */
+
+/* arithmetic selector */
+#define QX_PREC_C128 0       /* complex128 on FP64 MFMA                      */
+#define QX_PREC_C64 1        /* complex64 on FP32 MFMA, FP64 in/out          */
+
+/* GEMM operand ops */
+#define QX_OP_N 0
+#define QX_OP_C 2            /* conjugate transpose                          */
+
+/* selected_solve flags */
+#define QX_SOLVE_RETARDED 1u         /* write diagonal G^R blocks              */
+#define QX_SOLVE_CURRENT 2u          /* Meir-Wingreen layer current            */
+#define QX_SOLVE_RETARDED_OFFDIAG 4u /* also write off-diagonal G^R blocks     */
+#define QX_SOLVE_SKIP_LOWER_L 8u     /* lesser output is symmetric-stored      */
+#define QX_SOLVE_SKIP_LOWER_G 16u    /* greater output is symmetric-stored     */
+
+typedef struct qx_handle qx_handle;
+
+typedef struct {
+  int32_t n_blocks;
+  int32_t reserved;
+  int64_t n_stack;             /* matrices addressed by this call            */
+  int64_t stack_stride;        /* elements between consecutive matrices      */
+  const int32_t* block_sizes;  /* host [n_blocks]                             */
+  const int64_t* diag;         /* host [n_blocks]    offsets of (i, i)        */
+  const int64_t* upper;        /* host [n_blocks-1]  offsets of (i, i+1)      */
+  const int64_t* lower;        /* host [n_blocks-1]  offsets of (i+1, i)      */
+} qx_btd_layout;
+
+/* Library version of this header (QX_ABI_VERSION). */
+int qx_abi_version(void);
+
+/* Handle bound to HIP device `device`; owns a growable workspace. */
+int qx_create(qx_handle** handle, int device);
+int qx_destroy(qx_handle* handle);
+const char* qx_last_error(const qx_handle* handle);
+
+/* Workspace the RGF calls need for `layout` (bytes), for sizing budgets. */
+int qx_workspace_query(qx_handle* handle, const qx_btd_layout* layout,
+                       int precision, size_t* bytes);
+
+/* Wait for `stream`, then report (and clear) device-side errors recorded
+ * since the last check: QX_ESINGULAR if an inversion met a zero pivot. */
+int qx_sync_check(qx_handle* handle, void* stream);
+
+/* RGF selected inversion.  Replaces RGF.selected_inv
+ * (src/qttools/greens_function_solver/rgf.py:25-113) for one batch of the
+ * stack.  obc_r: host array [n_blocks] of device pointers to
+ * (n_stack, b_i, b_i) contiguous boundary self-energies or NULL entries
+ * (OBCBlocks.retarded, solver.py:12-28); may itself be NULL. */
+int qx_rgf_selected_inv(qx_handle* handle, void* stream,
+                        const qx_btd_layout* la, const void* a,
+                        const void* const* obc_r,
+                        const qx_btd_layout* lx, void* x, int precision);
+
+/* RGF selected solve X^{<>} = A^{-1} S^{<>} A^{-H}.  Replaces
+ * RGF.selected_solve (rgf.py:115-414) for one batch of the stack: G^<, G^>
+ * blocks (diagonal anti-Hermitian-projected, lower = -(upper)^H unless the
+ * SKIP_LOWER flag is set), G^R diagonal (QX_SOLVE_RETARDED) and optionally
+ * off-diagonal blocks, and the layer current (n_stack, n_blocks-1) complex
+ * when QX_SOLVE_CURRENT is set.  obc_*: OBCBlocks.retarded/lesser/greater. */
+int qx_rgf_selected_solve(qx_handle* handle, void* stream,
+                          const qx_btd_layout* la, const void* a,
+                          const qx_btd_layout* lsl, const void* sigma_lesser,
+                          const qx_btd_layout* lsg, const void* sigma_greater,
+                          const void* const* obc_r, const void* const* obc_l,
+                          const void* const* obc_g,
+                          const qx_btd_layout* lxl, void* x_lesser,
+                          const qx_btd_layout* lxg, void* x_greater,
+                          const qx_btd_layout* lxr, void* x_retarded,
+                          void* current, uint32_t flags, int precision);
+
+/* Batched complex GEMM C = alpha op(A) op(B) + beta C, op in {N, C}.
+ * Replaces the batched `@` of the reference (cuBLAS zgemm strided-batched).
+ * alpha/beta: host [2] (re, im). */
+int qx_zgemm_batched(qx_handle* handle, void* stream, int64_t batch, int m,
+                     int n, int k, int op_a, int op_b, const double* alpha,
+                     const void* a, int lda, int64_t stride_a, const void* b,
+                     int ldb, int64_t stride_b, const double* beta, void* c,
+                     int ldc, int64_t stride_c, int precision);
+
+/* Batched inverse with partial pivoting, X = inv(A - S) (S nullable).
+ * Replaces linalg.inv (src/qttools/kernels/linalg/inv.py:24-40).  n <= 512. */
+int qx_inv_batched(qx_handle* handle, void* stream, int64_t batch, int n,
+                   const void* a, int lda, int64_t stride_a, const void* s,
+                   int lds, int64_t stride_s, void* x, int ldx,
+                   int64_t stride_x, int precision);
+
+/* Sancho-Rubio surface Green's function on (n, b, b) stacks.  Replaces
+ * SanchoRubio.__call__ (src/qttools/boundary_conditions/obc/sancho_rubio.py:33-86):
+ * batch-global stop when max_e ||(|alpha|+|beta|)||_F / 2 < tol.  Writes
+ * the iteration count to *iterations (host).  Returns QX_NOT_CONVERGED when
+ * max_iterations is reached (x_ii is still written). */
+int qx_sancho_rubio(qx_handle* handle, void* stream, int64_t n, int b,
+                    const void* a_ii, const void* a_ij, const void* a_ji,
+                    void* x_ii, int max_iterations, double tol,
+                    int* iterations);
+
+/* Beyn contour moments.  Replaces Beyn._contour_integrate
+ * (src/qttools/nevp/beyn.py:177-235) with operator_inverse
+ * (src/qttools/kernels/operator.py:113-179): for the n_quad nodes
+ * z = r exp(2 pi i (k + 1.5) / n_quad), k = 0..n_quad-1 (beyn.py:198-202),
+ * on the outer (r_o) and inner (r_i) circles,
+ *   q0 = sum_k w (z_o T(z_o)^{-1} - z_i T(z_i)^{-1}),
+ *   q1 = sum_k w (z_o^2 T(z_o)^{-1} - z_i^2 T(z_i)^{-1}),  w = 1/n_quad,
+ * T(z) = a_ji / z + a_ii + a_ij z.  All (n, b, b) contiguous. */
+int qx_beyn_moments(qx_handle* handle, void* stream, int64_t n, int b,
+                    int n_quad, double r_o, double r_i, const void* a_ji,
+                    const void* a_ii, const void* a_ij, void* q0, void* q1,
+                    int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUATREX_AMD_H */

@@ -1,0 +1,293 @@
+// Batched complex GEMM on CDNA4 matrix cores (gfx950).
+//
+// C = alpha * op(A) * op(B) + beta * C, op in {N, C (conjugate transpose)},
+// for a strided stack of matrices.  This is the workhorse behind every `@`
+// of the RGF / Sancho-Rubio / Beyn recursions.
+//
+// Design
+// - One workgroup (256 threads = 4 waves of 64) owns a 64 x 64 complex tile
+//   of one matrix of the stack; each wave owns a 32 x 32 sub-tile made of
+//   2 x 2 MFMA 16x16 tiles, with separate real and imaginary accumulators.
+// - The complex product uses the 4M split on real MFMAs:
+//     Cr += Ar Br - Ai Bi ,  Ci += Ar Bi + Ai Br.
+// - Operands are staged through LDS split into real and imaginary planes,
+//   k-major, so that one MFMA operand fetch is 16 consecutive doubles.  The
+//   conjugate transpose is folded into the global->LDS copy (no extra pass).
+// - The next K slice is prefetched into registers while the current one is
+//   multiplied (single LDS buffer, two barriers per slice).
+// - blockIdx is remapped so that the tiles of one matrix land on the same
+//   XCD and share its L2.
+//
+// Precision: PREC_C128 runs v_mfma_f64_16x16x4_f64; PREC_C64 converts the
+// complex128 operands to float on the LDS store and runs
+// v_mfma_f32_16x16x4_f32, converting back on the store of C.
+//
+// MFMA 16x16x4 lane layout (checked on gfx950 hardware): lane l supplies
+// A[row = l % 16][k = l / 16] and B[k = l / 16][col = l % 16].  Result
+// element e of lane l is D[l / 16 + 4 e][l % 16] for the f64 instruction and
+// D[4 (l / 16) + e][l % 16] for the f32 instruction.
+#include "qx_internal.h"
+
+namespace qx {
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16, PAD = 1, NT = 256;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+struct GemmArgs {
+  int m, n, k;
+  double ar, ai, br, bi;
+  const z2* A;
+  int64_t sA;
+  int lda;
+  const z2* B;
+  int64_t sB;
+  int ldb;
+  z2* C;
+  int64_t sC;
+  int ldc;
+  int tilesM, tilesN;
+  int beta_zero;
+};
+
+// Hardware dispatches workgroup ids round-robin over the 8 XCDs.  Return a
+// logical id such that each XCD receives one contiguous logical range.
+__device__ inline int xcd_remap(int bid, int nblocks) {
+  const int nx = 8;
+  const int per = nblocks / nx, rem = nblocks % nx;
+  const int x = bid % nx, j = bid / nx;
+  return x * per + min(x, rem) + j;
+}
+
+template <typename T>
+struct MfmaTraits;
+
+template <>
+struct MfmaTraits<double> {
+  typedef d4 acc_t;
+  static __device__ inline acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of result element e of lane l inside a 16x16 tile
+  static __device__ inline int row(int lane, int e) { return (lane >> 4) + 4 * e; }
+};
+
+template <>
+struct MfmaTraits<float> {
+  typedef f4 acc_t;
+  static __device__ inline acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int e) { return 4 * (lane >> 4) + e; }
+};
+
+template <typename T, int OPA, int OPB>
+__global__ __launch_bounds__(NT) void zgemm_kernel(GemmArgs g) {
+  typedef MfmaTraits<T> MT;
+  typedef typename MT::acc_t acc_t;
+  __shared__ T As[2][BK][BM + PAD];
+  __shared__ T Bs[2][BK][BN + PAD];
+
+  const int id = xcd_remap(blockIdx.x, gridDim.x);
+  const int tiles = g.tilesM * g.tilesN;
+  const int64_t e = id / tiles;
+  const int t = id - (int)(e * tiles);
+  const int m0 = (t / g.tilesN) * BM, n0 = (t % g.tilesN) * BN;
+  const z2* __restrict__ A = g.A + e * g.sA;
+  const z2* __restrict__ B = g.B + e * g.sB;
+  z2* __restrict__ C = g.C + e * g.sC;
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+
+  acc_t accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = acc_t{0, 0, 0, 0};
+      accI[i][j] = acc_t{0, 0, 0, 0};
+    }
+
+  z2 ra[4], rb[4];
+  const z2 zero = make_double2(0.0, 0.0);
+
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + NT * r;
+      if (OPA == QX_OP_N) {  // A is m x k row-major
+        const int row = idx >> 4, col = idx & 15;
+        const int gm = m0 + row, gk = k0 + col;
+        ra[r] = zero;
+        if (gm < g.m && gk < g.k) ra[r] = A[(int64_t)gm * g.lda + gk];
+      } else {  // A stored k x m, op = conj transpose
+        const int kk = idx >> 6, mm = idx & 63;
+        const int gm = m0 + mm, gk = k0 + kk;
+        ra[r] = zero;
+        if (gm < g.m && gk < g.k) ra[r] = A[(int64_t)gk * g.lda + gm];
+      }
+      if (OPB == QX_OP_N) {  // B is k x n row-major
+        const int kk = idx >> 6, nn = idx & 63;
+        const int gn = n0 + nn, gk = k0 + kk;
+        rb[r] = zero;
+        if (gn < g.n && gk < g.k) rb[r] = B[(int64_t)gk * g.ldb + gn];
+      } else {  // B stored n x k, op = conj transpose
+        const int nn = idx >> 4, kk = idx & 15;
+        const int gn = n0 + nn, gk = k0 + kk;
+        rb[r] = zero;
+        if (gn < g.n && gk < g.k) rb[r] = B[(int64_t)gn * g.ldb + gk];
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + NT * r;
+      if (OPA == QX_OP_N) {
+        const int row = idx >> 4, col = idx & 15;
+        As[0][col][row] = (T)ra[r].x;
+        As[1][col][row] = (T)ra[r].y;
+      } else {
+        const int kk = idx >> 6, mm = idx & 63;
+        As[0][kk][mm] = (T)ra[r].x;
+        As[1][kk][mm] = (T)(-ra[r].y);
+      }
+      if (OPB == QX_OP_N) {
+        const int kk = idx >> 6, nn = idx & 63;
+        Bs[0][kk][nn] = (T)rb[r].x;
+        Bs[1][kk][nn] = (T)rb[r].y;
+      } else {
+        const int nn = idx >> 4, kk = idx & 15;
+        Bs[0][kk][nn] = (T)rb[r].x;
+        Bs[1][kk][nn] = (T)(-rb[r].y);
+      }
+    }
+  };
+
+  const int nk = (g.k + BK - 1) / BK;
+  load(0);
+  store();
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      const int kr = kk * 4 + (lane >> 4);
+      T a_r[2], a_i[2], na_i[2], b_r[2], b_i[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int mm = wm * 32 + i * 16 + (lane & 15);
+        a_r[i] = As[0][kr][mm];
+        a_i[i] = As[1][kr][mm];
+        na_i[i] = -a_i[i];
+        const int nn = wn * 32 + i * 16 + (lane & 15);
+        b_r[i] = Bs[0][kr][nn];
+        b_i[i] = Bs[1][kr][nn];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = MT::mma(a_r[i], b_r[j], accR[i][j]);
+          accI[i][j] = MT::mma(a_r[i], b_i[j], accI[i][j]);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = MT::mma(na_i[i], b_i[j], accR[i][j]);
+          accI[i][j] = MT::mma(a_i[i], b_r[j], accI[i][j]);
+        }
+    }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      store();
+      __syncthreads();
+    }
+  }
+
+  // Epilogue: C = alpha * acc + beta * C.
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + wm * 32 + i * 16 + MT::row(lane, q);
+        const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (row < g.m && col < g.n) {
+          const double vr = (double)accR[i][j][q], vi = (double)accI[i][j][q];
+          double outr = g.ar * vr - g.ai * vi, outi = g.ar * vi + g.ai * vr;
+          z2* dst = C + (int64_t)row * g.ldc + col;
+          if (!g.beta_zero) {
+            const z2 c = *dst;
+            outr += g.br * c.x - g.bi * c.y;
+            outi += g.br * c.y + g.bi * c.x;
+          }
+          *dst = make_double2(outr, outi);
+        }
+      }
+}
+
+template <typename T>
+void launch_typed(hipStream_t s, int64_t batch, int opA, int opB,
+                  const GemmArgs& g) {
+  const int64_t blocks = batch * g.tilesM * g.tilesN;
+  QX_REQUIRE(blocks < (int64_t(1) << 31), "zgemm: grid too large");
+  dim3 grid((unsigned)blocks), block(NT);
+  if (opA == QX_OP_N && opB == QX_OP_N)
+    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_N, QX_OP_N>), grid, block, 0, s, g);
+  else if (opA == QX_OP_N && opB == QX_OP_C)
+    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_N, QX_OP_C>), grid, block, 0, s, g);
+  else if (opA == QX_OP_C && opB == QX_OP_N)
+    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_C, QX_OP_N>), grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_C, QX_OP_C>), grid, block, 0, s, g);
+  QX_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+           Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& C,
+           int precision) {
+  QX_REQUIRE(!C.zero(), "zgemm: output is a zero matrix");
+  QX_REQUIRE(opA == QX_OP_N || opA == QX_OP_C, "zgemm: bad op_a");
+  QX_REQUIRE(opB == QX_OP_N || opB == QX_OP_C, "zgemm: bad op_b");
+  if (batch <= 0 || m <= 0 || n <= 0) return;
+  const bool beta_zero = beta.re == 0.0 && beta.im == 0.0;
+  if (A.zero() || B.zero() || k <= 0) {
+    // op(A) op(B) == 0: C = beta C.
+    ew_axpby(s, batch, m, n, beta, C, Cplx{0, 0}, Mat(), C);
+    return;
+  }
+  GemmArgs g;
+  g.m = m;
+  g.n = n;
+  g.k = k;
+  g.ar = alpha.re;
+  g.ai = alpha.im;
+  g.br = beta.re;
+  g.bi = beta.im;
+  g.A = A.p;
+  g.sA = A.stride;
+  g.lda = A.ld;
+  g.B = B.p;
+  g.sB = B.stride;
+  g.ldb = B.ld;
+  g.C = C.p;
+  g.sC = C.stride;
+  g.ldc = C.ld;
+  g.tilesM = (m + BM - 1) / BM;
+  g.tilesN = (n + BN - 1) / BN;
+  g.beta_zero = beta_zero ? 1 : 0;
+  if (precision == QX_PREC_C64)
+    launch_typed<float>(s, batch, opA, opB, g);
+  else
+    launch_typed<double>(s, batch, opA, opB, g);
+}
+
+}  // namespace qx

@@ -1,0 +1,107 @@
+// Internal declarations shared by the quatrex_amd HIP sources (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/quatrex_amd.h"
+
+namespace qx {
+
+// Complex128 element as stored in every array that crosses the C ABI.
+using z2 = double2;
+
+// A stack of dense row-major matrices: element (r, c) of matrix e sits at
+// p[e * stride + r * ld + c].  p == nullptr denotes the zero matrix.
+struct Mat {
+  z2* p = nullptr;
+  int64_t stride = 0;
+  int ld = 0;
+  Mat() = default;
+  Mat(z2* p_, int64_t s_, int ld_) : p(p_), stride(s_), ld(ld_) {}
+  Mat(const void* p_, int64_t s_, int ld_)
+      : p(const_cast<z2*>(static_cast<const z2*>(p_))), stride(s_), ld(ld_) {}
+  bool zero() const { return p == nullptr; }
+};
+
+struct Cplx {
+  double re, im;
+};
+
+// Status carried up to the C ABI.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define QX_HIP_CHECK(expr)                                                   \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      throw ::qx::Error{QX_EHIP, std::string(#expr) + ": " +                 \
+                                     hipGetErrorString(e_)};                 \
+  } while (0)
+
+#define QX_REQUIRE(cond, msg)                                                \
+  do {                                                                       \
+    if (!(cond)) throw ::qx::Error{QX_EINVAL, (msg)};                        \
+  } while (0)
+
+// ---- kernels (launchers) --------------------------------------------------
+
+// C = alpha op(A) op(B) + beta C for `batch` matrices; m x n output, inner k.
+void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+           Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& C,
+           int precision);
+
+// X = inv(A - S) with partial pivoting (S may be zero).  info[e] is set to a
+// nonzero value for an exactly singular matrix.  n <= kInvMaxN.
+constexpr int kInvMaxN = 512;
+void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S,
+          const Mat& X, int* info, int precision);
+
+// Elementwise helpers on batched m x n blocks.
+// Y = a*A + b*B            (A, B may be zero matrices)
+void ew_axpby(hipStream_t s, int64_t batch, int m, int n, Cplx a, const Mat& A,
+              Cplx b, const Mat& B, const Mat& Y);
+// Y = scale * (X - X^H)    (square)
+void ew_antiherm(hipStream_t s, int64_t batch, int n, double scale,
+                 const Mat& X, const Mat& Y);
+// Y += T^H - T             (square)
+void ew_add_hdiff(hipStream_t s, int64_t batch, int n, const Mat& T,
+                  const Mat& Y);
+// Y = -X^H                 (X is m x n, Y is n x m)
+void ew_neg_h(hipStream_t s, int64_t batch, int m, int n, const Mat& X,
+              const Mat& Y);
+// out[e * out_stride] = tr(X Y - Z W)   (square; complex)
+void trace_diff(hipStream_t s, int64_t batch, int n, const Mat& X, const Mat& Y,
+                const Mat& Z, const Mat& W, z2* out, int64_t out_stride);
+// out[e] = || |A| + |B| ||_F   (square)
+void absnorm2(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& B,
+              double* out);
+// Beyn: T = a_ji / z + a_ii + a_ij z for each (e, q) node; result stacks
+// ordered e-major then node.
+void contour_operator(hipStream_t s, int64_t batch, int nq, int b,
+                      const Mat& aji, const Mat& aii, const Mat& aij,
+                      const z2* z, const Mat& T);
+// q0 += sum_q w0[q] Tinv[e, q]; q1 += sum_q w1[q] Tinv[e, q]
+void contour_accumulate(hipStream_t s, int64_t batch, int nq, int b,
+                        const Mat& Tinv, const z2* w0, const z2* w1,
+                        const Mat& q0, const Mat& q1);
+
+}  // namespace qx
+
+// Handle: device, workspace and sticky error message.
+struct qx_handle {
+  int device = 0;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  int* info = nullptr;  // device error flags
+  size_t info_len = 0;
+  std::string last_error;
+  void* ensure_ws(size_t bytes);
+  int* ensure_info(size_t n);
+};

@@ -1,0 +1,324 @@
+// Host drivers of the RGF recursions (selected inversion / selected solve).
+//
+// Restates src/qttools/greens_function_solver/rgf.py:25-414 as a sequence of
+// batched device kernels on one stream: every block product is one launch of
+// the MFMA GEMM over all matrices of the batch, every block inverse one
+// launch of the batched Gauss-Jordan inverse.  Inputs and outputs stay in
+// the caller's DSDBCOO-layout arrays (fully populated blocks are contiguous
+// row-major slices, addressed with stack stride = nnz); the forward-sweep
+// blocks live in a handle-owned workspace.  Products with a block that is not
+// stored (e.g. Sigma_{i,i+1} of a block-diagonal self-energy) are skipped:
+// they are exactly zero.
+#include <algorithm>
+#include <cstring>
+
+#include "qx_internal.h"
+
+namespace qx {
+
+namespace {
+
+const Cplx kOne{1.0, 0.0}, kMinus{-1.0, 0.0}, kZero{0.0, 0.0}, kHalf{0.5, 0.0};
+
+struct Btd {
+  const qx_btd_layout* L = nullptr;
+  z2* base = nullptr;
+  Btd() = default;
+  Btd(const qx_btd_layout* l, const void* p) : L(l), base(const_cast<z2*>(static_cast<const z2*>(p))) {}
+  int bs(int i) const { return L->block_sizes[i]; }
+  Mat at(int64_t off, int ld) const {
+    if (!L || !base || off < 0) return Mat();
+    return Mat(base + off, L->stack_stride, ld);
+  }
+  Mat diag(int i) const { return L ? at(L->diag[i], bs(i)) : Mat(); }
+  Mat upper(int i) const { return L ? at(L->upper[i], bs(i + 1)) : Mat(); }  // (i, i+1)
+  Mat lower(int i) const { return L ? at(L->lower[i], bs(i)) : Mat(); }      // (i+1, i)
+};
+
+void check_layout(const qx_btd_layout* l, const qx_btd_layout* ref, const char* what) {
+  if (!l) return;
+  QX_REQUIRE(l->n_blocks == ref->n_blocks, std::string(what) + ": block count differs from A");
+  QX_REQUIRE(l->n_stack == ref->n_stack, std::string(what) + ": stack size differs from A");
+  for (int i = 0; i < l->n_blocks; ++i)
+    QX_REQUIRE(l->block_sizes[i] == ref->block_sizes[i],
+               std::string(what) + ": block sizes differ from A");
+}
+
+// Workspace of equally sized matrix stacks ("slots"), each S x bmax^2.
+struct Slots {
+  z2* base;
+  int64_t S;
+  int64_t cap;  // elements per matrix
+  Mat get(int k, int ld) const { return Mat(base + (int64_t)k * S * cap, cap, ld); }
+};
+
+// dst = a + sign * s (either may be zero); dst is a workspace matrix.
+void fold(hipStream_t st, int64_t S, int m, int n, const Mat& a, double sign,
+          const Mat& s, const Mat& dst) {
+  ew_axpby(st, S, m, n, kOne, a, Cplx{sign, 0.0}, s, dst);
+}
+
+Mat obc_mat(const void* const* arr, int i, int b) {
+  if (!arr || !arr[i]) return Mat();
+  return Mat(arr[i], (int64_t)b * b, b);
+}
+
+}  // namespace
+
+size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve) {
+  int bmax = 1;
+  for (int i = 0; i < la->n_blocks; ++i) bmax = std::max(bmax, (int)la->block_sizes[i]);
+  const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 20 : (int64_t)la->n_blocks + 8;
+  return (size_t)(slots * la->n_stack * (int64_t)bmax * bmax) * sizeof(z2);
+}
+
+void rgf_selected_inv(qx_handle* h, hipStream_t st, const qx_btd_layout* la,
+                      const void* a, const void* const* obc_r,
+                      const qx_btd_layout* lx, void* x, int prec) {
+  QX_REQUIRE(la && lx && a && x, "selected_inv: null argument");
+  QX_REQUIRE(la->n_blocks >= 1, "selected_inv: need at least one block");
+  check_layout(lx, la, "x");
+  const int N = la->n_blocks;
+  const int64_t S = la->n_stack;
+  if (S == 0) return;
+  int bmax = 1;
+  for (int i = 0; i < N; ++i) {
+    QX_REQUIRE(la->block_sizes[i] >= 1 && la->block_sizes[i] <= kInvMaxN,
+               "selected_inv: block sizes must be in [1, 512]");
+    bmax = std::max(bmax, (int)la->block_sizes[i]);
+  }
+  Btd A(la, a), X(lx, x);
+  Slots W{static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, false))), S, (int64_t)bmax * bmax};
+  int* info = h->ensure_info(S);
+  std::vector<int> xd(N);
+  for (int i = 0; i < N; ++i) xd[i] = i;
+  int spare = N, t1 = N + 1, t2 = N + 2, t3 = N + 3;
+  auto bs = [&](int i) { return A.bs(i); };
+
+  // Forward sweep (rgf.py:70-92).
+  fold(st, S, bs(0), bs(0), A.diag(0), -1.0, obc_mat(obc_r, 0, bs(0)), W.get(xd[0], bs(0)));
+  zinv(st, S, bs(0), W.get(xd[0], bs(0)), Mat(), W.get(xd[0], bs(0)), info, prec);
+  for (int i = 0; i + 1 < N; ++i) {
+    const int j = i + 1, bi = bs(i), bj = bs(j);
+    const Mat T1 = W.get(t1, bi);
+    zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get(xd[i], bi), kZero, T1, prec);
+    const Mat Xj = W.get(xd[j], bj);
+    fold(st, S, bj, bj, A.diag(j), -1.0, obc_mat(obc_r, j, bj), Xj);
+    zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xj, prec);
+    zinv(st, S, bj, Xj, Mat(), Xj, info, prec);
+  }
+  // rgf.py:94-95
+  if (!X.diag(N - 1).zero())
+    ew_axpby(st, S, bs(N - 1), bs(N - 1), kOne, W.get(xd[N - 1], bs(N - 1)), kZero, Mat(), X.diag(N - 1));
+
+  // Backward sweep (rgf.py:97-110).
+  for (int i = N - 2; i >= 0; --i) {
+    const int j = i + 1, bi = bs(i), bj = bs(j);
+    const Mat Xii = W.get(xd[i], bi), Xjj = W.get(xd[j], bj);
+    const Mat T1 = W.get(t1, bi), T2 = W.get(t2, bj);
+    // x_ji = -x_jj a_ji x_ii
+    zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xjj, A.lower(i), kZero, T1, prec);
+    Mat Xji = X.lower(i);
+    if (Xji.zero()) Xji = W.get(t3, bi);
+    zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, T1, Xii, kZero, Xji, prec);
+    // x_ij = -x_ii a_ij x_jj
+    zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xii, A.upper(i), kZero, T2, prec);
+    if (!X.upper(i).zero())
+      zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xjj, kZero, X.upper(i), prec);
+    // x_ii <- x_ii - x_ii a_ij x_ji
+    const Mat Z = W.get(spare, bi);
+    ew_axpby(st, S, bi, bi, kOne, Xii, kZero, Mat(), Z);
+    zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xji, kOne, Z, prec);
+    std::swap(spare, xd[i]);
+    if (!X.diag(i).zero())
+      ew_axpby(st, S, bi, bi, kOne, W.get(xd[i], bi), kZero, Mat(), X.diag(i));
+  }
+}
+
+void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la,
+                        const void* a, const qx_btd_layout* lsl, const void* sl,
+                        const qx_btd_layout* lsg, const void* sg,
+                        const void* const* obc_r, const void* const* obc_l,
+                        const void* const* obc_g, const qx_btd_layout* lxl,
+                        void* xl, const qx_btd_layout* lxg, void* xg,
+                        const qx_btd_layout* lxr, void* xr, void* current,
+                        uint32_t flags, int prec) {
+  QX_REQUIRE(la && a, "selected_solve: null A");
+  QX_REQUIRE(lxl && xl && lxg && xg, "selected_solve: null lesser/greater output");
+  const bool want_r = (flags & QX_SOLVE_RETARDED) != 0;
+  const bool want_r_off = (flags & QX_SOLVE_RETARDED_OFFDIAG) != 0;
+  const bool want_cur = (flags & QX_SOLVE_CURRENT) != 0;
+  QX_REQUIRE(!want_r || (lxr && xr), "selected_solve: retarded output requested but null");
+  QX_REQUIRE(!want_cur || current, "selected_solve: current requested but null");
+  check_layout(lsl, la, "sigma_lesser");
+  check_layout(lsg, la, "sigma_greater");
+  check_layout(lxl, la, "x_lesser");
+  check_layout(lxg, la, "x_greater");
+  if (want_r) check_layout(lxr, la, "x_retarded");
+  const int N = la->n_blocks;
+  const int64_t S = la->n_stack;
+  QX_REQUIRE(N >= 1, "selected_solve: need at least one block");
+  if (S == 0) return;
+  int bmax = 1;
+  for (int i = 0; i < N; ++i) {
+    QX_REQUIRE(la->block_sizes[i] >= 1 && la->block_sizes[i] <= kInvMaxN,
+               "selected_solve: block sizes must be in [1, 512]");
+    bmax = std::max(bmax, (int)la->block_sizes[i]);
+  }
+  Btd A(la, a), SL(lsl, sl), SG(lsg, sg), XL(lxl, xl), XG(lxg, xg),
+      XR(want_r ? lxr : nullptr, want_r ? xr : nullptr);
+  Slots W{static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, true))), S, (int64_t)bmax * bmax};
+  int* info = h->ensure_info(S);
+  auto bs = [&](int i) { return A.bs(i); };
+
+  // slot bookkeeping: forward blocks (rotated in the backward sweep) + temps
+  std::vector<int> sr(N), sl_(N), sg_(N);
+  for (int i = 0; i < N; ++i) {
+    sr[i] = i;
+    sl_[i] = N + i;
+    sg_[i] = 2 * N + i;
+  }
+  int spare = 3 * N;
+  const int T1s = 3 * N + 1, T2s = 3 * N + 2, T3s = 3 * N + 3, T4s = 3 * N + 4, T5s = 3 * N + 5,
+            T6s = 3 * N + 6, B1s = 3 * N + 7, B2s = 3 * N + 8, B3s = 3 * N + 9, B4s = 3 * N + 10,
+            C1s = 3 * N + 11, C2s = 3 * N + 12, C3s = 3 * N + 13, C4s = 3 * N + 14;
+
+  struct Q {
+    Btd* sig;
+    Btd* out;
+    std::vector<int>* slot;
+    const void* const* obc;
+    bool skip_lower;
+  };
+  Q qs[2] = {{&SL, &XL, &sl_, obc_l, (flags & QX_SOLVE_SKIP_LOWER_L) != 0},
+             {&SG, &XG, &sg_, obc_g, (flags & QX_SOLVE_SKIP_LOWER_G) != 0}};
+
+  // ---- block 0 (rgf.py:207-231) ----
+  {
+    const int b0 = bs(0);
+    const Mat X0 = W.get(sr[0], b0);
+    fold(st, S, b0, b0, A.diag(0), -1.0, obc_mat(obc_r, 0, b0), X0);
+    zinv(st, S, b0, X0, Mat(), X0, info, prec);
+    for (auto& q : qs) {
+      const Mat T3 = W.get(T3s, b0), T6 = W.get(T6s, b0);
+      fold(st, S, b0, b0, q.sig->diag(0), 1.0, obc_mat(q.obc, 0, b0), T3);
+      zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_N, kOne, X0, T3, kZero, T6, prec);
+      zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_C, kOne, T6, X0, kZero, W.get((*q.slot)[0], b0), prec);
+    }
+  }
+
+  // ---- forward sweep (rgf.py:234-295) ----
+  for (int i = 0; i + 1 < N; ++i) {
+    const int j = i + 1, bi = bs(i), bj = bs(j);
+    const Mat Xri = W.get(sr[i], bi), Xrj = W.get(sr[j], bj);
+    const Mat T1 = W.get(T1s, bi);
+    // a_ji xr_ii
+    zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), Xri, kZero, T1, prec);
+    // xr_jj = inv(a_jj - obc - a_ji xr_ii a_ij)
+    fold(st, S, bj, bj, A.diag(j), -1.0, obc_mat(obc_r, j, bj), Xrj);
+    zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xrj, prec);
+    zinv(st, S, bj, Xrj, Mat(), Xrj, info, prec);
+    for (auto& q : qs) {
+      const Mat T3 = W.get(T3s, bj), T4 = W.get(T4s, bi), T5 = W.get(T5s, bj), T6 = W.get(T6s, bj);
+      // sigma_jj + obc + a_ji x_ii a_ji^H + u^H - u,  u = a_ji xr_ii sigma_ij
+      fold(st, S, bj, bj, q.sig->diag(j), 1.0, obc_mat(q.obc, j, bj), T3);
+      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get((*q.slot)[i], bi), kZero, T4, prec);
+      zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, A.lower(i), kOne, T3, prec);
+      const Mat sij = q.sig->upper(i);
+      if (!sij.zero()) {
+        zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kOne, T1, sij, kZero, T5, prec);
+        ew_add_hdiff(st, S, bj, T5, T3);
+      }
+      zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_N, kOne, Xrj, T3, kZero, T6, prec);
+      zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_C, kOne, T6, Xrj, kZero, W.get((*q.slot)[j], bj), prec);
+    }
+  }
+
+  // ---- last diagonal blocks (rgf.py:297-305) ----
+  {
+    const int bn = bs(N - 1);
+    for (auto& q : qs)
+      if (!q.out->diag(N - 1).zero())
+        ew_antiherm(st, S, bn, 0.5, W.get((*q.slot)[N - 1], bn), q.out->diag(N - 1));
+    if (want_r && !XR.diag(N - 1).zero())
+      ew_axpby(st, S, bn, bn, kOne, W.get(sr[N - 1], bn), kZero, Mat(), XR.diag(N - 1));
+  }
+
+  // ---- backward sweep (rgf.py:308-404) ----
+  for (int i = N - 2; i >= 0; --i) {
+    const int j = i + 1, bi = bs(i), bj = bs(j);
+    const Mat Xri = W.get(sr[i], bi), Xrj = W.get(sr[j], bj);
+    const Mat Aij = A.upper(i), Aji = A.lower(i);
+    const Mat B1 = W.get(B1s, bj), B2 = W.get(B2s, bi), B3 = W.get(B3s, bj), B4 = W.get(B4s, bi);
+    zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, Aij, kZero, B1, prec);  // xr_ii a_ij
+    zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xrj, Aji, kZero, B2, prec);  // xr_jj a_ji
+    zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kOne, B1, Xrj, kZero, B3, prec);   // xr_ii a_ij xr_jj
+    zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kOne, B1, B2, kZero, B4, prec);    // ... a_ji
+
+    if (want_cur) {
+      // Meir-Wingreen layer current from the forward (left-connected) blocks
+      // of layer i and the full blocks of layer j (rgf.py:381-400).
+      const Mat C1 = W.get(C1s, bi), C2 = W.get(C2s, bj), C3 = W.get(C3s, bj), C4 = W.get(C4s, bj);
+      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, Aji, Xri, kZero, C1, prec);  // a_ji xr_ii
+      Mat tl[2] = {C2, C3};
+      for (int qi = 0; qi < 2; ++qi) {
+        const Q& q = qs[qi];
+        const Mat T4 = W.get(T4s, bi);
+        zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, Aji, W.get((*q.slot)[i], bi), kZero, T4, prec);
+        zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, Aji, kZero, tl[qi], prec);
+        const Mat sij = q.sig->upper(i);
+        if (!sij.zero()) {
+          zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kOne, C1, sij, kZero, C4, prec);
+          ew_add_hdiff(st, S, bj, C4, tl[qi]);
+        }
+      }
+      // tr(sigma_greater_tilde xl_jj - xg_jj sigma_lesser_tilde)
+      trace_diff(st, S, bj, C3, W.get(sl_[j], bj), W.get(sg_[j], bj), C2,
+                 static_cast<z2*>(current) + i, N - 1);
+    }
+
+    for (auto& q : qs) {
+      const Mat Xi = W.get((*q.slot)[i], bi), Xj = W.get((*q.slot)[j], bj);
+      const Mat T1 = W.get(T1s, bi), T2 = W.get(T2s, bi), T3 = W.get(T3s, bj), T5 = W.get(T5s, bj);
+      const Mat sij = q.sig->upper(i);
+      // temp_1x = B4 x_ii - xr_ii s_ij B3^H ;  temp_1x -= temp_1x^H
+      zgemm(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xi, kZero, T1, prec);
+      if (!sij.zero()) {
+        zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, sij, kZero, T5, prec);
+        zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, T5, B3, kOne, T1, prec);
+      }
+      ew_antiherm(st, S, bi, 1.0, T1, T2);
+      // temp_2x = B1 x_jj
+      zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kOne, B1, Xj, kZero, T3, prec);
+      // x_ij = -temp_2x - x_ii B2^H + xr_ii s_ij xr_jj^H
+      Mat Oij = q.out->upper(i);
+      if (Oij.zero()) Oij = W.get(T4s, bj);
+      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, Xi, B2, kZero, Oij, prec);
+      ew_axpby(st, S, bi, bj, kOne, Oij, kMinus, T3, Oij);
+      if (!sij.zero())
+        zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_C, kOne, T5, Xrj, kOne, Oij, prec);
+      if (!q.skip_lower && !q.out->lower(i).zero()) ew_neg_h(st, S, bi, bj, Oij, q.out->lower(i));
+      // x_ii <- x_ii + temp_2x B1^H + temp_1x
+      const Mat Z = W.get(spare, bi);
+      ew_axpby(st, S, bi, bi, kOne, Xi, kOne, T2, Z);
+      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kOne, T3, B1, kOne, Z, prec);
+      std::swap(spare, (*q.slot)[i]);
+      if (!q.out->diag(i).zero()) ew_antiherm(st, S, bi, 0.5, Z, q.out->diag(i));
+    }
+
+    // retarded: xr_ii <- xr_ii + B4 xr_ii ; off-diagonal blocks on request
+    if (want_r && want_r_off) {
+      if (!XR.upper(i).zero()) ew_axpby(st, S, bi, bj, kMinus, B3, kZero, Mat(), XR.upper(i));
+      if (!XR.lower(i).zero())
+        zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, B2, Xri, kZero, XR.lower(i), prec);
+    }
+    const Mat Z = W.get(spare, bi);
+    ew_axpby(st, S, bi, bi, kOne, Xri, kZero, Mat(), Z);
+    zgemm(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xri, kOne, Z, prec);
+    std::swap(spare, sr[i]);
+    if (want_r && !XR.diag(i).zero()) ew_axpby(st, S, bi, bi, kOne, Z, kZero, Mat(), XR.diag(i));
+  }
+  (void)kHalf;
+}
+
+}  // namespace qx

@@ -1,0 +1,291 @@
+"""Green's-function solvers: drop-in ``GFSolver`` / ``RGF`` on the HIP path.
+
+Mirrors src/qttools/greens_function_solver/solver.py:12-113 (``OBCBlocks``,
+``GFSolver``) and rgf.py:10-414 (``RGF``): same signatures, same batching
+over the first stack axis (``get_batches``), same outputs and errors:
+
+- ``selected_inv(a, obc_blocks=None, out=None)`` returns the selected inverse
+  (diagonal and both off-diagonal block bands) or writes ``out``.
+- ``selected_solve(a, sigma_lesser, sigma_greater, obc_blocks=None, out=None,
+  return_retarded=False, return_current=False)`` returns ``(xl, xg[, xr]
+  [, current])`` or writes ``out=(xl, xg[, xr])`` and returns ``current`` or
+  None; ``ValueError("Invalid number of output matrices.")`` as rgf.py:185-186.
+  Diagonal lesser/greater blocks are the anti-Hermitian projection
+  0.5 (X - X^H), lower blocks are -(upper)^H unless the output container is
+  symmetric-stored, and only diagonal retarded blocks are written (unless
+  ``RGF(retarded_offdiag=True)``).
+
+Every block product and inverse runs in libquatrex_amd on the tensors'
+HIP device; the containers' fully populated blocks are read and written in
+place.  An exactly singular block raises ``numpy.linalg.LinAlgError`` as the
+reference's NumPy backend does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from abc import ABC, abstractmethod
+
+import numpy as np
+import torch
+
+from quatrex_amd import _lib
+from quatrex_amd.datastructures import DSDBCOO
+from quatrex_amd.utils import get_batches
+
+
+class OBCBlocks:
+    """Per-block boundary self-energies (solver.py:12-28)."""
+
+    def __init__(self, num_blocks: int):
+        self.retarded: list = [None] * num_blocks
+        self.lesser: list = [None] * num_blocks
+        self.greater: list = [None] * num_blocks
+
+
+class GFSolver(ABC):
+    """Abstract Green's-function solver (solver.py:31-113)."""
+
+    @abstractmethod
+    def selected_inv(self, a, obc_blocks=None, out=None):
+        ...
+
+    @abstractmethod
+    def selected_solve(self, a, sigma_lesser, sigma_greater, obc_blocks=None, out=None,
+                       return_retarded=False, return_current=False):
+        ...
+
+
+class _Bound:
+    """One container's layout for one stack batch, possibly via a staging buffer."""
+
+    def __init__(self, mat: DSDBCOO, s0: int, s1: int, *, output: bool,
+                 write_lower: bool = True, write_offdiag: bool = True):
+        if not isinstance(mat, DSDBCOO):
+            raise TypeError("quatrex_amd solvers take quatrex_amd.datastructures.DSDBCOO")
+        _lib.require_device_tensor(mat.data, "DSDBCOO.data")
+        if mat.dtype != torch.complex128:
+            raise TypeError("quatrex_amd solvers store complex128 data "
+                            f"(got {mat.dtype}); choose the arithmetic with RGF(precision=...)")
+        self.mat, self.s0, self.s1, self.output = mat, s0, s1, output
+        n = mat.num_blocks
+        inner = int(np.prod(mat.stack_shape[1:], dtype=np.int64))
+        self.S = (s1 - s0) * inner
+        bs = mat.block_sizes
+        want = {}
+        for i in range(n):
+            want[("d", i)] = (i, i)
+            if i + 1 < n and write_offdiag:
+                want[("u", i)] = (i, i + 1)
+                if write_lower:
+                    want[("l", i)] = (i + 1, i)
+        direct, staged = {}, []
+        for key, (i, j) in want.items():
+            if mat.symmetry and (j < i or i == j):
+                # symmetric storage: diagonal blocks hold a triangle, lower
+                # blocks are implied -> densify through a staging buffer
+                present = mat.block_slice(min(i, j), max(i, j)) is not None
+                if present:
+                    staged.append(key)
+                continue
+            sl = mat.block_slice(i, j)
+            if sl is None:
+                continue
+            if mat.is_dense_block(i, j):
+                direct[key] = sl.start
+            else:
+                staged.append(key)
+        self.staging = None
+        diag = np.full(n, -1, np.int64)
+        upper = np.full(max(n - 1, 0), -1, np.int64)
+        lower = np.full(max(n - 1, 0), -1, np.int64)
+        arrs = {"d": diag, "u": upper, "l": lower}
+        if not staged:
+            for (kind, i), off in direct.items():
+                arrs[kind][i] = off
+            self.base = mat.data.data_ptr() + s0 * inner * mat.nnz * mat.data.element_size()
+            stride = mat.nnz
+        else:
+            # all blocks of this container go through one dense staging buffer
+            keys = sorted(set(direct) | set(staged), key=lambda k: (k[1], k[0]))
+            off, offs = 0, {}
+            for key in keys:
+                i, j = want[key]
+                offs[key] = off
+                off += int(bs[i]) * int(bs[j])
+            self.staging = torch.zeros((self.S, max(off, 1)), dtype=torch.complex128,
+                                       device=mat.device)
+            self._staged_blocks = {}
+            view = mat.stack[s0:s1]
+            for key in keys:
+                i, j = want[key]
+                arrs[key[0]][key[1]] = offs[key]
+                seg = self.staging[:, offs[key]:offs[key] + int(bs[i]) * int(bs[j])]
+                self._staged_blocks[key] = (i, j, seg)
+                if not output:
+                    seg.copy_(view.blocks[i, j].reshape(self.S, -1))
+            self.base = self.staging.data_ptr()
+            stride = self.staging.shape[1]
+        self.layout, self._keep = _lib.make_layout(bs, self.S, stride, diag, upper, lower)
+
+    def ptr(self):
+        return ctypes.c_void_p(self.base)
+
+    def lay(self):
+        return ctypes.byref(self.layout)
+
+    def write_back(self):
+        if self.staging is None or not self.output:
+            return
+        view = self.mat.stack[self.s0:self.s1]
+        lead = (self.s1 - self.s0,) + tuple(self.mat.stack_shape[1:])
+        for key, (i, j, seg) in self._staged_blocks.items():
+            bi, bj = int(self.mat.block_sizes[i]), int(self.mat.block_sizes[j])
+            view.blocks[i, j] = seg.reshape(lead + (bi, bj))
+
+
+def _obc_array(blocks, s0, s1, n, device, keep):
+    """Host array of device pointers to the (S, b, b) slices, or None."""
+    if blocks is None or all(b is None for b in blocks):
+        return None
+    arr = (ctypes.c_void_p * n)()
+    for j in range(n):
+        b = blocks[j]
+        if b is None:
+            arr[j] = None
+            continue
+        t = torch.as_tensor(b)
+        if t.device.type != "cuda":
+            t = t.to(device)
+        t = t[s0:s1].to(torch.complex128).contiguous()
+        keep.append(t)
+        arr[j] = t.data_ptr()
+    return arr
+
+
+class RGF(GFSolver):
+    """Recursive Green's function solver on the HIP path (rgf.py:10-23).
+
+    Parameters
+    ----------
+    max_batch_size : int
+        Largest number of first-axis stack entries per batch (as the
+        reference; batches may exceed it, see ``get_batches``).
+    precision : {"c128", "c64"}
+        Arithmetic of the block products and inverses: complex128 on FP64
+        matrix cores, or complex64 on FP32 matrix cores (inputs/outputs stay
+        complex128).
+    retarded_offdiag : bool
+        Also write the off-diagonal retarded blocks in ``selected_solve``.
+    """
+
+    def __init__(self, max_batch_size: int = 100, precision: str = "c128",
+                 retarded_offdiag: bool = False) -> None:
+        self.max_batch_size = max_batch_size
+        if precision not in _lib.PRECISIONS:
+            raise ValueError(f"Unknown precision {precision!r}")
+        self.precision = precision
+        self.retarded_offdiag = retarded_offdiag
+
+    # rgf.py:25-113
+    def selected_inv(self, a: DSDBCOO, obc_blocks: OBCBlocks | None = None,
+                     out: DSDBCOO | None = None):
+        lib = _lib.load()
+        _lib.require_device_tensor(a.data, "a.data")
+        dev = a.device
+        if obc_blocks is None:
+            obc_blocks = OBCBlocks(num_blocks=a.num_blocks)
+        x = out if out is not None else a.__class__.zeros_like(a)
+        sizes, slices = get_batches(a.shape[0], self.max_batch_size)
+        h, st = _lib.handle(dev), _lib.stream_ptr(dev)
+        prec = _lib.PRECISIONS[self.precision]
+        for b in range(len(sizes)):
+            s0, s1 = int(slices[b]), int(slices[b + 1])
+            keep: list = []
+            A = _Bound(a, s0, s1, output=False)
+            X = _Bound(x, s0, s1, output=True, write_lower=not x.symmetry)
+            obc = _obc_array(obc_blocks.retarded, s0, s1, a.num_blocks, dev, keep)
+            rc = lib.qx_rgf_selected_inv(h, st, A.lay(), A.ptr(), obc, X.lay(), X.ptr(), prec)
+            _lib.check(rc, h, "RGF.selected_inv")
+            X.write_back()
+        _lib.sync_check(dev, "RGF.selected_inv")
+        if out is None:
+            return x
+        return None
+
+    # rgf.py:115-414
+    def selected_solve(self, a: DSDBCOO, sigma_lesser: DSDBCOO, sigma_greater: DSDBCOO,
+                       obc_blocks: OBCBlocks | None = None, out=None,
+                       return_retarded: bool = False, return_current: bool = False):
+        lib = _lib.load()
+        _lib.require_device_tensor(a.data, "a.data")
+        dev = a.device
+        n = a.num_blocks
+        if obc_blocks is None:
+            obc_blocks = OBCBlocks(num_blocks=n)
+        current = None
+        if return_current:
+            current = torch.zeros((*a.shape[:-2], n - 1), dtype=a.dtype, device=dev)
+        sizes, slices = get_batches(a.shape[0], self.max_batch_size)
+        if out is not None:
+            xl, xg, *xr = out
+            if return_retarded:
+                if len(xr) != 1:
+                    raise ValueError("Invalid number of output matrices.")
+                xr = xr[0]
+        else:
+            xl = a.__class__.zeros_like(a)
+            xg = a.__class__.zeros_like(a)
+            if return_retarded:
+                xr = a.__class__.zeros_like(a)
+        flags = 0
+        if return_retarded:
+            flags |= _lib.SOLVE_RETARDED
+            if self.retarded_offdiag:
+                flags |= _lib.SOLVE_RETARDED_OFFDIAG
+        if return_current:
+            flags |= _lib.SOLVE_CURRENT
+        if xl.symmetry:
+            flags |= _lib.SOLVE_SKIP_LOWER_L
+        if xg.symmetry:
+            flags |= _lib.SOLVE_SKIP_LOWER_G
+        h, st = _lib.handle(dev), _lib.stream_ptr(dev)
+        prec = _lib.PRECISIONS[self.precision]
+        inner = int(np.prod(a.stack_shape[1:], dtype=np.int64))
+        for b in range(len(sizes)):
+            s0, s1 = int(slices[b]), int(slices[b + 1])
+            keep: list = []
+            A = _Bound(a, s0, s1, output=False)
+            SL = _Bound(sigma_lesser, s0, s1, output=False)
+            SG = _Bound(sigma_greater, s0, s1, output=False)
+            XL = _Bound(xl, s0, s1, output=True, write_lower=not xl.symmetry)
+            XG = _Bound(xg, s0, s1, output=True, write_lower=not xg.symmetry)
+            XR = None
+            if return_retarded:
+                XR = _Bound(xr, s0, s1, output=True, write_lower=not xr.symmetry,
+                            write_offdiag=self.retarded_offdiag)
+            o_r = _obc_array(obc_blocks.retarded, s0, s1, n, dev, keep)
+            o_l = _obc_array(obc_blocks.lesser, s0, s1, n, dev, keep)
+            o_g = _obc_array(obc_blocks.greater, s0, s1, n, dev, keep)
+            cur_ptr = None
+            if current is not None:
+                cur_ptr = ctypes.c_void_p(current.data_ptr()
+                                          + s0 * inner * (n - 1) * current.element_size())
+            rc = lib.qx_rgf_selected_solve(
+                h, st, A.lay(), A.ptr(), SL.lay(), SL.ptr(), SG.lay(), SG.ptr(), o_r, o_l, o_g,
+                XL.lay(), XL.ptr(), XG.lay(), XG.ptr(),
+                XR.lay() if XR else None, XR.ptr() if XR else None, cur_ptr, flags, prec)
+            _lib.check(rc, h, "RGF.selected_solve")
+            for X in (XL, XG, XR):
+                if X is not None:
+                    X.write_back()
+        _lib.sync_check(dev, "RGF.selected_solve")
+        if out is None:
+            if return_retarded:
+                if return_current:
+                    return xl, xg, xr, current
+                return xl, xg, xr
+            return xl, xg
+        if return_current:
+            return current
+        return None

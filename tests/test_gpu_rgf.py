@@ -1,0 +1,211 @@
+"""RGF on the HIP path against the NumPy oracle (same seeded inputs).
+
+Cases follow the reference tests (tests/qttools/greens_function_solver/
+conftest.py:10-124: block sizes [2]*10 and mixed [2,2,2,4,4,2,2,2], stacks
+(10,), (7,2), (9,2,4), max_batch_size 1/2/100, out given or not,
+return_retarded) plus the benchmark workload (16 energies, 8 blocks of 32).
+Bar: complex128 arithmetic within 1e-10 relative Frobenius error per block
+(elementwise rtol 1e-10 on the reference patterns); complex64 arithmetic
+within 1e-4 per block.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sps
+import torch
+
+import oracle.rgf as orgf
+from oracle.dsdbcoo import DSDBCOO as ODSDBCOO
+from quatrex_amd.datastructures import DSDBCOO
+from quatrex_amd.greens_function_solver import RGF, OBCBlocks
+from quatrex_amd.workloads import make_workload
+from tests.conftest import bt_dense, rel_err
+
+pytestmark = pytest.mark.gpu
+
+BLOCK_SIZES = [np.array([2] * 10), np.array([2] * 3 + [4] * 2 + [2] * 3)]
+STACKS = [(10,), (7, 2), (9, 2, 4)]
+
+
+def to_gpu(o: ODSDBCOO, dev) -> DSDBCOO:
+    return DSDBCOO(torch.as_tensor(o.data, device=dev), o.rows, o.cols, o.block_sizes,
+                   o.global_stack_shape, symmetry=o.symmetry)
+
+
+def close(g: DSDBCOO, o: ODSDBCOO, rtol):
+    got = g.data.cpu().numpy()
+    ref = o.data
+    scale = np.max(np.abs(ref)) if ref.size else 1.0
+    return np.allclose(got, ref, rtol=rtol, atol=rtol * scale)
+
+
+@pytest.mark.parametrize("block_sizes", BLOCK_SIZES, ids=["const", "mixed"])
+@pytest.mark.parametrize("stack", STACKS, ids=["1D", "2D", "3D"])
+@pytest.mark.parametrize("max_batch_size", [1, 2, 100])
+@pytest.mark.parametrize("out", [True, False])
+def test_selected_inv(device, block_sizes, stack, max_batch_size, out):
+    dense = bt_dense(block_sizes)
+    oa = ODSDBCOO.from_sparray(sps.coo_matrix(dense), block_sizes, stack)
+    ref = orgf.RGF(max_batch_size).selected_inv(oa)
+    a = to_gpu(oa, device)
+    solver = RGF(max_batch_size=max_batch_size)
+    if out:
+        x = DSDBCOO.zeros_like(a)
+        assert solver.selected_inv(a, out=x) is None
+    else:
+        x = solver.selected_inv(a)
+    assert close(x, ref, 1e-10)
+
+
+@pytest.mark.parametrize("block_sizes", BLOCK_SIZES, ids=["const", "mixed"])
+@pytest.mark.parametrize("stack", STACKS, ids=["1D", "2D", "3D"])
+@pytest.mark.parametrize("max_batch_size", [1, 2, 100])
+@pytest.mark.parametrize("return_retarded", [True, False])
+@pytest.mark.parametrize("out", [True, False])
+def test_selected_solve(device, block_sizes, stack, max_batch_size, return_retarded, out):
+    dense = bt_dense(block_sizes)
+    coo = sps.coo_matrix(dense)
+    b = coo - coo.conj().T
+    oa = ODSDBCOO.from_sparray(coo, block_sizes, stack)
+    ob = ODSDBCOO.from_sparray(sps.coo_matrix(b), block_sizes, stack)
+    refs = orgf.RGF(max_batch_size).selected_solve(oa, ob, ob, return_retarded=True)
+    a, bl, bg = to_gpu(oa, device), to_gpu(ob, device), to_gpu(ob, device)
+    solver = RGF(max_batch_size=max_batch_size)
+    if out:
+        xs = [DSDBCOO.zeros_like(a) for _ in range(3)]
+        assert solver.selected_solve(a, bl, bg, out=xs, return_retarded=return_retarded) is None
+    else:
+        xs = solver.selected_solve(a, bl, bg, return_retarded=return_retarded)
+    assert close(xs[0], refs[0], 1e-10)
+    assert close(xs[1], refs[1], 1e-10)
+    if return_retarded:
+        assert close(xs[2], refs[2], 1e-10)
+
+
+def test_invalid_number_of_outputs(device):
+    dense = bt_dense(BLOCK_SIZES[0])
+    a = to_gpu(ODSDBCOO.from_sparray(sps.coo_matrix(dense), BLOCK_SIZES[0], (2,)), device)
+    with pytest.raises(ValueError, match="Invalid number of output matrices."):
+        RGF().selected_solve(a, a, a, out=[DSDBCOO.zeros_like(a), DSDBCOO.zeros_like(a)],
+                             return_retarded=True)
+
+
+def _oracle_containers(w):
+    from tests.test_oracle_rgf import _containers, _dense_workload
+
+    A, SL, SG = _dense_workload(w)
+    return _containers(w, A, SL, SG)
+
+
+@pytest.fixture(scope="module")
+def workload():
+    w = make_workload(seed=0)
+    oa, osl, osg = _oracle_containers(w)
+    ref = orgf.RGF(16).selected_solve(oa, osl, osg, return_retarded=True, return_current=True,
+                                      retarded_offdiag=True)
+    return w, (oa, osl, osg), ref
+
+
+def _per_block_worst(w, g, o):
+    worst = 0.0
+    for i in range(w.n_blocks):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < w.n_blocks:
+                gb = g.blocks[i, j].cpu().numpy()
+                worst = max(worst, rel_err(gb, o.blocks[i, j]))
+    return worst
+
+
+@pytest.mark.parametrize("max_batch_size", [16, 5])
+def test_workload_parity_c128(device, workload, max_batch_size):
+    w, (oa, osl, osg), (rl, rg, rr, rcur) = workload
+    a, sl, sg = to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device)
+    xl, xg, xr, cur = RGF(max_batch_size, retarded_offdiag=True).selected_solve(
+        a, sl, sg, return_retarded=True, return_current=True)
+    for g, o in ((xl, rl), (xg, rg), (xr, rr)):
+        assert _per_block_worst(w, g, o) < 1e-10
+    assert rel_err(cur.cpu().numpy(), rcur) < 1e-10
+
+
+def test_workload_device_assembly_matches(device, workload):
+    """Workload.assemble (device, dense-block layout) equals the oracle containers."""
+    w, (oa, osl, osg), _ = workload
+    a, sl, sg = w.assemble(device)
+    for g, o in ((a, oa), (sl, osl), (sg, osg)):
+        assert np.array_equal(g.rows, o.rows) and np.array_equal(g.cols, o.cols)
+        assert np.array_equal(g.data.cpu().numpy(), o.data)
+
+
+def test_workload_c64_accuracy(device, workload):
+    w, (oa, osl, osg), (rl, rg, rr, _) = workload
+    a, sl, sg = to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device)
+    xl, xg, xr = RGF(16, precision="c64").selected_solve(a, sl, sg, return_retarded=True)
+    for g, o in ((xl, rl), (xg, rg)):
+        assert _per_block_worst(w, g, o) < 1e-4
+    for i in range(w.n_blocks):
+        assert rel_err(xr.blocks[i, i].cpu().numpy(), rr.blocks[i, i]) < 1e-4
+
+
+def test_obc_blocks_path(device, workload):
+    w, (oa, osl, osg), _ = workload
+    n, b, ne = w.n_blocks, w.block_sizes[0], oa.data.shape[0]
+    rng = np.random.default_rng(7)
+    obc = orgf.OBCBlocks(n)
+    extra = 0.05 * (rng.standard_normal((ne, b, b)) + 1j * rng.standard_normal((ne, b, b)))
+    obc.retarded[0] = extra
+    obc.retarded[n - 1] = -1j * 0.01 * np.broadcast_to(np.eye(b), (ne, b, b))
+    obc.lesser[0] = 1j * 0.1 * np.broadcast_to(np.eye(b), (ne, b, b))
+    obc.greater[n - 1] = -1j * 0.1 * np.broadcast_to(np.eye(b), (ne, b, b))
+    ref = orgf.RGF(16).selected_solve(oa, osl, osg, obc_blocks=obc, return_retarded=True)
+    gobc = OBCBlocks(n)
+    for name in ("retarded", "lesser", "greater"):
+        for j, blk in enumerate(getattr(obc, name)):
+            if blk is not None:
+                getattr(gobc, name)[j] = torch.as_tensor(np.ascontiguousarray(blk), device=device)
+    a, sl, sg = to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device)
+    got = RGF(16).selected_solve(a, sl, sg, obc_blocks=gobc, return_retarded=True)
+    for g, o in zip(got, ref):
+        assert _per_block_worst(w, g, o) < 1e-10
+
+
+def test_selected_inv_workload_with_obc(device, workload):
+    w, (oa, _, _), _ = workload
+    n, b, ne = w.n_blocks, w.block_sizes[0], oa.data.shape[0]
+    obc = orgf.OBCBlocks(n)
+    obc.retarded[n - 1] = 0.02j * np.broadcast_to(np.eye(b), (ne, b, b)).copy()
+    ref = orgf.RGF(16).selected_inv(oa, obc_blocks=obc)
+    gobc = OBCBlocks(n)
+    gobc.retarded[n - 1] = torch.as_tensor(obc.retarded[n - 1], device=device)
+    x = RGF(16).selected_inv(to_gpu(oa, device), obc_blocks=gobc)
+    assert _per_block_worst(w, x, ref) < 1e-10
+
+
+def test_symmetric_output_containers(device):
+    """Symmetric-stored outputs: lower blocks are not written (rgf.py:351-352)."""
+    bs = BLOCK_SIZES[1]
+    dense = bt_dense(bs)
+    coo = sps.coo_matrix(dense)
+    b = coo - coo.conj().T
+    oa = ODSDBCOO.from_sparray(coo, bs, (3,))
+    ob = ODSDBCOO.from_sparray(sps.coo_matrix(b), bs, (3,))
+    osym = ODSDBCOO.from_sparray(sps.coo_matrix(b), bs, (3,), symmetry=True)
+    oxl, oxg = ODSDBCOO.zeros_like(osym), ODSDBCOO.zeros_like(osym)
+    orgf.RGF().selected_solve(oa, ob, ob, out=(oxl, oxg))
+    a, bl = to_gpu(oa, device), to_gpu(ob, device)
+    xl = DSDBCOO.zeros_like(to_gpu(osym, device))
+    xg = DSDBCOO.zeros_like(to_gpu(osym, device))
+    RGF().selected_solve(a, bl, bl, out=(xl, xg))
+    assert close(xl, oxl, 1e-10) and close(xg, oxg, 1e-10)
+
+
+def test_non_dense_blocks_staging(device):
+    """Blocks with partial sparsity go through staging; results match the oracle."""
+    bs = BLOCK_SIZES[0]
+    dense = bt_dense(bs, seed=4)
+    coo = sps.coo_matrix(dense)
+    oa = ODSDBCOO.from_sparray(coo, bs, (4,))
+    assert any(sl is not None and (sl.stop - sl.start) < 4
+               for sl in (oa.block_slice(i, i) for i in range(len(bs))))
+    ref = orgf.RGF().selected_inv(oa)
+    x = RGF().selected_inv(to_gpu(oa, device))
+    assert close(x, ref, 1e-10)
