@@ -76,6 +76,17 @@ const char* qx_last_error(const qx_handle* handle);
 int qx_workspace_query(qx_handle* handle, const qx_btd_layout* layout,
                        int precision, size_t* bytes);
 
+/* Kernel timing (no counterpart in the reference; replaces its NVTX/timer
+ * profiler, src/qttools/profiling/profiler.py:187-429, for the kernels of
+ * this path).  Between qx_timer_start and qx_timer_stop every GEMM and
+ * inverse launch made through `handle` is bracketed by HIP events on its
+ * stream.  qx_timer_stop waits for `stream` and writes host stats[6] =
+ * {gemm launches, gemm ms, gemm flops, inverse launches, inverse ms,
+ * inverse flops} (flops = algorithmic 8 m n k per complex product and
+ * 8 n^3 per complex inverse). */
+int qx_timer_start(qx_handle* handle);
+int qx_timer_stop(qx_handle* handle, void* stream, double* stats);
+
 /* Wait for `stream`, then report (and clear) device-side errors recorded
  * since the last check: QX_ESINGULAR if an inversion met a zero pivot. */
 int qx_sync_check(qx_handle* handle, void* stream);

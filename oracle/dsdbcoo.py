@@ -108,21 +108,42 @@ class DSDBCOO:
 
     @classmethod
     def zeros_like(cls, other: "DSDBCOO") -> "DSDBCOO":
-        """dsdbcoo.py:837-870."""
-        return cls(np.zeros_like(other.data), other.rows, other.cols,
-                   other.block_sizes, other.global_stack_shape, other.symmetry,
-                   other.symmetry_op)
+        """dsdbcoo.py:837-870 (the block-slice cache is shared: same pattern)."""
+        out = cls(np.zeros_like(other.data), other.rows, other.cols,
+                  other.block_sizes, other.global_stack_shape, other.symmetry,
+                  other.symmetry_op)
+        out._slices = other._slices
+        return out
 
     # -- block access --------------------------------------------------
+    def _block_keys(self):
+        if "_keys" not in self._slices:
+            o = self.block_offsets
+            brow = np.searchsorted(o, self.rows, side="right") - 1
+            bcol = np.searchsorted(o, self.cols, side="right") - 1
+            keys = brow.astype(np.int64) * self.num_blocks + bcol
+            self._slices["_keys"] = keys if np.all(keys[1:] >= keys[:-1]) else None
+        return self._slices["_keys"]
+
     def block_slice(self, i: int, j: int):
-        """Contiguous nnz slice of block (i, j) or ``None`` when empty (dsdbcoo.py:285-319)."""
+        """Contiguous nnz slice of block (i, j) or ``None`` when empty (dsdbcoo.py:285-319).
+
+        Entries are block-sorted, so the slice is found by bisection on the
+        (block-row, block-col) keys; unsorted input falls back to a mask.
+        """
         key = (i, j)
         if key not in self._slices:
-            o = self.block_offsets
-            m = ((self.rows >= o[i]) & (self.rows < o[i + 1])
-                 & (self.cols >= o[j]) & (self.cols < o[j + 1]))
-            idx = np.nonzero(m)[0]
-            self._slices[key] = None if idx.size == 0 else slice(int(idx[0]), int(idx[-1]) + 1)
+            keys = self._block_keys()
+            if keys is not None:
+                k = i * self.num_blocks + j
+                lo, hi = np.searchsorted(keys, k, "left"), np.searchsorted(keys, k, "right")
+                self._slices[key] = None if hi == lo else slice(int(lo), int(hi))
+            else:
+                o = self.block_offsets
+                m = ((self.rows >= o[i]) & (self.rows < o[i + 1])
+                     & (self.cols >= o[j]) & (self.cols < o[j + 1]))
+                idx = np.nonzero(m)[0]
+                self._slices[key] = None if idx.size == 0 else slice(int(idx[0]), int(idx[-1]) + 1)
         return self._slices[key]
 
     def _get_block(self, stack_index, i, j):

@@ -40,7 +40,7 @@ SOLVE_SKIP_LOWER_G = 16
 EXPORTED_SYMBOLS = (
     "qx_abi_version", "qx_create", "qx_destroy", "qx_last_error", "qx_workspace_query",
     "qx_sync_check", "qx_rgf_selected_inv", "qx_rgf_selected_solve", "qx_zgemm_batched",
-    "qx_inv_batched", "qx_sancho_rubio", "qx_beyn_moments",
+    "qx_inv_batched", "qx_sancho_rubio", "qx_beyn_moments", "qx_timer_start", "qx_timer_stop",
 )
 
 
@@ -110,6 +110,8 @@ def load() -> ctypes.CDLL:
         lib.qx_sancho_rubio.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, i32, f64,
                                         ctypes.POINTER(ctypes.c_int)]
         lib.qx_beyn_moments.argtypes = [vp, vp, i64, i32, i32, f64, f64, vp, vp, vp, vp, vp, i32]
+        lib.qx_timer_start.argtypes = [vp]
+        lib.qx_timer_stop.argtypes = [vp, vp, ctypes.POINTER(f64)]
         for name in EXPORTED_SYMBOLS:
             getattr(lib, name).restype = getattr(lib, name).restype or i32
         if lib.qx_abi_version() != 1:
@@ -162,6 +164,33 @@ def sync_check(device: torch.device, what: str) -> None:
     """Wait for the device stream and raise on recorded device errors."""
     h = handle(device)
     check(load().qx_sync_check(h, stream_ptr(device)), h, what)
+
+
+class KernelTimer:
+    """Context manager timing the library's GEMM / inverse launches with HIP events.
+
+    ``stats`` after exit: dict with launches, milliseconds and algorithmic
+    flops per kernel family ("gemm", "inv").
+    """
+
+    def __init__(self, device):
+        self.device = torch.device(device)
+        self.stats = None
+
+    def __enter__(self):
+        h = handle(self.device)
+        check(load().qx_timer_start(h), h, "timer_start")
+        return self
+
+    def __exit__(self, *exc):
+        h = handle(self.device)
+        out = (ctypes.c_double * 6)()
+        check(load().qx_timer_stop(h, stream_ptr(self.device), out), h, "timer_stop")
+        self.stats = {
+            "gemm": {"launches": int(out[0]), "ms": out[1], "flops": out[2]},
+            "inv": {"launches": int(out[3]), "ms": out[4], "flops": out[5]},
+        }
+        return False
 
 
 def warn_not_converged(message: str) -> None:

@@ -134,6 +134,33 @@ int beyn_moments(qx_handle* h, hipStream_t st, int64_t n, int b, int nq, double 
 }
 
 }  // namespace
+
+thread_local KernelTimer* active_timer = nullptr;
+
+hipEvent_t KernelTimer::get() {
+  if (next == pool.size()) {
+    hipEvent_t e;
+    QX_HIP_CHECK(hipEventCreate(&e));
+    pool.push_back(e);
+  }
+  return pool[next++];
+}
+
+void KernelTimer::begin(hipStream_t s, hipEvent_t* ev) {
+  *ev = get();
+  QX_HIP_CHECK(hipEventRecord(*ev, s));
+}
+
+void KernelTimer::end(hipStream_t s, hipEvent_t a, int kind, double flops) {
+  hipEvent_t b = get();
+  QX_HIP_CHECK(hipEventRecord(b, s));
+  recs.push_back(Rec{a, b, kind, flops});
+}
+
+KernelTimer::~KernelTimer() {
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
 }  // namespace qx
 
 void* qx_handle::ensure_ws(size_t bytes) {
@@ -179,9 +206,15 @@ int* qx_handle::ensure_info(size_t n) {
 
 namespace {
 
+struct TimerScope {
+  explicit TimerScope(qx_handle* h) { qx::active_timer = h ? h->timer : nullptr; }
+  ~TimerScope() { qx::active_timer = nullptr; }
+};
+
 template <typename F>
 int guarded(qx_handle* h, F&& f) {
   try {
+    TimerScope scope(h);
     if (h) {
       int cur = -1;
       if (hipGetDevice(&cur) == hipSuccess && cur != h->device) QX_HIP_CHECK(hipSetDevice(h->device));
@@ -213,10 +246,41 @@ int qx_create(qx_handle** handle, int device) {
   return QX_OK;
 }
 
+int qx_timer_start(qx_handle* h) {
+  return guarded(h, [&]() {
+    QX_REQUIRE(h, "null handle");
+    if (!h->timer) h->timer = new qx::KernelTimer();
+    h->timer->recs.clear();
+    h->timer->next = 0;
+    return QX_OK;
+  });
+}
+
+int qx_timer_stop(qx_handle* h, void* stream, double* stats) {
+  return guarded(h, [&]() {
+    QX_REQUIRE(h && stats, "timer_stop: null argument");
+    for (int i = 0; i < 3 * qx::KernelTimer::NKIND; ++i) stats[i] = 0.0;
+    if (!h->timer) return QX_OK;
+    QX_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    for (const auto& r : h->timer->recs) {
+      float ms = 0.f;
+      QX_HIP_CHECK(hipEventSynchronize(r.b));
+      QX_HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      stats[3 * r.kind + 0] += 1.0;
+      stats[3 * r.kind + 1] += ms;
+      stats[3 * r.kind + 2] += r.flops;
+    }
+    delete h->timer;
+    h->timer = nullptr;
+    return QX_OK;
+  });
+}
+
 int qx_destroy(qx_handle* h) {
   if (!h) return QX_OK;
   (void)hipSetDevice(h->device);
   (void)hipDeviceSynchronize();
+  delete h->timer;
   if (h->ws) (void)hipFree(h->ws);
   if (h->info) (void)hipFree(h->info);
   delete h;

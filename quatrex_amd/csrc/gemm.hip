@@ -238,6 +238,9 @@ void launch_typed(hipStream_t s, int64_t batch, int opA, int opB,
   const int64_t blocks = batch * g.tilesM * g.tilesN;
   QX_REQUIRE(blocks < (int64_t(1) << 31), "zgemm: grid too large");
   dim3 grid((unsigned)blocks), block(NT);
+  KernelTimer* timer = active_timer;
+  hipEvent_t ev0 = nullptr;
+  if (timer) timer->begin(s, &ev0);
   if (opA == QX_OP_N && opB == QX_OP_N)
     hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_N, QX_OP_N>), grid, block, 0, s, g);
   else if (opA == QX_OP_N && opB == QX_OP_C)
@@ -247,6 +250,7 @@ void launch_typed(hipStream_t s, int64_t batch, int opA, int opB,
   else
     hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_C, QX_OP_C>), grid, block, 0, s, g);
   QX_HIP_CHECK(hipGetLastError());
+  if (timer) timer->end(s, ev0, KernelTimer::GEMM, 8.0 * g.m * g.n * (double)g.k * batch);
 }
 
 }  // namespace

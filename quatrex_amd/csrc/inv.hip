@@ -327,6 +327,9 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S,
   InvArgs g{n, A.p, A.stride, A.ld, S.p, S.stride, S.ld, X.p, X.stride, X.ld, info};
   const bool f32 = precision == QX_PREC_C64;
   const size_t sm = smem_bytes(n, f32 ? 4 : 8);
+  KernelTimer* timer = active_timer;
+  hipEvent_t ev0 = nullptr;
+  if (timer) timer->begin(s, &ev0);
   if (f32) {
     static bool attr = false;
     if (!attr) {
@@ -345,6 +348,7 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S,
     hipLaunchKernelGGL(zinv_kernel<double>, dim3((unsigned)batch), dim3(NT), sm, s, g);
   }
   QX_HIP_CHECK(hipGetLastError());
+  if (timer) timer->end(s, ev0, KernelTimer::INV, 8.0 * (double)n * n * n * batch);
 }
 
 }  // namespace qx

@@ -92,6 +92,26 @@ void contour_accumulate(hipStream_t s, int64_t batch, int nq, int b,
                         const Mat& Tinv, const z2* w0, const z2* w1,
                         const Mat& q0, const Mat& q1);
 
+// Opt-in kernel timing: while a handle's profiler is active, the GEMM and
+// inverse launchers bracket every launch with HIP events on the launch
+// stream and record the algorithmic flops of the launch.
+struct KernelTimer {
+  enum Kind { GEMM = 0, INV = 1, NKIND = 2 };
+  struct Rec {
+    hipEvent_t a, b;
+    int kind;
+    double flops;
+  };
+  std::vector<hipEvent_t> pool;
+  std::vector<Rec> recs;
+  size_t next = 0;
+  hipEvent_t get();
+  void begin(hipStream_t s, hipEvent_t* ev);
+  void end(hipStream_t s, hipEvent_t a, int kind, double flops);
+  ~KernelTimer();
+};
+extern thread_local KernelTimer* active_timer;
+
 }  // namespace qx
 
 // Handle: device, workspace and sticky error message.
@@ -102,6 +122,7 @@ struct qx_handle {
   int* info = nullptr;  // device error flags
   size_t info_len = 0;
   std::string last_error;
+  qx::KernelTimer* timer = nullptr;
   void* ensure_ws(size_t bytes);
   int* ensure_info(size_t n);
 };

@@ -209,14 +209,25 @@ class DSDBCOO:
 
     # -- block metadata ----------------------------------------------------
     def block_slice(self, i: int, j: int):
-        """Contiguous nnz slice of block (i, j), or None when it stores nothing."""
+        """Contiguous nnz slice of block (i, j), or None when it stores nothing.
+
+        Entries are block-sorted, so the slice is a bisection on the
+        (block-row, block-col) key of every entry (computed once per pattern).
+        """
         key = (i, j)
         if key not in self._slices:
-            o = self._block_offsets
-            m = ((self.rows >= o[i]) & (self.rows < o[i + 1])
-                 & (self.cols >= o[j]) & (self.cols < o[j + 1]))
-            idx = np.nonzero(m)[0]
-            self._slices[key] = None if idx.size == 0 else slice(int(idx[0]), int(idx[-1]) + 1)
+            if "_keys" not in self._slices:
+                o = self._block_offsets
+                brow = np.searchsorted(o, self.rows, side="right") - 1
+                bcol = np.searchsorted(o, self.cols, side="right") - 1
+                keys = brow.astype(np.int64) * self.num_blocks + bcol
+                if keys.size and not np.all(keys[1:] >= keys[:-1]):
+                    raise ValueError("DSDBCOO entries must be sorted by (block-row, block-col)")
+                self._slices["_keys"] = keys
+            keys = self._slices["_keys"]
+            k = i * self.num_blocks + j
+            lo, hi = np.searchsorted(keys, k, "left"), np.searchsorted(keys, k, "right")
+            self._slices[key] = None if hi == lo else slice(int(lo), int(hi))
         return self._slices[key]
 
     def is_dense_block(self, i: int, j: int) -> bool:
