@@ -2,7 +2,10 @@
 // open-boundary drivers (Sancho-Rubio decimation, Beyn contour moments).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <thread>
 
 #include "qx_internal.h"
 
@@ -39,9 +42,11 @@ int sancho_rubio(qx_handle* h, hipStream_t st, int64_t n, int b, const void* a_i
   if (n == 0) return QX_OK;
   const int64_t bb = (int64_t)b * b;
   const size_t slot = (size_t)(n * bb) * sizeof(z2);
-  z2* ws = static_cast<z2*>(h->ensure_ws(slot * 10 + (size_t)n * sizeof(double) * 2));
+  // 10 matrix slots, inverse scratch (buffer + pivot ints), norms
+  z2* ws = static_cast<z2*>(h->ensure_ws(slot * 12 + (size_t)n * sizeof(double) * 2));
   auto M = [&](int k) { return Mat(ws + (int64_t)k * n * bb, bb, b); };
-  double* dnorm = reinterpret_cast<double*>(ws + 10 * n * bb);
+  const InvScratch scr{ws + 10 * n * bb, bb, reinterpret_cast<int*>(ws + 11 * n * bb), 4 * bb};
+  double* dnorm = reinterpret_cast<double*>(ws + 12 * n * bb);
   int* info = h->ensure_info(n);
   const Mat Aii(a_ii, bb, b), Aij(a_ij, bb, b), Aji(a_ji, bb, b), X(x_ii, bb, b);
   int eps = 0, eps_s = 1, alpha = 2, beta = 3, g = 4, t1 = 5, t2 = 6, p = 7, na = 8, nb = 9;
@@ -54,7 +59,7 @@ int sancho_rubio(qx_handle* h, hipStream_t st, int64_t n, int b, const void* a_i
   const int prec = QX_PREC_C128;
   for (int it = 0; it < max_iter; ++it) {
     if (iters) *iters = it + 1;
-    zinv(st, n, b, M(eps), Mat(), M(g), info, prec);
+    zinv(st, n, b, M(eps), Mat(), M(g), info, prec, scr);
     zgemm(st, n, b, b, b, QX_OP_N, QX_OP_N, kOne, M(alpha), M(g), kZero, M(t1), prec);  // a g
     zgemm(st, n, b, b, b, QX_OP_N, QX_OP_N, kOne, M(beta), M(g), kZero, M(t2), prec);   // b g
     zgemm(st, n, b, b, b, QX_OP_N, QX_OP_N, kOne, M(t1), M(beta), kZero, M(p), prec);   // a g b
@@ -75,7 +80,7 @@ int sancho_rubio(qx_handle* h, hipStream_t st, int64_t n, int b, const void* a_i
       break;
     }
   }
-  zinv(st, n, b, M(eps_s), Mat(), X, info, prec);
+  zinv(st, n, b, M(eps_s), Mat(), X, info, prec, scr);
   return converged ? QX_OK : QX_NOT_CONVERGED;
 }
 
@@ -101,8 +106,9 @@ int beyn_moments(qx_handle* h, hipStream_t st, int64_t n, int b, int nq, double 
     w0i[k] = make_double2(-w * zi[k].x, -w * zi[k].y);
     w1i[k] = make_double2(-w * (zi[k].x * zi[k].x - zi[k].y * zi[k].y), -w * 2 * zi[k].x * zi[k].y);
   }
-  // energies per chunk: operator stack of chunk * nq * b^2 (two of them)
-  const int64_t per_e = 2 * nq * bb;
+  // energies per chunk: operator, inverse and inverse scratch stacks of
+  // chunk * nq matrices each, plus the pivot ints
+  const int64_t per_e = 4 * nq * bb;
   int64_t chunk = std::max<int64_t>(1, (int64_t(1) << 28) / per_e);  // <= 4 GiB
   chunk = std::min(chunk, n);
   const size_t dev_aux = 6 * nq * sizeof(z2);
@@ -110,6 +116,8 @@ int beyn_moments(qx_handle* h, hipStream_t st, int64_t n, int b, int nq, double 
   z2* dz = ws;
   z2* T = ws + 6 * nq;
   z2* Ti = T + chunk * nq * bb;
+  const InvScratch scr{Ti + chunk * nq * bb, bb, reinterpret_cast<int*>(Ti + 2 * chunk * nq * bb),
+                       4 * bb};
   QX_HIP_CHECK(hipMemcpyAsync(dz, host.data(), dev_aux, hipMemcpyHostToDevice, st));
   int* info = h->ensure_info(chunk * nq);
   const Mat Q0(q0, bb, b), Q1(q1, bb, b);
@@ -124,7 +132,7 @@ int beyn_moments(qx_handle* h, hipStream_t st, int64_t n, int b, int nq, double 
     for (int side = 0; side < 2; ++side) {
       const z2* z = dz + side * nq;
       contour_operator(st, ne, nq, b, Aji, Aii, Aij, z, Mat(T, bb, b));
-      zinv(st, ne * nq, b, Mat(T, bb, b), Mat(), Mat(Ti, bb, b), info, prec);
+      zinv(st, ne * nq, b, Mat(T, bb, b), Mat(), Mat(Ti, bb, b), info, prec, scr);
       const z2* w0 = dz + (side == 0 ? 2 : 4) * nq;
       const z2* w1 = dz + (side == 0 ? 3 : 5) * nq;
       contour_accumulate(st, ne, nq, b, Mat(Ti, bb, b), w0, w1, q0c, q1c);
@@ -137,6 +145,55 @@ int beyn_moments(qx_handle* h, hipStream_t st, int64_t n, int b, int nq, double 
 
 thread_local KernelTimer* active_timer = nullptr;
 
+void run_split(qx_handle* h, hipStream_t st, int64_t S,
+               const std::function<void(hipStream_t, int64_t, int64_t)>& f) {
+  int P = (int)std::min<int64_t>(h->parts, std::max<int64_t>(1, S / 4));
+  if (P <= 1) {
+    f(st, 0, S);
+    return;
+  }
+  while ((int)h->streams.size() < P - 1) {
+    hipStream_t s2;
+    hipEvent_t ev;
+    QX_HIP_CHECK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+    QX_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    h->streams.push_back(s2);
+    h->join.push_back(ev);
+  }
+  if (!h->fork) QX_HIP_CHECK(hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
+  QX_HIP_CHECK(hipEventRecord(h->fork, st));
+  for (int k = 1; k < P; ++k) QX_HIP_CHECK(hipStreamWaitEvent(h->streams[k - 1], h->fork, 0));
+  std::vector<int64_t> e0(P + 1, 0);
+  for (int k = 0; k < P; ++k) e0[k + 1] = e0[k] + S / P + (k < S % P ? 1 : 0);
+  std::vector<std::exception_ptr> err(P);
+  KernelTimer* timer = active_timer;
+  std::vector<std::thread> threads;
+  for (int k = 1; k < P; ++k) {
+    threads.emplace_back([&, k]() {
+      try {
+        active_timer = timer;
+        QX_HIP_CHECK(hipSetDevice(h->device));
+        f(h->streams[k - 1], e0[k], e0[k + 1] - e0[k]);
+      } catch (...) {
+        err[k] = std::current_exception();
+      }
+      active_timer = nullptr;
+    });
+  }
+  try {
+    f(st, 0, e0[1]);
+  } catch (...) {
+    err[0] = std::current_exception();
+  }
+  for (auto& t : threads) t.join();
+  for (int k = 1; k < P; ++k) {
+    QX_HIP_CHECK(hipEventRecord(h->join[k - 1], h->streams[k - 1]));
+    QX_HIP_CHECK(hipStreamWaitEvent(st, h->join[k - 1], 0));
+  }
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
+
 hipEvent_t KernelTimer::get() {
   if (next == pool.size()) {
     hipEvent_t e;
@@ -147,11 +204,13 @@ hipEvent_t KernelTimer::get() {
 }
 
 void KernelTimer::begin(hipStream_t s, hipEvent_t* ev) {
+  std::lock_guard<std::mutex> lock(mu);
   *ev = get();
   QX_HIP_CHECK(hipEventRecord(*ev, s));
 }
 
 void KernelTimer::end(hipStream_t s, hipEvent_t a, int kind, double flops) {
+  std::lock_guard<std::mutex> lock(mu);
   hipEvent_t b = get();
   QX_HIP_CHECK(hipEventRecord(b, s));
   recs.push_back(Rec{a, b, kind, flops});
@@ -242,6 +301,10 @@ int qx_create(qx_handle** handle, int device) {
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return QX_EHIP;
   qx_handle* h = new qx_handle();
   h->device = device;
+  if (const char* env = std::getenv("QX_RGF_STREAMS")) {
+    const int p = std::atoi(env);
+    if (p >= 1 && p <= 8) h->parts = p;
+  }
   *handle = h;
   return QX_OK;
 }
@@ -281,6 +344,9 @@ int qx_destroy(qx_handle* h) {
   (void)hipSetDevice(h->device);
   (void)hipDeviceSynchronize();
   delete h->timer;
+  for (hipStream_t s2 : h->streams) (void)hipStreamDestroy(s2);
+  for (hipEvent_t ev : h->join) (void)hipEventDestroy(ev);
+  if (h->fork) (void)hipEventDestroy(h->fork);
   if (h->ws) (void)hipFree(h->ws);
   if (h->info) (void)hipFree(h->info);
   delete h;
@@ -360,9 +426,14 @@ int qx_inv_batched(qx_handle* h, void* stream, int64_t batch, int n, const void*
                    int64_t stride_x, int precision) {
   return guarded(h, [&]() {
     QX_REQUIRE(h, "null handle");
+    QX_REQUIRE(batch >= 0 && n >= 1, "inv: bad sizes");
     int* info = h->ensure_info(batch);
+    const int64_t nn = (int64_t)n * n;
+    qx::z2* ws = static_cast<qx::z2*>(h->ensure_ws((size_t)(batch * nn) * sizeof(qx::z2) +
+                                                   (size_t)(batch * 3 * n) * sizeof(int)));
+    const qx::InvScratch scr{ws, nn, reinterpret_cast<int*>(ws + batch * nn), 3 * (int64_t)n};
     qx::zinv(static_cast<hipStream_t>(stream), batch, n, qx::Mat(a, stride_a, lda),
-             qx::Mat(s, stride_s, lds), qx::Mat(x, stride_x, ldx), info, precision);
+             qx::Mat(s, stride_s, lds), qx::Mat(x, stride_x, ldx), info, precision, scr);
     return QX_OK;
   });
 }

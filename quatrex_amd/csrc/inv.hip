@@ -3,50 +3,42 @@
 // Replaces the batched `linalg.inv` of the reference
 // (src/qttools/kernels/linalg/inv.py:24-40, LAPACK/cuSOLVER getrf+getri).
 //
-// Algorithm: blocked in-place Gauss-Jordan elimination, one workgroup per
-// matrix, the matrix itself resident in HBM / L2:
-//   for each panel of NB = 16 columns
-//     1. the n x NB panel is copied to LDS and reduced there with partial
-//        pivoting (pivot = first row of maximal |re| + |im|, as LAPACK's
-//        izamax); every step scales the pivot row and eliminates the pivot
-//        column from all other rows;
-//     2. the panel's row interchanges are applied to the other columns;
-//     3. the other columns receive the rank-NB update
-//          X[:, C] <- X[:, C] (pivot rows zeroed) + P * X[pivot rows, C]
-//        on the f64 (or f32) matrix cores, 16 x 16 tiles per wave;
-//     4. the reduced panel is written back.
-//   Finally the column interchanges are undone in reverse order.
-// The flop count equals getrf + getri (8 n^3 real flops for complex).
+// Algorithm: blocked in-place Gauss-Jordan elimination split into kernels so
+// that the O(n^3) part runs on every CU of the chip while the sequential
+// pivoting chain stays short:
+//   for each panel of NB columns (NB = 32 for n <= 256, 16 up to n = 512)
+//     panel kernel  (one workgroup per matrix): each thread holds one or two
+//        rows of the n x NB panel in registers; per column the pivot (first
+//        row of maximal |re| + |im| among the unreduced rows, compared through
+//        a 32-bit key: float magnitude with 9 low mantissa bits replaced by the
+//        row index) is found with wave shuffles plus one LDS exchange, the
+//        pivot row is scaled and broadcast through LDS, and every other row
+//        eliminates the pivot column -- two barriers per column.  The reduced
+//        panel, the pivot rows and the composed row gather sig are written out.
+//     update kernel (64 x 64 tiles of every matrix, MFMA): the other columns
+//        receive the row interchanges and the rank-NB update in one pass,
+//          Xn[r, C] = (r in pivot rows ? 0 : Xc[sig r, C]) + P[r, :] Xc[sig K, C],
+//        reading buffer Xc and writing buffer Xn (ping-pong), so the gather
+//        is race free across workgroups.
+//   A final gather undoes the column interchanges (reverse order of the
+//   pivots) into the output.  The flop count equals getrf + getri
+//   (8 n^3 real flops per complex matrix).
 //
-// Precision: PREC_C128 computes in double; PREC_C64 rounds the panel and the
-// update operands to float (the update runs on f32 MFMA) while storing the
-// result back in complex128.
+// Precision: PREC_C128 computes in double; PREC_C64 holds the panel in
+// float registers and runs the update on f32 MFMA; storage stays complex128.
+//
+// MFMA 16x16x4 layouts as in gemm.hip: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; result element e of lane l is D[l/16 + 4e][l%16] (f64) or
+// D[4(l/16) + e][l%16] (f32).
 #include "qx_internal.h"
 
 namespace qx {
 namespace {
 
-constexpr int NB = 16;
 constexpr int NT = 256;
-constexpr int PST = NB + 1;  // LDS row stride of the panel (bank spread)
-constexpr int XST = NB + 1;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
-
-struct InvArgs {
-  int n;
-  const z2* A;
-  int64_t sA;
-  int lda;
-  const z2* S;
-  int64_t sS;
-  int lds;
-  z2* X;
-  int64_t sX;
-  int ldx;
-  int* info;
-};
 
 template <typename T>
 struct InvMma;
@@ -67,8 +59,8 @@ struct InvMma<float> {
   static __device__ inline int row(int lane, int e) { return 4 * (lane >> 4) + e; }
 };
 
-__device__ inline void cdiv_one(double ar, double ai, double& qr, double& qi) {
-  // 1 / (ar + i ai) with scaling against overflow (Smith's method).
+__device__ inline void crecip(double ar, double ai, double& qr, double& qi) {
+  // 1 / (ar + i ai), Smith's scaling against overflow.
   if (fabs(ar) >= fabs(ai)) {
     const double r = ai / ar, d = ar + ai * r;
     qr = 1.0 / d;
@@ -80,272 +72,402 @@ __device__ inline void cdiv_one(double ar, double ai, double& qr, double& qi) {
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(NT) void zinv_kernel(InvArgs g) {
-  typedef InvMma<T> MM;
-  typedef typename MM::acc_t acc_t;
-  extern __shared__ double smem[];
-  const int n = g.n;
-  T* Pr = reinterpret_cast<T*>(smem);
-  T* Pi = Pr + n * PST;
-  T* Xr = Pi + n * PST;
-  T* Xi = Xr + NB * XST;
-  int* perm = reinterpret_cast<int*>(Xi + NB * XST);
-  double* red_v = reinterpret_cast<double*>(perm + ((n + 1) & ~1));
-  int* red_i = reinterpret_cast<int*>(red_v + NT / 64);
+// Pivot key: larger key = larger magnitude, ties -> smaller row (n <= 512).
+__device__ inline unsigned pivot_key(double re, double im, int r) {
+  const float m = (float)(fabs(re) + fabs(im));
+  return (__float_as_uint(m) & 0xFFFFFE00u) | (unsigned)(511 - r);
+}
 
+__device__ inline unsigned wave_max(unsigned v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+struct PanelArgs {
+  int n, k0, nbw, last;
+  const z2* Xc;
+  int64_t sXc;
+  int ldc;
+  z2* Xn;
+  int64_t sXn;
+  int ldn;
+  int* ints;  // per matrix: perm[n], sig[n], idx[n]
+  int64_t sI;
+  int* info;
+};
+
+// Each thread keeps its row(s) of the panel in registers, rotated so that
+// the column being eliminated is always element 0: at step kk the row holds
+// columns (kk, kk+1, ..., NB-1, 0, ..., kk-1).  All register indices are then
+// compile-time constants; a rotation by one element closes every step.
+template <typename T, int NB, int RPT>
+__global__ __launch_bounds__(NT) void inv_panel(PanelArgs g) {
+  __shared__ T prow_r[NB], prow_i[NB], krow_r[NB], krow_i[NB];
+  __shared__ unsigned red[NT / 64];
+  __shared__ int perm_l[kInvMaxN];
+  __shared__ int sig_l[kInvMaxN];
   const int64_t e = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  z2* __restrict__ X = g.X + e * g.sX;
-  const int ldx = g.ldx;
+  const int n = g.n, k0 = g.k0, nbw = g.nbw;
+  const z2* Xc = g.Xc + e * g.sXc;
+  z2* Xn = g.Xn + e * g.sXn;
+  int* perm = g.ints + e * g.sI;
+  int* sig = perm + n;
+  int* idx = sig + n;
 
-  // 0. X = A - S
-  {
-    const z2* A = g.A + e * g.sA;
-    const z2* S = g.S ? g.S + e * g.sS : nullptr;
-    const bool same = (A == X) && S == nullptr && g.lda == ldx;
-    if (!same) {
-      for (int idx = tid; idx < n * n; idx += NT) {
-        const int r = idx / n, c = idx - r * n;
-        z2 v = A[(int64_t)r * g.lda + c];
-        if (S) {
-          const z2 sv = S[(int64_t)r * g.lds + c];
-          v.x -= sv.x;
-          v.y -= sv.y;
-        }
-        X[(int64_t)r * ldx + c] = v;
-      }
+  T rr[RPT][NB], ri[RPT][NB];
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) {
+    const int r = tid + NT * j;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      z2 v = make_double2(0.0, 0.0);
+      if (r < n && c < nbw) v = Xc[(int64_t)r * g.ldc + k0 + c];
+      rr[j][c] = (T)v.x;
+      ri[j][c] = (T)v.y;
     }
   }
+  // pivot candidates of the first column
+  unsigned key = 0;
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) {
+    const int r = tid + NT * j;
+    if (r >= k0 && r < n) {
+      const unsigned kj = pivot_key((double)rr[j][0], (double)ri[j][0], r);
+      key = kj > key ? kj : key;
+    }
+  }
+  key = wave_max(key);
+  if (lane == 0) red[w] = key;
   __syncthreads();
 
-  for (int k0 = 0; k0 < n; k0 += NB) {
-    const int nbw = min(NB, n - k0);
-    // 1a. panel -> LDS
-    for (int idx = tid; idx < n * NB; idx += NT) {
-      const int r = idx / NB, c = idx - r * NB;
-      z2 v = make_double2(0.0, 0.0);
-      if (c < nbw) v = X[(int64_t)r * ldx + k0 + c];
-      Pr[r * PST + c] = (T)v.x;
-      Pi[r * PST + c] = (T)v.y;
+  for (int kk = 0; kk < nbw; ++kk) {
+    const int k = k0 + kk;
+    unsigned kmax = red[0];
+#pragma unroll
+    for (int q = 1; q < NT / 64; ++q) kmax = red[q] > kmax ? red[q] : kmax;
+    int p = 511 - (int)(kmax & 0x1FFu);
+    if (p < k || p >= n) p = k;
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int r = tid + NT * j;
+      if (r == p) {
+        double vr, vi;
+        crecip((double)rr[j][0], (double)ri[j][0], vr, vi);
+        prow_r[0] = (T)vr;
+        prow_i[0] = (T)vi;
+#pragma unroll
+        for (int c = 1; c < NB; ++c) {
+          const double a = (double)rr[j][c], b = (double)ri[j][c];
+          prow_r[c] = (T)(a * vr - b * vi);
+          prow_i[c] = (T)(a * vi + b * vr);
+        }
+      }
+      if (r == k && p != k) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          krow_r[c] = rr[j][c];
+          krow_i[c] = ri[j][c];
+        }
+      }
+    }
+    if (tid == 0) {
+      perm_l[kk] = p;
+      if ((kmax >> 9) == 0u) g.info[e] = 1;  // no nonzero pivot: singular
     }
     __syncthreads();
 
-    // 1b. unblocked Gauss-Jordan on the panel
-    for (int kk = 0; kk < nbw; ++kk) {
-      const int k = k0 + kk;
-      double best = -1.0;
-      int bi = n;
-      for (int r = k + tid; r < n; r += NT) {
-        const double v = fabs((double)Pr[r * PST + kk]) + fabs((double)Pi[r * PST + kk]);
-        if (v > best) {
-          best = v;
-          bi = r;
-        }
-      }
+    key = 0;
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(best, off, 64);
-        const int oi = __shfl_xor(bi, off, 64);
-        if (ov > best || (ov == best && oi < bi)) {
-          best = ov;
-          bi = oi;
-        }
-      }
-      if (lane == 0) {
-        red_v[w] = best;
-        red_i[w] = bi;
-      }
-      __syncthreads();
-      best = red_v[0];
-      bi = red_i[0];
-#pragma unroll
-      for (int q = 1; q < NT / 64; ++q) {
-        const double ov = red_v[q];
-        const int oi = red_i[q];
-        if (ov > best || (ov == best && oi < bi)) {
-          best = ov;
-          bi = oi;
-        }
-      }
-      const int p = (bi < n) ? bi : k;
-      if (tid == 0) {
-        perm[k] = p;
-        if (!(best > 0.0)) g.info[e] = 1;  // plain vector store, idempotent
-      }
-      if (p != k && tid < NB) {
-        const T tr = Pr[k * PST + tid], ti = Pi[k * PST + tid];
-        Pr[k * PST + tid] = Pr[p * PST + tid];
-        Pi[k * PST + tid] = Pi[p * PST + tid];
-        Pr[p * PST + tid] = tr;
-        Pi[p * PST + tid] = ti;
-      }
-      __syncthreads();
-
-      // pivot row, scaled, in registers of every thread
-      double rkr[NB], rki[NB];
-#pragma unroll
-      for (int c = 0; c < NB; ++c) {
-        rkr[c] = (double)Pr[k * PST + c];
-        rki[c] = (double)Pi[k * PST + c];
-      }
-      double ivr, ivi;
-      cdiv_one((double)Pr[k * PST + kk], (double)Pi[k * PST + kk], ivr, ivi);
-#pragma unroll
-      for (int c = 0; c < NB; ++c) {
-        const double a = rkr[c], b = rki[c];
-        rkr[c] = a * ivr - b * ivi;
-        rki[c] = a * ivi + b * ivr;
-      }
-      // in-place trick: the pivot column of the pivot row holds 1/pivot
-#pragma unroll
-      for (int c = 0; c < NB; ++c)
-        if (c == kk) {
-          rkr[c] = ivr;
-          rki[c] = ivi;
-        }
-      __syncthreads();
-      for (int r = tid; r < n; r += NT) {
+    for (int j = 0; j < RPT; ++j) {
+      const int r = tid + NT * j;
+      if (r < n) {
         if (r == k) {
 #pragma unroll
           for (int c = 0; c < NB; ++c) {
-            Pr[r * PST + c] = (T)rkr[c];
-            Pi[r * PST + c] = (T)rki[c];
+            rr[j][c] = prow_r[c];
+            ri[j][c] = prow_i[c];
           }
         } else {
-          const double fr = (double)Pr[r * PST + kk], fi = (double)Pi[r * PST + kk];
+          if (r == p) {
 #pragma unroll
-          for (int c = 0; c < NB; ++c) {
-            double xr = (c == kk) ? 0.0 : (double)Pr[r * PST + c];
-            double xi = (c == kk) ? 0.0 : (double)Pi[r * PST + c];
-            xr -= fr * rkr[c] - fi * rki[c];
-            xi -= fr * rki[c] + fi * rkr[c];
-            Pr[r * PST + c] = (T)xr;
-            Pi[r * PST + c] = (T)xi;
+            for (int c = 0; c < NB; ++c) {
+              rr[j][c] = krow_r[c];
+              ri[j][c] = krow_i[c];
+            }
+          }
+          const T fr = rr[j][0], fi = ri[j][0];
+          {
+            const T br = prow_r[0], bi = prow_i[0];
+            rr[j][0] = -(fr * br - fi * bi);
+            ri[j][0] = -(fr * bi + fi * br);
+          }
+#pragma unroll
+          for (int c = 1; c < NB; ++c) {
+            const T br = prow_r[c], bi = prow_i[c];
+            rr[j][c] -= fr * br - fi * bi;
+            ri[j][c] -= fr * bi + fi * br;
+          }
+          if (kk + 1 < nbw && r > k) {
+            const unsigned kj = pivot_key((double)rr[j][1], (double)ri[j][1], r);
+            key = kj > key ? kj : key;
           }
         }
       }
-      __syncthreads();
-    }
-
-    // 2. row interchanges of this panel on the other columns
-    for (int c = tid; c < n; c += NT) {
-      if (c >= k0 && c < k0 + nbw) continue;
-      for (int kk = 0; kk < nbw; ++kk) {
-        const int k = k0 + kk, p = perm[k];
-        if (p != k) {
-          const z2 a = X[(int64_t)k * ldx + c];
-          X[(int64_t)k * ldx + c] = X[(int64_t)p * ldx + c];
-          X[(int64_t)p * ldx + c] = a;
-        }
-      }
-    }
-    __syncthreads();
-
-    // 3. rank-NB update of the other columns, NB columns at a time
-    const int nrt = (n + 15) / 16;
-    for (int cc = 0; cc < n; cc += NB) {
-      if (cc == k0) continue;
-      for (int idx = tid; idx < NB * NB; idx += NT) {
-        const int r = idx / NB, c = idx - r * NB;
-        z2 v = make_double2(0.0, 0.0);
-        if (r < nbw && cc + c < n) v = X[(int64_t)(k0 + r) * ldx + cc + c];
-        Xr[r * XST + c] = (T)v.x;
-        Xi[r * XST + c] = (T)v.y;
-      }
-      __syncthreads();
-      const int col = cc + (lane & 15);
-      for (int rt = w; rt < nrt; rt += NT / 64) {
-        acc_t aR, aI;
+      // rotate left by one: the next column moves to element 0
+      const T t_r = rr[j][0], t_i = ri[j][0];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int row = rt * 16 + MM::row(lane, q);
-          z2 v = make_double2(0.0, 0.0);
-          if (row < n && col < n && !(row >= k0 && row < k0 + nbw))
-            v = X[(int64_t)row * ldx + col];
-          aR[q] = (T)v.x;
-          aI[q] = (T)v.y;
-        }
-        const int arow = rt * 16 + (lane & 15);
-#pragma unroll
-        for (int kq = 0; kq < NB / 4; ++kq) {
-          const int kc = kq * 4 + (lane >> 4);
-          const T ar = (arow < n) ? Pr[arow * PST + kc] : (T)0;
-          const T ai = (arow < n) ? Pi[arow * PST + kc] : (T)0;
-          const T br = Xr[kc * XST + (lane & 15)];
-          const T bi2 = Xi[kc * XST + (lane & 15)];
-          aR = MM::mma(ar, br, aR);
-          aI = MM::mma(ar, bi2, aI);
-          aR = MM::mma(-ai, bi2, aR);
-          aI = MM::mma(ai, br, aI);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int row = rt * 16 + MM::row(lane, q);
-          if (row < n && col < n)
-            X[(int64_t)row * ldx + col] = make_double2((double)aR[q], (double)aI[q]);
-        }
+      for (int c = 0; c + 1 < NB; ++c) {
+        rr[j][c] = rr[j][c + 1];
+        ri[j][c] = ri[j][c + 1];
       }
-      __syncthreads();
+      rr[j][NB - 1] = t_r;
+      ri[j][NB - 1] = t_i;
     }
-
-    // 4. reduced panel -> X
-    for (int idx = tid; idx < n * NB; idx += NT) {
-      const int r = idx / NB, c = idx - r * NB;
-      if (c < nbw) X[(int64_t)r * ldx + k0 + c] = make_double2((double)Pr[r * PST + c], (double)Pi[r * PST + c]);
+    if (kk + 1 < nbw) {
+      key = wave_max(key);
+      if (lane == 0) red[w] = key;
     }
     __syncthreads();
   }
 
-  // undo the column interchanges in reverse order
-  for (int r = tid; r < n; r += NT) {
-    z2* xr = X + (int64_t)r * ldx;
-    for (int k = n - 1; k >= 0; --k) {
-      const int p = perm[k];
-      if (p != k) {
-        const z2 a = xr[k];
-        xr[k] = xr[p];
-        xr[p] = a;
+  // reduced panel -> Xn (element c now holds panel column (c + nbw) mod NB)
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) {
+    const int r = tid + NT * j;
+    if (r < n) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        int col = c + nbw;
+        col -= (col >= NB) ? NB : 0;
+        if (col < nbw)
+          Xn[(int64_t)r * g.ldn + k0 + col] = make_double2((double)rr[j][c], (double)ri[j][c]);
       }
     }
+  }
+  // composed row gather of this panel, pivot record
+  for (int r = tid; r < n; r += NT) sig_l[r] = r;
+  if (g.last)
+    for (int r = tid; r < k0; r += NT) perm_l[NB + r] = perm[r];  // earlier panels
+  __syncthreads();
+  if (tid == 0) {
+    for (int kk = 0; kk < nbw; ++kk) {
+      const int q = perm_l[kk];
+      const int t = sig_l[k0 + kk];
+      sig_l[k0 + kk] = sig_l[q];
+      sig_l[q] = t;
+    }
+  }
+  __syncthreads();
+  for (int r = tid; r < n; r += NT) sig[r] = sig_l[r];
+  for (int kk = tid; kk < nbw; kk += NT) perm[k0 + kk] = perm_l[kk];
+  if (g.last) {
+    // column gather of the whole inverse: reverse order of all pivots
+    __syncthreads();
+    for (int r = tid; r < n; r += NT) sig_l[r] = r;
+    __syncthreads();
+    if (tid == 0) {
+      for (int k = n - 1; k >= 0; --k) {
+        const int q = (k >= k0) ? perm_l[k - k0] : perm_l[NB + k];
+        const int t = sig_l[k];
+        sig_l[k] = sig_l[q];
+        sig_l[q] = t;
+      }
+    }
+    __syncthreads();
+    for (int r = tid; r < n; r += NT) idx[r] = sig_l[r];
   }
 }
 
-size_t smem_bytes(int n, int elem) {
-  return (size_t)(2 * n * PST + 2 * NB * XST) * elem + (size_t)((n + 1) & ~1) * 4 +
-         (NT / 64) * (8 + 4);
+struct UpdArgs {
+  int n, k0, nbw, tiles;
+  const z2* Xc;
+  int64_t sXc;
+  int ldc;
+  z2* Xn;
+  int64_t sXn;
+  int ldn;
+  const int* ints;
+  int64_t sI;
+};
+
+// Xn[r, c] = (r in K ? 0 : Xc[sig r, c]) + sum_k Xn[r, k0+k] Xc[sig(k0+k), c]
+// for the columns c outside the panel; 64 x 64 tiles, 4 waves of 32 x 32.
+template <typename T, int NB>
+__global__ __launch_bounds__(NT) void inv_update(UpdArgs g) {
+  typedef InvMma<T> MM;
+  typedef typename MM::acc_t acc_t;
+  __shared__ T As[2][NB][64 + 1];
+  __shared__ T Bs[2][NB][64 + 1];
+  __shared__ int sig_r[64], sig_k[NB];
+  const int tiles = g.tiles;  // per matrix, row-major over (row tile, col tile)
+  const int tn = (g.n + 63) / 64;
+  const int64_t e = blockIdx.x / tiles;
+  const int t = blockIdx.x - (int)(e * tiles);
+  const int r0 = (t / tn) * 64, c0 = (t % tn) * 64;
+  const int n = g.n, k0 = g.k0, nbw = g.nbw;
+  // a column tile that lies inside the panel has nothing to do
+  if (c0 >= k0 && c0 + 64 <= k0 + nbw) return;
+  const z2* Xc = g.Xc + e * g.sXc;
+  z2* Xn = g.Xn + e * g.sXn;
+  const int* sig = g.ints + e * g.sI + n;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+
+  if (tid < 64) sig_r[tid] = (r0 + tid < n) ? sig[r0 + tid] : 0;
+  if (tid < NB) sig_k[tid] = (tid < nbw) ? sig[k0 + tid] : 0;
+  __syncthreads();
+  // A = reduced panel rows of this tile, B = gathered pivot rows
+  for (int q = tid; q < 64 * NB; q += NT) {
+    const int r = q / NB, k = q - r * NB;
+    z2 v = make_double2(0.0, 0.0);
+    if (r0 + r < n && k < nbw) v = Xn[(int64_t)(r0 + r) * g.ldn + k0 + k];
+    As[0][k][r] = (T)v.x;
+    As[1][k][r] = (T)v.y;
+    const int k2 = q >> 6, c = q & 63;
+    z2 u = make_double2(0.0, 0.0);
+    if (k2 < nbw && c0 + c < n) u = Xc[(int64_t)sig_k[k2] * g.ldc + c0 + c];
+    Bs[0][k2][c] = (T)u.x;
+    Bs[1][k2][c] = (T)u.y;
+  }
+  // C init: gathered rows, pivot rows zeroed
+  acc_t aR[2][2], aI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rl = wm * 32 + i * 16 + MM::row(lane, q);
+        const int r = r0 + rl;
+        const int c = c0 + wn * 32 + j * 16 + (lane & 15);
+        z2 v = make_double2(0.0, 0.0);
+        if (r < n && c < n && !(r >= k0 && r < k0 + nbw)) v = Xc[(int64_t)sig_r[rl] * g.ldc + c];
+        aR[i][j][q] = (T)v.x;
+        aI[i][j][q] = (T)v.y;
+      }
+  __syncthreads();
+#pragma unroll
+  for (int kq = 0; kq < NB / 4; ++kq) {
+    const int kr = kq * 4 + (lane >> 4);
+    T a_r[2], a_i[2], na_i[2], b_r[2], b_i[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      a_r[i] = As[0][kr][wm * 32 + i * 16 + (lane & 15)];
+      a_i[i] = As[1][kr][wm * 32 + i * 16 + (lane & 15)];
+      na_i[i] = -a_i[i];
+      b_r[i] = Bs[0][kr][wn * 32 + i * 16 + (lane & 15)];
+      b_i[i] = Bs[1][kr][wn * 32 + i * 16 + (lane & 15)];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        aR[i][j] = MM::mma(a_r[i], b_r[j], aR[i][j]);
+        aI[i][j] = MM::mma(a_r[i], b_i[j], aI[i][j]);
+      }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        aR[i][j] = MM::mma(na_i[i], b_i[j], aR[i][j]);
+        aI[i][j] = MM::mma(a_i[i], b_r[j], aI[i][j]);
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + wm * 32 + i * 16 + MM::row(lane, q);
+        const int c = c0 + wn * 32 + j * 16 + (lane & 15);
+        if (r < n && c < n && !(c >= k0 && c < k0 + nbw))
+          Xn[(int64_t)r * g.ldn + c] = make_double2((double)aR[i][j][q], (double)aI[i][j][q]);
+      }
+}
+
+// X[r, c] = Y[r, idx c]; X may not alias Y.
+__global__ __launch_bounds__(NT) void inv_gather(int n, const z2* Y, int64_t sY, int ldy, z2* X,
+                                                 int64_t sX, int ldx, const int* ints, int64_t sI,
+                                                 int64_t total) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t q = blockIdx.x * (int64_t)NT + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * NT) {
+    const int64_t e = q / nn;
+    const int64_t rc = q - e * nn;
+    const int r = (int)(rc / n), c = (int)(rc - (int64_t)r * n);
+    const int* idx = ints + e * sI + 2 * n;
+    X[e * sX + (int64_t)r * ldx + c] = Y[e * sY + (int64_t)r * ldy + idx[c]];
+  }
+}
+
+template <typename T, int NB, int RPT>
+void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScratch& scr,
+                 int s0, int* info) {
+  const int npan = (n + NB - 1) / NB;
+  const Mat B[2] = {X, Mat(scr.buf, scr.buf_stride, n)};
+  int cur = s0;
+  const int tn = (n + 63) / 64;
+  for (int p = 0; p < npan; ++p) {
+    const int k0 = p * NB, nbw = min(NB, n - k0), nxt = 1 - cur;
+    PanelArgs pa{n, k0, nbw, p == npan - 1 ? 1 : 0, B[cur].p, B[cur].stride, B[cur].ld,
+                 B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, info};
+    hipLaunchKernelGGL((inv_panel<T, NB, RPT>), dim3((unsigned)batch), dim3(NT), 0, s, pa);
+    if (n > nbw) {
+      UpdArgs ua{n, k0, nbw, tn * tn, B[cur].p, B[cur].stride, B[cur].ld,
+                 B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride};
+      const int64_t blocks = batch * tn * tn;
+      QX_REQUIRE(blocks < (int64_t(1) << 31), "inv: grid too large");
+      hipLaunchKernelGGL((inv_update<T, NB>), dim3((unsigned)blocks), dim3(NT), 0, s, ua);
+    }
+    cur = nxt;
+  }
+  // the parity of s0 leaves the reduced matrix in the scratch buffer B[1]
+  const int64_t total = batch * (int64_t)n * n;
+  int64_t blocks = (total + NT - 1) / NT;
+  if (blocks > 65536 * 4) blocks = 65536 * 4;
+  hipLaunchKernelGGL(inv_gather, dim3((unsigned)blocks), dim3(NT), 0, s, n, B[1].p,
+                     B[1].stride, B[1].ld, X.p, X.stride, X.ld, scr.ints, scr.int_stride, total);
 }
 
 }  // namespace
 
-void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S,
-          const Mat& X, int* info, int precision) {
+void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const Mat& X,
+          int* info, int precision, const InvScratch& scr) {
   QX_REQUIRE(n >= 1 && n <= kInvMaxN, "inv: block size must be in [1, 512]");
   QX_REQUIRE(!A.zero() && !X.zero(), "inv: null operand");
+  QX_REQUIRE(scr.buf && scr.ints && scr.buf_stride >= (int64_t)n * n &&
+                 scr.int_stride >= 3 * (int64_t)n,
+             "inv: scratch too small");
+  QX_REQUIRE(X.p != scr.buf, "inv: output aliases the scratch buffer");
   if (batch <= 0) return;
   QX_REQUIRE(batch < (int64_t(1) << 31), "inv: batch too large");
-  InvArgs g{n, A.p, A.stride, A.ld, S.p, S.stride, S.ld, X.p, X.stride, X.ld, info};
-  const bool f32 = precision == QX_PREC_C64;
-  const size_t sm = smem_bytes(n, f32 ? 4 : 8);
   KernelTimer* timer = active_timer;
   hipEvent_t ev0 = nullptr;
   if (timer) timer->begin(s, &ev0);
-  if (f32) {
-    static bool attr = false;
-    if (!attr) {
-      QX_HIP_CHECK(hipFuncSetAttribute((const void*)zinv_kernel<float>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-      attr = true;
-    }
-    hipLaunchKernelGGL(zinv_kernel<float>, dim3((unsigned)batch), dim3(NT), sm, s, g);
+
+  const bool big = n > 256;
+  const int NB = big ? 16 : 32;
+  const int npan = (n + NB - 1) / NB;
+  // start in the buffer that makes the last panel write the scratch buffer
+  const int s0 = 1 ^ (npan % 2);
+  const Mat W(scr.buf, scr.buf_stride, n);
+  const Mat start = s0 == 0 ? X : W;
+  const bool in_place =
+      (A.p == start.p) && S.zero() && A.ld == start.ld && A.stride == start.stride;
+  if (!in_place) ew_axpby(s, batch, n, n, Cplx{1.0, 0.0}, A, Cplx{-1.0, 0.0}, S, start);
+
+  if (precision == QX_PREC_C64) {
+    if (big) run_inverse<float, 16, 2>(s, batch, n, X, scr, s0, info);
+    else run_inverse<float, 32, 1>(s, batch, n, X, scr, s0, info);
   } else {
-    static bool attr = false;
-    if (!attr) {
-      QX_HIP_CHECK(hipFuncSetAttribute((const void*)zinv_kernel<double>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-      attr = true;
-    }
-    hipLaunchKernelGGL(zinv_kernel<double>, dim3((unsigned)batch), dim3(NT), sm, s, g);
+    if (big) run_inverse<double, 16, 2>(s, batch, n, X, scr, s0, info);
+    else run_inverse<double, 32, 1>(s, batch, n, X, scr, s0, info);
   }
   QX_HIP_CHECK(hipGetLastError());
   if (timer) timer->end(s, ev0, KernelTimer::INV, 8.0 * (double)n * n * n * batch);

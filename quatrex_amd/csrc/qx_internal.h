@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -58,10 +60,18 @@ void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
            int precision);
 
 // X = inv(A - S) with partial pivoting (S may be zero).  info[e] is set to a
-// nonzero value for an exactly singular matrix.  n <= kInvMaxN.
+// nonzero value for an exactly singular matrix.  n <= kInvMaxN.  The scratch
+// holds, per matrix, an n x n complex buffer (buf + e * buf_stride) and 3 n
+// ints (ints + e * int_stride); A may alias X but neither may alias buf.
 constexpr int kInvMaxN = 512;
+struct InvScratch {
+  z2* buf;
+  int64_t buf_stride;
+  int* ints;
+  int64_t int_stride;
+};
 void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S,
-          const Mat& X, int* info, int precision);
+          const Mat& X, int* info, int precision, const InvScratch& scr);
 
 // Elementwise helpers on batched m x n blocks.
 // Y = a*A + b*B            (A, B may be zero matrices)
@@ -105,12 +115,21 @@ struct KernelTimer {
   std::vector<hipEvent_t> pool;
   std::vector<Rec> recs;
   size_t next = 0;
+  std::mutex mu;  // launches may come from several host threads
   hipEvent_t get();
   void begin(hipStream_t s, hipEvent_t* ev);
   void end(hipStream_t s, hipEvent_t a, int kind, double flops);
   ~KernelTimer();
 };
 extern thread_local KernelTimer* active_timer;
+
+// Splits a batch of S matrices into up to `handle->parts` contiguous
+// sub-batches and runs f(stream, e0, n) for each concurrently: sub-batch 0 on
+// the caller's stream `st` in this thread, the others on handle-owned streams
+// from helper host threads.  Events order all sub-batches after the work
+// already queued on `st`, and `st` waits for all of them before returning.
+void run_split(qx_handle* h, hipStream_t st, int64_t S,
+               const std::function<void(hipStream_t, int64_t, int64_t)>& f);
 
 }  // namespace qx
 
@@ -123,6 +142,10 @@ struct qx_handle {
   size_t info_len = 0;
   std::string last_error;
   qx::KernelTimer* timer = nullptr;
+  int parts = 4;                      // concurrent sub-batches of one RGF call
+  std::vector<hipStream_t> streams;   // helper streams (default priority)
+  std::vector<hipEvent_t> join;       // one per helper stream
+  hipEvent_t fork = nullptr;
   void* ensure_ws(size_t bytes);
   int* ensure_info(size_t n);
 };

@@ -18,13 +18,20 @@ namespace qx {
 
 namespace {
 
-const Cplx kOne{1.0, 0.0}, kMinus{-1.0, 0.0}, kZero{0.0, 0.0}, kHalf{0.5, 0.0};
+const Cplx kOne{1.0, 0.0}, kMinus{-1.0, 0.0}, kZero{0.0, 0.0};
 
+// One BTD stack in DSDBCOO layout, optionally restricted to the matrices
+// [e0, e0 + n) of the call (sub-batch).
 struct Btd {
   const qx_btd_layout* L = nullptr;
   z2* base = nullptr;
   Btd() = default;
   Btd(const qx_btd_layout* l, const void* p) : L(l), base(const_cast<z2*>(static_cast<const z2*>(p))) {}
+  Btd sub(int64_t e0) const {
+    Btd b = *this;
+    if (b.base) b.base += e0 * L->stack_stride;
+    return b;
+  }
   int bs(int i) const { return L->block_sizes[i]; }
   Mat at(int64_t off, int ld) const {
     if (!L || !base || off < 0) return Mat();
@@ -44,12 +51,30 @@ void check_layout(const qx_btd_layout* l, const qx_btd_layout* ref, const char* 
                std::string(what) + ": block sizes differ from A");
 }
 
-// Workspace of equally sized matrix stacks ("slots"), each S x bmax^2.
+// Workspace of equally sized matrix stacks ("slots"), each S x bmax^2;
+// e0 selects the sub-batch.
 struct Slots {
   z2* base;
   int64_t S;
   int64_t cap;  // elements per matrix
-  Mat get(int k, int ld) const { return Mat(base + (int64_t)k * S * cap, cap, ld); }
+  int64_t e0;
+  Mat get(int k, int ld) const { return Mat(base + (int64_t)k * S * cap + e0 * cap, cap, ld); }
+  // inverse scratch: complex buffer in slot k, pivot ints in slot k + 1
+  InvScratch inv(int k) const {
+    return InvScratch{base + (int64_t)k * S * cap + e0 * cap, cap,
+                      reinterpret_cast<int*>(base + (int64_t)(k + 1) * S * cap + e0 * cap),
+                      4 * cap};
+  }
+};
+
+// Per-block boundary self-energies, (S, b, b) contiguous, or absent.
+struct Obc {
+  const void* const* arr;
+  int64_t e0;
+  Mat get(int i, int b) const {
+    if (!arr || !arr[i]) return Mat();
+    return Mat(static_cast<const z2*>(arr[i]) + e0 * (int64_t)b * b, (int64_t)b * b, b);
+  }
 };
 
 // dst = a + sign * s (either may be zero); dst is a workspace matrix.
@@ -58,9 +83,14 @@ void fold(hipStream_t st, int64_t S, int m, int n, const Mat& a, double sign,
   ew_axpby(st, S, m, n, kOne, a, Cplx{sign, 0.0}, s, dst);
 }
 
-Mat obc_mat(const void* const* arr, int i, int b) {
-  if (!arr || !arr[i]) return Mat();
-  return Mat(arr[i], (int64_t)b * b, b);
+int block_limits(const qx_btd_layout* la, const char* what) {
+  int bmax = 1;
+  for (int i = 0; i < la->n_blocks; ++i) {
+    QX_REQUIRE(la->block_sizes[i] >= 1 && la->block_sizes[i] <= kInvMaxN,
+               std::string(what) + ": block sizes must be in [1, 512]");
+    bmax = std::max(bmax, (int)la->block_sizes[i]);
+  }
+  return bmax;
 }
 
 }  // namespace
@@ -68,74 +98,79 @@ Mat obc_mat(const void* const* arr, int i, int b) {
 size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve) {
   int bmax = 1;
   for (int i = 0; i < la->n_blocks; ++i) bmax = std::max(bmax, (int)la->block_sizes[i]);
-  const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 20 : (int64_t)la->n_blocks + 8;
+  const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 22 : (int64_t)la->n_blocks + 10;
   return (size_t)(slots * la->n_stack * (int64_t)bmax * bmax) * sizeof(z2);
 }
 
-void rgf_selected_inv(qx_handle* h, hipStream_t st, const qx_btd_layout* la,
+void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
                       const void* a, const void* const* obc_r,
                       const qx_btd_layout* lx, void* x, int prec) {
   QX_REQUIRE(la && lx && a && x, "selected_inv: null argument");
   QX_REQUIRE(la->n_blocks >= 1, "selected_inv: need at least one block");
   check_layout(lx, la, "x");
   const int N = la->n_blocks;
-  const int64_t S = la->n_stack;
-  if (S == 0) return;
-  int bmax = 1;
-  for (int i = 0; i < N; ++i) {
-    QX_REQUIRE(la->block_sizes[i] >= 1 && la->block_sizes[i] <= kInvMaxN,
-               "selected_inv: block sizes must be in [1, 512]");
-    bmax = std::max(bmax, (int)la->block_sizes[i]);
-  }
-  Btd A(la, a), X(lx, x);
-  Slots W{static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, false))), S, (int64_t)bmax * bmax};
-  int* info = h->ensure_info(S);
-  std::vector<int> xd(N);
-  for (int i = 0; i < N; ++i) xd[i] = i;
-  int spare = N, t1 = N + 1, t2 = N + 2, t3 = N + 3;
-  auto bs = [&](int i) { return A.bs(i); };
+  const int64_t Stot = la->n_stack;
+  if (Stot == 0) return;
+  const int bmax = block_limits(la, "selected_inv");
+  z2* ws = static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, false)));
+  int* info_all = h->ensure_info(Stot);
+  const Btd A0(la, a), X0(lx, x);
 
-  // Forward sweep (rgf.py:70-92).
-  fold(st, S, bs(0), bs(0), A.diag(0), -1.0, obc_mat(obc_r, 0, bs(0)), W.get(xd[0], bs(0)));
-  zinv(st, S, bs(0), W.get(xd[0], bs(0)), Mat(), W.get(xd[0], bs(0)), info, prec);
-  for (int i = 0; i + 1 < N; ++i) {
-    const int j = i + 1, bi = bs(i), bj = bs(j);
-    const Mat T1 = W.get(t1, bi);
-    zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get(xd[i], bi), kZero, T1, prec);
-    const Mat Xj = W.get(xd[j], bj);
-    fold(st, S, bj, bj, A.diag(j), -1.0, obc_mat(obc_r, j, bj), Xj);
-    zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xj, prec);
-    zinv(st, S, bj, Xj, Mat(), Xj, info, prec);
-  }
-  // rgf.py:94-95
-  if (!X.diag(N - 1).zero())
-    ew_axpby(st, S, bs(N - 1), bs(N - 1), kOne, W.get(xd[N - 1], bs(N - 1)), kZero, Mat(), X.diag(N - 1));
+  run_split(h, st0, Stot, [&](hipStream_t st, int64_t e0, int64_t S) {
+    const Btd A = A0.sub(e0), X = X0.sub(e0);
+    const Slots W{ws, Stot, (int64_t)bmax * bmax, e0};
+    const Obc OR{obc_r, e0};
+    int* info = info_all + e0;
+    std::vector<int> xd(N);
+    for (int i = 0; i < N; ++i) xd[i] = i;
+    int spare = N;
+    const int t1 = N + 1, t2 = N + 2, t3 = N + 3;
+    const InvScratch scr = W.inv(N + 4);
+    auto bs = [&](int i) { return A.bs(i); };
 
-  // Backward sweep (rgf.py:97-110).
-  for (int i = N - 2; i >= 0; --i) {
-    const int j = i + 1, bi = bs(i), bj = bs(j);
-    const Mat Xii = W.get(xd[i], bi), Xjj = W.get(xd[j], bj);
-    const Mat T1 = W.get(t1, bi), T2 = W.get(t2, bj);
-    // x_ji = -x_jj a_ji x_ii
-    zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xjj, A.lower(i), kZero, T1, prec);
-    Mat Xji = X.lower(i);
-    if (Xji.zero()) Xji = W.get(t3, bi);
-    zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, T1, Xii, kZero, Xji, prec);
-    // x_ij = -x_ii a_ij x_jj
-    zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xii, A.upper(i), kZero, T2, prec);
-    if (!X.upper(i).zero())
-      zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xjj, kZero, X.upper(i), prec);
-    // x_ii <- x_ii - x_ii a_ij x_ji
-    const Mat Z = W.get(spare, bi);
-    ew_axpby(st, S, bi, bi, kOne, Xii, kZero, Mat(), Z);
-    zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xji, kOne, Z, prec);
-    std::swap(spare, xd[i]);
-    if (!X.diag(i).zero())
-      ew_axpby(st, S, bi, bi, kOne, W.get(xd[i], bi), kZero, Mat(), X.diag(i));
-  }
+    // Forward sweep (rgf.py:70-92).
+    fold(st, S, bs(0), bs(0), A.diag(0), -1.0, OR.get(0, bs(0)), W.get(xd[0], bs(0)));
+    zinv(st, S, bs(0), W.get(xd[0], bs(0)), Mat(), W.get(xd[0], bs(0)), info, prec, scr);
+    for (int i = 0; i + 1 < N; ++i) {
+      const int j = i + 1, bi = bs(i), bj = bs(j);
+      const Mat T1 = W.get(t1, bi);
+      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get(xd[i], bi), kZero, T1, prec);
+      const Mat Xj = W.get(xd[j], bj);
+      fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xj);
+      zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xj, prec);
+      zinv(st, S, bj, Xj, Mat(), Xj, info, prec, scr);
+    }
+    // rgf.py:94-95
+    if (!X.diag(N - 1).zero())
+      ew_axpby(st, S, bs(N - 1), bs(N - 1), kOne, W.get(xd[N - 1], bs(N - 1)), kZero, Mat(),
+               X.diag(N - 1));
+
+    // Backward sweep (rgf.py:97-110).
+    for (int i = N - 2; i >= 0; --i) {
+      const int j = i + 1, bi = bs(i), bj = bs(j);
+      const Mat Xii = W.get(xd[i], bi), Xjj = W.get(xd[j], bj);
+      const Mat T1 = W.get(t1, bi), T2 = W.get(t2, bj);
+      // x_ji = -x_jj a_ji x_ii
+      zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xjj, A.lower(i), kZero, T1, prec);
+      Mat Xji = X.lower(i);
+      if (Xji.zero()) Xji = W.get(t3, bi);
+      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, T1, Xii, kZero, Xji, prec);
+      // x_ij = -x_ii a_ij x_jj
+      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xii, A.upper(i), kZero, T2, prec);
+      if (!X.upper(i).zero())
+        zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xjj, kZero, X.upper(i), prec);
+      // x_ii <- x_ii - x_ii a_ij x_ji
+      const Mat Z = W.get(spare, bi);
+      ew_axpby(st, S, bi, bi, kOne, Xii, kZero, Mat(), Z);
+      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xji, kOne, Z, prec);
+      std::swap(spare, xd[i]);
+      if (!X.diag(i).zero())
+        ew_axpby(st, S, bi, bi, kOne, W.get(xd[i], bi), kZero, Mat(), X.diag(i));
+    }
+  });
 }
 
-void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la,
+void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
                         const void* a, const qx_btd_layout* lsl, const void* sl,
                         const qx_btd_layout* lsg, const void* sg,
                         const void* const* obc_r, const void* const* obc_l,
@@ -156,19 +191,22 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la,
   check_layout(lxg, la, "x_greater");
   if (want_r) check_layout(lxr, la, "x_retarded");
   const int N = la->n_blocks;
-  const int64_t S = la->n_stack;
+  const int64_t Stot = la->n_stack;
   QX_REQUIRE(N >= 1, "selected_solve: need at least one block");
-  if (S == 0) return;
-  int bmax = 1;
-  for (int i = 0; i < N; ++i) {
-    QX_REQUIRE(la->block_sizes[i] >= 1 && la->block_sizes[i] <= kInvMaxN,
-               "selected_solve: block sizes must be in [1, 512]");
-    bmax = std::max(bmax, (int)la->block_sizes[i]);
-  }
-  Btd A(la, a), SL(lsl, sl), SG(lsg, sg), XL(lxl, xl), XG(lxg, xg),
-      XR(want_r ? lxr : nullptr, want_r ? xr : nullptr);
-  Slots W{static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, true))), S, (int64_t)bmax * bmax};
-  int* info = h->ensure_info(S);
+  if (Stot == 0) return;
+  const int bmax = block_limits(la, "selected_solve");
+  z2* ws = static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, true)));
+  int* info_all = h->ensure_info(Stot);
+  const Btd A0(la, a), SL0(lsl, sl), SG0(lsg, sg), XL0(lxl, xl), XG0(lxg, xg),
+      XR0(want_r ? lxr : nullptr, want_r ? xr : nullptr);
+
+  run_split(h, st0, Stot, [&](hipStream_t st, int64_t e0, int64_t S) {
+  Btd A = A0.sub(e0), SL = SL0.sub(e0), SG = SG0.sub(e0), XL = XL0.sub(e0), XG = XG0.sub(e0),
+      XR = XR0.sub(e0);
+  const Slots W{ws, Stot, (int64_t)bmax * bmax, e0};
+  const Obc OR{obc_r, e0}, OL{obc_l, e0}, OG{obc_g, e0};
+  int* info = info_all + e0;
+  z2* cur = current ? static_cast<z2*>(current) + e0 * (N - 1) : nullptr;
   auto bs = [&](int i) { return A.bs(i); };
 
   // slot bookkeeping: forward blocks (rotated in the backward sweep) + temps
@@ -182,26 +220,27 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la,
   const int T1s = 3 * N + 1, T2s = 3 * N + 2, T3s = 3 * N + 3, T4s = 3 * N + 4, T5s = 3 * N + 5,
             T6s = 3 * N + 6, B1s = 3 * N + 7, B2s = 3 * N + 8, B3s = 3 * N + 9, B4s = 3 * N + 10,
             C1s = 3 * N + 11, C2s = 3 * N + 12, C3s = 3 * N + 13, C4s = 3 * N + 14;
+  const InvScratch scr = W.inv(3 * N + 15);
 
   struct Q {
     Btd* sig;
     Btd* out;
     std::vector<int>* slot;
-    const void* const* obc;
+    const Obc* obc;
     bool skip_lower;
   };
-  Q qs[2] = {{&SL, &XL, &sl_, obc_l, (flags & QX_SOLVE_SKIP_LOWER_L) != 0},
-             {&SG, &XG, &sg_, obc_g, (flags & QX_SOLVE_SKIP_LOWER_G) != 0}};
+  Q qs[2] = {{&SL, &XL, &sl_, &OL, (flags & QX_SOLVE_SKIP_LOWER_L) != 0},
+             {&SG, &XG, &sg_, &OG, (flags & QX_SOLVE_SKIP_LOWER_G) != 0}};
 
   // ---- block 0 (rgf.py:207-231) ----
   {
     const int b0 = bs(0);
     const Mat X0 = W.get(sr[0], b0);
-    fold(st, S, b0, b0, A.diag(0), -1.0, obc_mat(obc_r, 0, b0), X0);
-    zinv(st, S, b0, X0, Mat(), X0, info, prec);
+    fold(st, S, b0, b0, A.diag(0), -1.0, OR.get(0, b0), X0);
+    zinv(st, S, b0, X0, Mat(), X0, info, prec, scr);
     for (auto& q : qs) {
       const Mat T3 = W.get(T3s, b0), T6 = W.get(T6s, b0);
-      fold(st, S, b0, b0, q.sig->diag(0), 1.0, obc_mat(q.obc, 0, b0), T3);
+      fold(st, S, b0, b0, q.sig->diag(0), 1.0, q.obc->get(0, b0), T3);
       zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_N, kOne, X0, T3, kZero, T6, prec);
       zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_C, kOne, T6, X0, kZero, W.get((*q.slot)[0], b0), prec);
     }
@@ -215,13 +254,13 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la,
     // a_ji xr_ii
     zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), Xri, kZero, T1, prec);
     // xr_jj = inv(a_jj - obc - a_ji xr_ii a_ij)
-    fold(st, S, bj, bj, A.diag(j), -1.0, obc_mat(obc_r, j, bj), Xrj);
+    fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xrj);
     zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xrj, prec);
-    zinv(st, S, bj, Xrj, Mat(), Xrj, info, prec);
+    zinv(st, S, bj, Xrj, Mat(), Xrj, info, prec, scr);
     for (auto& q : qs) {
       const Mat T3 = W.get(T3s, bj), T4 = W.get(T4s, bi), T5 = W.get(T5s, bj), T6 = W.get(T6s, bj);
       // sigma_jj + obc + a_ji x_ii a_ji^H + u^H - u,  u = a_ji xr_ii sigma_ij
-      fold(st, S, bj, bj, q.sig->diag(j), 1.0, obc_mat(q.obc, j, bj), T3);
+      fold(st, S, bj, bj, q.sig->diag(j), 1.0, q.obc->get(j, bj), T3);
       zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get((*q.slot)[i], bi), kZero, T4, prec);
       zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, A.lower(i), kOne, T3, prec);
       const Mat sij = q.sig->upper(i);
@@ -274,7 +313,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la,
       }
       // tr(sigma_greater_tilde xl_jj - xg_jj sigma_lesser_tilde)
       trace_diff(st, S, bj, C3, W.get(sl_[j], bj), W.get(sg_[j], bj), C2,
-                 static_cast<z2*>(current) + i, N - 1);
+                 cur + i, N - 1);
     }
 
     for (auto& q : qs) {
@@ -318,7 +357,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la,
     std::swap(spare, sr[i]);
     if (want_r && !XR.diag(i).zero()) ew_axpby(st, S, bi, bi, kOne, Z, kZero, Mat(), XR.diag(i));
   }
-  (void)kHalf;
+  });
 }
 
 }  // namespace qx
