@@ -1,0 +1,96 @@
+"""Batched complex linear algebra on torch device tensors through the C ABI.
+
+``gemm`` and ``inv`` are the two workhorses of the open-boundary solvers:
+they run libquatrex_amd's MFMA GEMM and pivoted Gauss-Jordan inverse on the
+current torch stream.  Inputs are (..., m, k) stacks; they are made
+contiguous complex128 on the HIP device (no CPU fallback).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from quatrex_amd import _lib
+
+
+def as_device(x, device=None) -> torch.Tensor:
+    """complex128 HIP tensor view/copy of ``x`` (numpy arrays are uploaded)."""
+    t = x if isinstance(x, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(x))
+    if device is None:
+        device = t.device if t.device.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
+    t = t.to(device=device, dtype=torch.complex128)
+    _lib.require_device_tensor(t, "operand")
+    return t.contiguous()
+
+
+def _flat(t: torch.Tensor):
+    return t.reshape((-1,) + tuple(t.shape[-2:]))
+
+
+def gemm(a, b, op_a: str = "N", op_b: str = "N", alpha=1.0, beta=0.0, out=None,
+         precision: str = "c128") -> torch.Tensor:
+    """out = alpha op(a) op(b) + beta out, op in {"N", "C"} (conjugate transpose).
+
+    Leading (batch) dimensions of ``a`` and ``b`` must be equal.
+    """
+    a = as_device(a)
+    b = as_device(b, a.device)
+    ops = {"N": _lib.OP_N, "C": _lib.OP_C}
+    oa, ob = ops[op_a], ops[op_b]
+    m, k = (a.shape[-2], a.shape[-1]) if oa == _lib.OP_N else (a.shape[-1], a.shape[-2])
+    kb, n = (b.shape[-2], b.shape[-1]) if ob == _lib.OP_N else (b.shape[-1], b.shape[-2])
+    if k != kb:
+        raise ValueError(f"inner dimensions differ: {k} != {kb}")
+    if a.shape[:-2] != b.shape[:-2]:
+        raise ValueError("batch shapes of a and b differ")
+    a3, b3 = _flat(a), _flat(b)
+    batch = a3.shape[0]
+    if out is None:
+        if beta != 0:
+            raise ValueError("beta != 0 needs out")
+        out = torch.empty(a.shape[:-2] + (m, n), dtype=torch.complex128, device=a.device)
+    if out.dtype != torch.complex128 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous complex128 tensor")
+    c3 = _flat(out)
+    al = (ctypes.c_double * 2)(complex(alpha).real, complex(alpha).imag)
+    be = (ctypes.c_double * 2)(complex(beta).real, complex(beta).imag)
+    h, st = _lib.handle(a.device), _lib.stream_ptr(a.device)
+    rc = _lib.load().qx_zgemm_batched(
+        h, st, batch, m, n, k, oa, ob, al, a3.data_ptr(), a3.shape[-1], a3.stride(0),
+        b3.data_ptr(), b3.shape[-1], b3.stride(0), be, c3.data_ptr(), c3.shape[-1],
+        c3.stride(0), _lib.PRECISIONS[precision])
+    _lib.check(rc, h, "gemm")
+    return out
+
+
+def mm(*factors, precision: str = "c128") -> torch.Tensor:
+    """Left-to-right product of several stacks (as ``a @ b @ c`` in NumPy)."""
+    out = factors[0]
+    for f in factors[1:]:
+        out = gemm(out, f, precision=precision)
+    return out
+
+
+def inv(a, precision: str = "c128", check: bool = True) -> torch.Tensor:
+    """Batched inverse with partial pivoting; raises LinAlgError if singular."""
+    a = as_device(a)
+    n = a.shape[-1]
+    if a.shape[-2] != n:
+        raise ValueError("inv needs square matrices")
+    a3 = _flat(a)
+    out = torch.empty_like(a)
+    x3 = _flat(out)
+    h, st = _lib.handle(a.device), _lib.stream_ptr(a.device)
+    rc = _lib.load().qx_inv_batched(h, st, a3.shape[0], n, a3.data_ptr(), n, n * n, None, n,
+                                    n * n, x3.data_ptr(), n, n * n, _lib.PRECISIONS[precision])
+    _lib.check(rc, h, "inv")
+    if check:
+        _lib.sync_check(a.device, "inv")
+    return out
+
+
+def dagger(x: torch.Tensor) -> torch.Tensor:
+    return x.conj().transpose(-2, -1).resolve_conj().contiguous()

@@ -1,0 +1,118 @@
+"""Open-boundary solvers on the HIP path against the NumPy oracle.
+
+Sancho-Rubio (same batch-global stop rule and iteration count), the memoizing
+OBCSystem, the Beyn contour moments and eigenpairs (same probe stream), Full
+and Spectral.  Bar: 1e-10 relative Frobenius error for the complex128
+quantities; eigenvalues compared as sets.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.nevp as onevp
+import oracle.obc as oobc
+from quatrex_amd.boundary_conditions import OBCSystem, SanchoRubio, Spectral
+from quatrex_amd.nevp import Beyn, Full, operator_inverse
+from tests.conftest import rel_err
+from tests.test_oracle_obc import chain_surface_g, lead
+
+pytestmark = pytest.mark.gpu
+
+
+def np_(t):
+    return t.cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+
+
+@pytest.mark.parametrize("b,ne", [(12, 3), (64, 8)])
+def test_sancho_rubio_matches_oracle(device, b, ne):
+    a_ji, a_ii, a_ij = lead(b=b, energies=np.linspace(-0.8, 0.8, ne), eta=1e-3, seed=b)
+    o = oobc.SanchoRubio(max_iterations=100, convergence_tol=1e-12)
+    ref = o(a_ii, a_ij, a_ji)
+    g = SanchoRubio(max_iterations=100, convergence_tol=1e-12)
+    x = g(torch.as_tensor(a_ii, device=device), a_ij, a_ji)
+    assert g.iterations == o.iterations
+    assert rel_err(np_(x), ref) < 1e-10
+
+
+def test_sancho_rubio_chain_closed_form(device):
+    z = np.array([-1.5, -0.3, 0.0, 0.7, 2.5]) + 1e-3j
+    a_ii = (z - 0.2)[:, None, None]
+    a_ij = np.full((len(z), 1, 1), -0.8, complex)
+    x = SanchoRubio(max_iterations=200, convergence_tol=1e-13)(a_ii, a_ij, a_ij.copy())
+    assert np.allclose(np_(x)[:, 0, 0], chain_surface_g(z, 0.2, 0.8), rtol=1e-9, atol=1e-12)
+
+
+def test_sancho_rubio_single_matrix_and_warning(device):
+    a_ji, a_ii, a_ij = lead()
+    x = SanchoRubio(convergence_tol=1e-10)(a_ii[0], a_ij[0], a_ji[0])
+    assert x.shape == a_ii[0].shape
+    xn = np_(x)
+    assert np.allclose(xn, np.linalg.inv(a_ii[0] - a_ji[0] @ xn @ a_ij[0]))
+    with pytest.warns(RuntimeWarning):
+        SanchoRubio(max_iterations=1, convergence_tol=1e-8)(a_ii, a_ij, a_ji)
+
+
+@pytest.mark.parametrize("mode", ["off", "auto", "force-after-first"])
+def test_obc_system_matches_oracle(device, mode):
+    a_ji, a_ii, a_ij = lead()
+    ref_sys = oobc.OBCSystem(oobc.SanchoRubio(convergence_tol=1e-12), memoization_mode=mode)
+    sys_ = OBCSystem(SanchoRubio(convergence_tol=1e-12), memoization_mode=mode)
+    for scale in (1.0, 1 + 1e-6):
+        xr, rr, ar = ref_sys((a_ii * scale, a_ij * scale, a_ji * scale), "left")
+        x, r, a = sys_((a_ii * scale, a_ij * scale, a_ji * scale), "left")
+        assert rel_err(np_(x), xr) < 1e-9
+        assert np.allclose(np_(a), ar, rtol=1e-6, atol=1e-12)
+
+
+def test_operator_inverse(device):
+    rng = np.random.default_rng(0)
+    a_xx = tuple(rng.random((3, 1, 6, 6)) + 1j * rng.random((3, 1, 6, 6)) for _ in range(3))
+    z = rng.random((1, 5, 1, 1)) + 1j * rng.random((1, 5, 1, 1))
+    got = np_(operator_inverse(a_xx, z))
+    ref = np.linalg.inv(sum(z ** n * a for a, n in zip(a_xx, range(-1, 2))))
+    assert rel_err(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("b,nq", [(10, 64), (64, 32)])
+def test_beyn_moments_match_oracle(device, b, nq):
+    a_xx = lead(b=b, rank=b // 2, seed=3)
+    ob = onevp.Beyn(r_o=3.0, r_i=1 / 3.0, m_0=b, num_quad_points=nq)
+    q0r, q1r = ob._contour_integrate(a_xx)
+    gb = Beyn(r_o=3.0, r_i=1 / 3.0, m_0=b, num_quad_points=nq)
+    q0, q1 = gb._contour_integrate([torch.as_tensor(a, device=device) for a in a_xx])
+    assert rel_err(np_(q0), q0r) < 1e-10
+    assert rel_err(np_(q1), q1r) < 1e-10
+
+
+@pytest.mark.parametrize("use_qr", [False, True])
+def test_beyn_eigenpairs_match_oracle(device, use_qr):
+    a_xx = lead(b=10, rank=5)
+    wr, vr = onevp.Beyn(r_o=3.0, r_i=1 / 3.0, m_0=10, num_quad_points=64, use_qr=use_qr)(a_xx)
+    w, v = Beyn(r_o=3.0, r_i=1 / 3.0, m_0=10, num_quad_points=64, use_qr=use_qr)(a_xx)
+    w = np_(w)
+    for e in range(w.shape[0]):
+        ref = wr[e][np.abs(wr[e]) > 0]
+        got = w[e][np.abs(w[e]) > 0]
+        assert len(ref) == len(got)
+        for lam in ref:
+            assert np.min(np.abs(got - lam)) < 1e-8 * max(1.0, abs(lam))
+
+
+def test_full_matches_oracle(device):
+    a_xx = lead(b=8)
+    wr, _ = onevp.Full()(a_xx)
+    w, _ = Full()(a_xx)
+    w = np_(w)
+    for e in range(w.shape[0]):
+        for lam in wr[e]:
+            assert np.min(np.abs(w[e] - lam)) < 1e-8 * max(1.0, abs(lam))
+
+
+def test_spectral_full_matches_sancho_rubio(device):
+    a_ji, a_ii, a_ij = lead(b=8, eta=1e-3)
+    x_ref = oobc.Spectral(onevp.Full(), residual_tolerance=1e-6)(a_ii, a_ij, a_ji)
+    x = Spectral(Full(), residual_tolerance=1e-6)(a_ii, a_ij, a_ji)
+    assert rel_err(np_(x), x_ref) < 1e-8
+    x_sr = SanchoRubio(max_iterations=500, convergence_tol=1e-13)(a_ii, a_ij, a_ji)
+    assert rel_err(np_(x), np_(x_sr)) < 1e-7
