@@ -187,20 +187,26 @@ def main():
     kernels = None
     if timer:
         g, inv = timer.stats["gemm"], timer.stats["inv"]
-        achieved = g["flops"] / (g["ms"] / 1e3) / 1e12
+        # Sub-batches run on concurrent streams, so launches overlap: the GEMM
+        # throughput is its flops over the wall time during which at least one
+        # GEMM launch was in flight (union of the HIP-event intervals).
+        achieved = g["flops"] / (g["busy_ms"] / 1e3) / 1e12
         peak = PEAK_TFLOPS[args.precision]
+        per_launch = g["flops"] / max(g["launches"], 1)
+        avg_ms = g["ms"] / max(g["launches"], 1)
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
                     "kernel": "zgemm_kernel (batched complex GEMM, 64x64 tiles)",
-                    "avg_launch_ms": round(g["ms"] / max(g["launches"], 1), 4),
-                    "flops_per_launch": g["flops"] / max(g["launches"], 1)}
-        gpu_ms = g["ms"] + inv["ms"]
+                    "flops_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
+                    "per_launch_tflops": round(per_launch / (avg_ms / 1e3) / 1e12, 3),
+                    "busy_ms": round(g["busy_ms"], 2)}
         kernels = {
-            "gemm": {"launches": g["launches"], "ms": round(g["ms"], 2),
-                     "tflops": round(achieved, 3)},
-            "inverse": {"launches": inv["launches"], "ms": round(inv["ms"], 2),
-                        "tflops": round(inv["flops"] / max(inv["ms"], 1e-9) / 1e9, 3)},
-            "gemm_plus_inverse_share_of_wall": round(gpu_ms / (elapsed * 1e3), 4),
+            "gemm": {"launches": g["launches"], "sum_ms": round(g["ms"], 2),
+                     "busy_ms": round(g["busy_ms"], 2), "tflops": round(achieved, 3)},
+            "inverse": {"launches": inv["launches"], "sum_ms": round(inv["ms"], 2),
+                        "busy_ms": round(inv["busy_ms"], 2),
+                        "tflops": round(inv["flops"] / max(inv["busy_ms"], 1e-9) / 1e9, 3)},
+            "wall_ms": round(elapsed * 1e3, 2),
         }
     flops = solve_flops(N, b)
     result = {

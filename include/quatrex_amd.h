@@ -80,11 +80,12 @@ int qx_workspace_query(qx_handle* handle, const qx_btd_layout* layout,
  * profiler, src/qttools/profiling/profiler.py:187-429, for the kernels of
  * this path).  Between qx_timer_start and qx_timer_stop every GEMM and
  * inverse launch made through `handle` is bracketed by HIP events on its
- * stream.  qx_timer_stop waits for `stream` and writes host stats[6] =
- * {gemm launches, gemm ms, gemm flops, inverse launches, inverse ms,
- * inverse flops} (flops = algorithmic 8 m n k per complex product and
- * 8 n^3 per complex inverse). */
-int qx_timer_start(qx_handle* handle);
+ * stream.  qx_timer_stop waits for the device and writes host stats[8] =
+ * {gemm launches, gemm ms summed over launches, gemm flops, gemm ms of wall
+ * time covered by at least one gemm launch, then the same four for the
+ * inverse} (flops = algorithmic 8 m n k per complex product and 8 n^3 per
+ * complex inverse).  `stream` of qx_timer_start anchors the time origin. */
+int qx_timer_start(qx_handle* handle, void* stream);
 int qx_timer_stop(qx_handle* handle, void* stream, double* stats);
 
 /* Wait for `stream`, then report (and clear) device-side errors recorded

@@ -110,7 +110,7 @@ def load() -> ctypes.CDLL:
         lib.qx_sancho_rubio.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, i32, f64,
                                         ctypes.POINTER(ctypes.c_int)]
         lib.qx_beyn_moments.argtypes = [vp, vp, i64, i32, i32, f64, f64, vp, vp, vp, vp, vp, i32]
-        lib.qx_timer_start.argtypes = [vp]
+        lib.qx_timer_start.argtypes = [vp, vp]
         lib.qx_timer_stop.argtypes = [vp, vp, ctypes.POINTER(f64)]
         for name in EXPORTED_SYMBOLS:
             getattr(lib, name).restype = getattr(lib, name).restype or i32
@@ -179,16 +179,17 @@ class KernelTimer:
 
     def __enter__(self):
         h = handle(self.device)
-        check(load().qx_timer_start(h), h, "timer_start")
+        check(load().qx_timer_start(h, stream_ptr(self.device)), h, "timer_start")
         return self
 
     def __exit__(self, *exc):
         h = handle(self.device)
-        out = (ctypes.c_double * 6)()
+        out = (ctypes.c_double * 8)()
         check(load().qx_timer_stop(h, stream_ptr(self.device), out), h, "timer_stop")
         self.stats = {
-            "gemm": {"launches": int(out[0]), "ms": out[1], "flops": out[2]},
-            "inv": {"launches": int(out[3]), "ms": out[4], "flops": out[5]},
+            name: {"launches": int(out[4 * k]), "ms": out[4 * k + 1], "flops": out[4 * k + 2],
+                   "busy_ms": out[4 * k + 3]}
+            for k, name in enumerate(("gemm", "inv"))
         }
         return False
 

@@ -309,12 +309,14 @@ int qx_create(qx_handle** handle, int device) {
   return QX_OK;
 }
 
-int qx_timer_start(qx_handle* h) {
+int qx_timer_start(qx_handle* h, void* stream) {
   return guarded(h, [&]() {
     QX_REQUIRE(h, "null handle");
     if (!h->timer) h->timer = new qx::KernelTimer();
     h->timer->recs.clear();
     h->timer->next = 0;
+    h->timer->t0 = h->timer->get();
+    QX_HIP_CHECK(hipEventRecord(h->timer->t0, static_cast<hipStream_t>(stream)));
     return QX_OK;
   });
 }
@@ -322,16 +324,36 @@ int qx_timer_start(qx_handle* h) {
 int qx_timer_stop(qx_handle* h, void* stream, double* stats) {
   return guarded(h, [&]() {
     QX_REQUIRE(h && stats, "timer_stop: null argument");
-    for (int i = 0; i < 3 * qx::KernelTimer::NKIND; ++i) stats[i] = 0.0;
+    for (int i = 0; i < 4 * qx::KernelTimer::NKIND; ++i) stats[i] = 0.0;
     if (!h->timer) return QX_OK;
     QX_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    QX_HIP_CHECK(hipDeviceSynchronize());
+    std::vector<std::pair<double, double>> iv[qx::KernelTimer::NKIND];
     for (const auto& r : h->timer->recs) {
-      float ms = 0.f;
-      QX_HIP_CHECK(hipEventSynchronize(r.b));
+      float ms = 0.f, s0 = 0.f, s1 = 0.f;
       QX_HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
-      stats[3 * r.kind + 0] += 1.0;
-      stats[3 * r.kind + 1] += ms;
-      stats[3 * r.kind + 2] += r.flops;
+      QX_HIP_CHECK(hipEventElapsedTime(&s0, h->timer->t0, r.a));
+      QX_HIP_CHECK(hipEventElapsedTime(&s1, h->timer->t0, r.b));
+      stats[4 * r.kind + 0] += 1.0;
+      stats[4 * r.kind + 1] += ms;
+      stats[4 * r.kind + 2] += r.flops;
+      iv[r.kind].emplace_back(s0, s1);
+    }
+    // wall time covered by at least one launch of the family
+    for (int k = 0; k < qx::KernelTimer::NKIND; ++k) {
+      std::sort(iv[k].begin(), iv[k].end());
+      double covered = 0.0, lo = -1.0, hi = -1.0;
+      for (const auto& p : iv[k]) {
+        if (p.first > hi) {
+          if (hi > lo) covered += hi - lo;
+          lo = p.first;
+          hi = p.second;
+        } else if (p.second > hi) {
+          hi = p.second;
+        }
+      }
+      if (hi > lo) covered += hi - lo;
+      stats[4 * k + 3] = covered;
     }
     delete h->timer;
     h->timer = nullptr;

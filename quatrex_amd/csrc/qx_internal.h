@@ -115,6 +115,7 @@ struct KernelTimer {
   std::vector<hipEvent_t> pool;
   std::vector<Rec> recs;
   size_t next = 0;
+  hipEvent_t t0 = nullptr;  // recorded on the caller's stream at start
   std::mutex mu;  // launches may come from several host threads
   hipEvent_t get();
   void begin(hipStream_t s, hipEvent_t* ev);
