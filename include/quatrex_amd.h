@@ -107,7 +107,11 @@ int qx_rgf_selected_inv(qx_handle* handle, void* stream,
  * blocks (diagonal anti-Hermitian-projected, lower = -(upper)^H unless the
  * SKIP_LOWER flag is set), G^R diagonal (QX_SOLVE_RETARDED) and optionally
  * off-diagonal blocks, and the layer current (n_stack, n_blocks-1) complex
- * when QX_SOLVE_CURRENT is set.  obc_*: OBCBlocks.retarded/lesser/greater. */
+ * when QX_SOLVE_CURRENT is set.  obc_*: OBCBlocks.retarded/lesser/greater.
+ * corner (nullable, no counterpart in the reference): receives the retarded
+ * corner block G^R_{N-1,0}, (n_stack, b_{N-1}, b_0) contiguous, from the
+ * forward recursion g_{j,0} = -x^R_jj A_{j,j-1} g_{j-1,0}; it gives the
+ * transmission Tr[Gamma_R G^R_{N-1,0} Gamma_L (G^R_{N-1,0})^H]. */
 int qx_rgf_selected_solve(qx_handle* handle, void* stream,
                           const qx_btd_layout* la, const void* a,
                           const qx_btd_layout* lsl, const void* sigma_lesser,
@@ -117,7 +121,8 @@ int qx_rgf_selected_solve(qx_handle* handle, void* stream,
                           const qx_btd_layout* lxl, void* x_lesser,
                           const qx_btd_layout* lxg, void* x_greater,
                           const qx_btd_layout* lxr, void* x_retarded,
-                          void* current, uint32_t flags, int precision);
+                          void* current, void* corner, uint32_t flags,
+                          int precision);
 
 /* Batched complex GEMM C = alpha op(A) op(B) + beta C, op in {N, C}.
  * Replaces the batched `@` of the reference (cuBLAS zgemm strided-batched).

@@ -69,8 +69,13 @@ class RGF:
 
     def selected_solve(self, a, sigma_lesser, sigma_greater, obc_blocks=None,
                        out=None, return_retarded=False, return_current=False,
-                       retarded_offdiag=False):
-        """rgf.py:115-414."""
+                       retarded_offdiag=False, corner=None):
+        """rgf.py:115-414.
+
+        Extension: ``corner`` (array (*stack, b_{N-1}, b_0)) receives the
+        retarded corner block G^R_{N-1,0} from the forward recursion
+        g_{j,0} = -x^R_jj A_{j,j-1} g_{j-1,0}, g_{0,0} = x^R_00.
+        """
         n = a.num_blocks
         if obc_blocks is None:
             obc_blocks = OBCBlocks(n)
@@ -111,6 +116,7 @@ class RGF:
             xr0 = np.linalg.inv(a00)
             xr0h = _h(xr0)
             xrd[0], xld[0], xgd[0] = xr0, xr0 @ sl00 @ xr0h, xr0 @ sg00 @ xr0h
+            g_c = xr0
 
             # Forward sweep (rgf.py:234-295).
             for i in range(n - 1):
@@ -123,11 +129,15 @@ class RGF:
                 xrjj = np.linalg.inv(ajj - aji_xrii @ a_.blocks[i, j])
                 xrjjh = _h(xrjj)
                 xrd[j] = xrjj
+                if corner is not None:
+                    g_c = -xrjj @ (aji @ g_c)
                 t = aji_xrii @ sl_.blocks[i, j]
                 xld[j] = xrjj @ (sljj + aji @ xld[i] @ ajih + _h(t) - t) @ xrjjh
                 t = aji_xrii @ sg_.blocks[i, j]
                 xgd[j] = xrjj @ (sgjj + aji @ xgd[i] @ ajih + _h(t) - t) @ xrjjh
 
+            if corner is not None:
+                corner[sl] = g_c
             xl_.blocks[n - 1, n - 1] = 0.5 * (xld[-1] - _h(xld[-1]))
             xg_.blocks[n - 1, n - 1] = 0.5 * (xgd[-1] - _h(xgd[-1]))
             if return_retarded:
@@ -188,3 +198,8 @@ class RGF:
         if return_current:
             return current
         return None
+
+
+def transmission(g_n1, gamma_l, gamma_r):
+    """T(E) = Tr[Gamma_R G^R_{N-1,0} Gamma_L (G^R_{N-1,0})^H] (Caroli/Landauer)."""
+    return np.trace(gamma_r @ g_n1 @ gamma_l @ _h(g_n1), axis1=-2, axis2=-1).real

@@ -101,7 +101,7 @@ def load() -> ctypes.CDLL:
         lib.qx_rgf_selected_inv.argtypes = [vp, vp, lay_p, vp, vp, lay_p, vp, i32]
         lib.qx_rgf_selected_solve.argtypes = [
             vp, vp, lay_p, vp, lay_p, vp, lay_p, vp, vp, vp, vp, lay_p, vp, lay_p, vp,
-            lay_p, vp, vp, ctypes.c_uint32, i32]
+            lay_p, vp, vp, vp, ctypes.c_uint32, i32]
         lib.qx_zgemm_batched.argtypes = [
             vp, vp, i64, i32, i32, i32, i32, i32, ctypes.POINTER(f64), vp, i32, i64, vp, i32,
             i64, ctypes.POINTER(f64), vp, i32, i64, i32]

@@ -18,7 +18,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, c
                         const void* sg, const void* const* obc_r, const void* const* obc_l,
                         const void* const* obc_g, const qx_btd_layout* lxl, void* xl,
                         const qx_btd_layout* lxg, void* xg, const qx_btd_layout* lxr, void* xr,
-                        void* current, uint32_t flags, int prec);
+                        void* current, void* corner, uint32_t flags, int prec);
 
 namespace {
 
@@ -421,11 +421,12 @@ int qx_rgf_selected_solve(qx_handle* h, void* stream, const qx_btd_layout* la, c
                           const void* sg, const void* const* obc_r, const void* const* obc_l,
                           const void* const* obc_g, const qx_btd_layout* lxl, void* xl,
                           const qx_btd_layout* lxg, void* xg, const qx_btd_layout* lxr, void* xr,
-                          void* current, uint32_t flags, int precision) {
+                          void* current, void* corner, uint32_t flags, int precision) {
   return guarded(h, [&]() {
     QX_REQUIRE(h, "null handle");
     qx::rgf_selected_solve(h, static_cast<hipStream_t>(stream), la, a, lsl, sl, lsg, sg, obc_r,
-                           obc_l, obc_g, lxl, xl, lxg, xg, lxr, xr, current, flags, precision);
+                           obc_l, obc_g, lxl, xl, lxg, xg, lxr, xr, current, corner, flags,
+                           precision);
     return QX_OK;
   });
 }

@@ -98,7 +98,7 @@ int block_limits(const qx_btd_layout* la, const char* what) {
 size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve) {
   int bmax = 1;
   for (int i = 0; i < la->n_blocks; ++i) bmax = std::max(bmax, (int)la->block_sizes[i]);
-  const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 22 : (int64_t)la->n_blocks + 10;
+  const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 22 : (int64_t)la->n_blocks + 10;  // see slot maps
   return (size_t)(slots * la->n_stack * (int64_t)bmax * bmax) * sizeof(z2);
 }
 
@@ -177,7 +177,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
                         const void* const* obc_g, const qx_btd_layout* lxl,
                         void* xl, const qx_btd_layout* lxg, void* xg,
                         const qx_btd_layout* lxr, void* xr, void* current,
-                        uint32_t flags, int prec) {
+                        void* corner, uint32_t flags, int prec) {
   QX_REQUIRE(la && a, "selected_solve: null A");
   QX_REQUIRE(lxl && xl && lxg && xg, "selected_solve: null lesser/greater output");
   const bool want_r = (flags & QX_SOLVE_RETARDED) != 0;
@@ -207,6 +207,10 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   const Obc OR{obc_r, e0}, OL{obc_l, e0}, OG{obc_g, e0};
   int* info = info_all + e0;
   z2* cur = current ? static_cast<z2*>(current) + e0 * (N - 1) : nullptr;
+  // corner block G^R_{N-1,0}: ping-pong slots + product temp
+  const int b0 = A.bs(0), bl = A.bs(N - 1);
+  int gc_cur = 3 * N + 17, gc_nxt = 3 * N + 18;
+  const int gc_tmp = 3 * N + 19;
   auto bs = [&](int i) { return A.bs(i); };
 
   // slot bookkeeping: forward blocks (rotated in the backward sweep) + temps
@@ -238,6 +242,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const Mat X0 = W.get(sr[0], b0);
     fold(st, S, b0, b0, A.diag(0), -1.0, OR.get(0, b0), X0);
     zinv(st, S, b0, X0, Mat(), X0, info, prec, scr);
+    if (corner) ew_axpby(st, S, b0, b0, kOne, X0, kZero, Mat(), W.get(gc_cur, b0));
     for (auto& q : qs) {
       const Mat T3 = W.get(T3s, b0), T6 = W.get(T6s, b0);
       fold(st, S, b0, b0, q.sig->diag(0), 1.0, q.obc->get(0, b0), T3);
@@ -257,6 +262,14 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xrj);
     zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xrj, prec);
     zinv(st, S, bj, Xrj, Mat(), Xrj, info, prec, scr);
+    if (corner) {
+      // g_{j,0} = -x^R_jj (a_ji g_{i,0})
+      zgemm(st, S, bj, b0, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get(gc_cur, b0), kZero,
+            W.get(gc_tmp, b0), prec);
+      zgemm(st, S, bj, b0, bj, QX_OP_N, QX_OP_N, kMinus, Xrj, W.get(gc_tmp, b0), kZero,
+            W.get(gc_nxt, b0), prec);
+      std::swap(gc_cur, gc_nxt);
+    }
     for (auto& q : qs) {
       const Mat T3 = W.get(T3s, bj), T4 = W.get(T4s, bi), T5 = W.get(T5s, bj), T6 = W.get(T6s, bj);
       // sigma_jj + obc + a_ji x_ii a_ji^H + u^H - u,  u = a_ji xr_ii sigma_ij
@@ -272,6 +285,10 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_C, kOne, T6, Xrj, kZero, W.get((*q.slot)[j], bj), prec);
     }
   }
+
+  if (corner)
+    ew_axpby(st, S, bl, b0, kOne, W.get(gc_cur, b0), kZero, Mat(),
+             Mat(static_cast<z2*>(corner) + e0 * (int64_t)bl * b0, (int64_t)bl * b0, b0));
 
   // ---- last diagonal blocks (rgf.py:297-305) ----
   {

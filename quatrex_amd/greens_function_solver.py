@@ -216,7 +216,10 @@ class RGF(GFSolver):
     # rgf.py:115-414
     def selected_solve(self, a: DSDBCOO, sigma_lesser: DSDBCOO, sigma_greater: DSDBCOO,
                        obc_blocks: OBCBlocks | None = None, out=None,
-                       return_retarded: bool = False, return_current: bool = False):
+                       return_retarded: bool = False, return_current: bool = False,
+                       *, corner: torch.Tensor | None = None):
+        """rgf.py:115-414.  Extension: ``corner`` (complex128 device tensor of
+        shape (*stack, b_{N-1}, b_0)) receives G^R_{N-1,0} for the transmission."""
         lib = _lib.load()
         _lib.require_device_tensor(a.data, "a.data")
         dev = a.device
@@ -267,6 +270,15 @@ class RGF(GFSolver):
             o_r = _obc_array(obc_blocks.retarded, s0, s1, n, dev, keep)
             o_l = _obc_array(obc_blocks.lesser, s0, s1, n, dev, keep)
             o_g = _obc_array(obc_blocks.greater, s0, s1, n, dev, keep)
+            corner_ptr = None
+            if corner is not None:
+                nl, n0 = int(a.block_sizes[-1]), int(a.block_sizes[0])
+                if (corner.dtype != torch.complex128 or not corner.is_contiguous()
+                        or tuple(corner.shape) != tuple(a.shape[:-2]) + (nl, n0)):
+                    raise ValueError("corner must be a contiguous complex128 tensor of shape "
+                                     "(*stack, b_{N-1}, b_0)")
+                corner_ptr = ctypes.c_void_p(corner.data_ptr()
+                                             + s0 * inner * nl * n0 * corner.element_size())
             cur_ptr = None
             if current is not None:
                 cur_ptr = ctypes.c_void_p(current.data_ptr()
@@ -274,7 +286,8 @@ class RGF(GFSolver):
             rc = lib.qx_rgf_selected_solve(
                 h, st, A.lay(), A.ptr(), SL.lay(), SL.ptr(), SG.lay(), SG.ptr(), o_r, o_l, o_g,
                 XL.lay(), XL.ptr(), XG.lay(), XG.ptr(),
-                XR.lay() if XR else None, XR.ptr() if XR else None, cur_ptr, flags, prec)
+                XR.lay() if XR else None, XR.ptr() if XR else None, cur_ptr, corner_ptr,
+                flags, prec)
             _lib.check(rc, h, "RGF.selected_solve")
             for X in (XL, XG, XR):
                 if X is not None:
@@ -289,3 +302,17 @@ class RGF(GFSolver):
         if return_current:
             return current
         return None
+
+
+def transmission(corner, gamma_l, gamma_r) -> torch.Tensor:
+    """T(E) = Tr[Gamma_R G^R_{N-1,0} Gamma_L (G^R_{N-1,0})^H] on the device.
+
+    ``corner`` from ``RGF.selected_solve(..., corner=...)``; the broadening
+    matrices are (*stack, b, b) stacks (Gamma = i (Sigma^R - Sigma^A)).
+    """
+    from quatrex_amd.ops import as_device, gemm
+
+    g = as_device(corner)
+    t = gemm(gemm(as_device(gamma_r, g.device), g), as_device(gamma_l, g.device))
+    t = gemm(t, g, op_b="C")
+    return torch.diagonal(t, dim1=-2, dim2=-1).sum(-1).real

@@ -209,3 +209,22 @@ def test_non_dense_blocks_staging(device):
     ref = orgf.RGF().selected_inv(oa)
     x = RGF().selected_inv(to_gpu(oa, device))
     assert close(x, ref, 1e-10)
+
+
+def test_corner_and_transmission(device, workload):
+    from oracle.rgf import transmission as o_transmission
+    from quatrex_amd.greens_function_solver import transmission
+
+    w, (oa, osl, osg), _ = workload
+    b0, bl, ne = w.block_sizes[0], w.block_sizes[-1], oa.data.shape[0]
+    ref = np.zeros((ne, bl, b0), complex)
+    orgf.RGF(16).selected_solve(oa, osl, osg, corner=ref)
+    corner = torch.zeros((ne, bl, b0), dtype=torch.complex128, device=device)
+    for mbs in (16, 3):
+        RGF(mbs).selected_solve(to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device),
+                                corner=corner)
+        assert rel_err(corner.cpu().numpy(), ref) < 1e-10
+    gl = 1j * (w.sigma_b_left - w.sigma_b_left.conj().T)
+    gr = 1j * (w.sigma_b_right - w.sigma_b_right.conj().T)
+    t = transmission(corner, np.broadcast_to(gl, (ne, b0, b0)), np.broadcast_to(gr, (ne, bl, bl)))
+    assert np.allclose(t.cpu().numpy(), o_transmission(ref, gl, gr), rtol=1e-10)
