@@ -9,12 +9,20 @@ in chunks of ``--chunk`` whose outputs (G^<, G^> with the BTD pattern of A,
 G^R block-diagonal) are written into one reused set of output stacks, since
 the outputs of all 512 energies (~240 GB) do not fit next to the inputs.
 
+The primary arithmetic (``--precision``, default complex128) gives
+``value``; every mode of ``--compare`` (default: mixed = complex64 products
+and inverses + one FP64 Newton-Schulz step per block inverse) is timed the
+same way and reported beside it with its speed-up and its error against the
+complex128 GPU result (per block, electron density diag G^<, transmission
+T(E) = Tr[Gamma_R G^R_{N-1,0} Gamma_L G^A_{0,N-1}]) on the first
+``--accuracy-energies`` energies.  The CPU baseline leg times the NumPy
+oracle on a bounded energy sample and compares both GPU modes with it on
+those energies.
+
 Printed: one JSON line with energies/s (whole job), ms/step, per-call
 latency percentiles, the roofline of the dominant kernel (the batched MFMA
 GEMM; achieved = algorithmic 8 m n k flops per launch / HIP-event launch
-time over the timed region) and the CPU baseline (the NumPy oracle, a
-restatement of the reference algorithm, timed on a bounded energy sample on
-the host cores).
+time over the timed region) and the CPU baseline.
 """
 
 import argparse
@@ -48,8 +56,11 @@ def host_cores() -> int:
     return max(1, n)
 
 
-def cpu_baseline(w, budget_s: float):
-    """Time the NumPy oracle (reference algorithm restated) on a few energies."""
+def cpu_baseline(w, budget_s: float, max_energies: int):
+    """Time the NumPy oracle (reference algorithm restated) on a few energies.
+
+    Returns (oracle outputs on the timed energies, baseline record).
+    """
     import numpy as np
     from threadpoolctl import threadpool_limits
 
@@ -89,12 +100,17 @@ def cpu_baseline(w, budget_s: float):
         t0 = time.perf_counter()
         solver.selected_solve(a, sl, sg, return_retarded=True)
         t1 = time.perf_counter() - t0
-        n = int(max(1, min(8, budget_s // max(t1, 1e-3))))
+        n = int(max(1, min(max_energies - 1, budget_s // max(t1, 1e-3))))
         a, sl, sg = containers(w.energies[1:1 + n])
+        corner = np.zeros((n, bs[-1], bs[0]), complex)
         t0 = time.perf_counter()
-        solver.selected_solve(a, sl, sg, return_retarded=True)
+        xl, xg, xr = solver.selected_solve(a, sl, sg, return_retarded=True, corner=corner)
         dt = time.perf_counter() - t0
-    return {
+    ref = {"energies": (1, 1 + n), "density": np.stack(
+               [np.diagonal(xl.blocks[i, i], axis1=-2, axis2=-1) for i in range(len(bs))], 1),
+           "diag_l": [xl.blocks[i, i] for i in range(len(bs))],
+           "corner": corner}
+    return ref, {
         "value": n / dt,
         "unit": "energies/s",
         "cores": cores,
@@ -106,16 +122,154 @@ def cpu_baseline(w, budget_s: float):
     }
 
 
+def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak):
+    """Warm-up + timed steps of one arithmetic mode; returns its record."""
+    import numpy as np
+    import torch
+
+    from quatrex_amd import _lib
+
+    def step(lat):
+        for va, vl, vg in views:
+            t = time.perf_counter()
+            rgf.selected_solve(va, vl, vg, out=out, return_retarded=True)
+            lat.append(time.perf_counter() - t)
+
+    for i in range(warmup):
+        t = time.perf_counter()
+        step([])
+        log(f"{rgf.precision}: warmup step {i} done ({time.perf_counter() - t:.2f} s)")
+    torch.cuda.synchronize()
+    lat = []
+    timer = _lib.KernelTimer(dev) if timer_on else None
+    if timer:
+        timer.__enter__()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(lat)
+        log(f"{rgf.precision}: timed step {i} done")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if timer:
+        timer.__exit__(None, None, None)
+    lat_ms = np.array(lat) * 1e3
+    rec = {"elapsed": elapsed,
+           "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
+                          "p99": round(float(np.percentile(lat_ms, 99)), 2)}}
+    if timer:
+        g, inv = timer.stats["gemm"], timer.stats["inv"]
+        # Sub-batches run on concurrent streams, so launches overlap: the GEMM
+        # throughput is its flops over the wall time during which at least one
+        # GEMM launch was in flight (union of the HIP-event intervals).
+        achieved = g["flops"] / (g["busy_ms"] / 1e3) / 1e12
+        per_launch = g["flops"] / max(g["launches"], 1)
+        avg_ms = g["ms"] / max(g["launches"], 1)
+        rec["roofline"] = {
+            "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+            "kernel": "zgemm_kernel (batched complex GEMM, 64x64 tiles, 3M split)",
+            "flops_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
+            "per_launch_tflops": round(per_launch / (avg_ms / 1e3) / 1e12, 3),
+            "busy_ms": round(g["busy_ms"], 2)}
+        rec["kernels"] = {
+            "gemm": {"launches": g["launches"], "sum_ms": round(g["ms"], 2),
+                     "busy_ms": round(g["busy_ms"], 2), "tflops": round(achieved, 3)},
+            "inverse": {"launches": inv["launches"], "sum_ms": round(inv["ms"], 2),
+                        "busy_ms": round(inv["busy_ms"], 2),
+                        "tflops": round(inv["flops"] / max(inv["busy_ms"], 1e-9) / 1e9, 3)},
+            "wall_ms": round(elapsed * 1e3, 2),
+        }
+    return rec
+
+
+def accuracy_outputs(w, rgf, a, sl, sg, n_acc):
+    """G^<, G^>, G^R blocks, density and transmission on energies [0, n_acc)."""
+    import numpy as np
+    import torch
+
+    from quatrex_amd.datastructures import DSDBCOO, empty_btd
+    from quatrex_amd.greens_function_solver import transmission
+
+    bs = w.block_sizes
+    va, vl, vg = a.stack_view(0, n_acc), sl.stack_view(0, n_acc), sg.stack_view(0, n_acc)
+    xl, xg = DSDBCOO.zeros_like(va), DSDBCOO.zeros_like(va)
+    xr, _ = empty_btd(bs, (n_acc,), offdiag=False, device=a.device)
+    corner = torch.zeros((n_acc, bs[-1], bs[0]), dtype=torch.complex128, device=a.device)
+    rgf.selected_solve(va, vl, vg, out=(xl, xg, xr), return_retarded=True, corner=corner)
+    gl = 1j * (w.sigma_b_left - w.sigma_b_left.conj().T)
+    gr = 1j * (w.sigma_b_right - w.sigma_b_right.conj().T)
+    t = transmission(corner, np.broadcast_to(gl, (n_acc,) + gl.shape),
+                     np.broadcast_to(gr, (n_acc,) + gr.shape))
+    dens = torch.stack([torch.diagonal(xl.blocks[i, i], dim1=-2, dim2=-1)
+                        for i in range(len(bs))], 1)
+    torch.cuda.synchronize()
+    return {"xl": xl, "xg": xg, "xr": xr, "corner": corner, "transmission": t, "density": dens}
+
+
+def rel(x, y):
+    import numpy as np
+    import torch
+
+    if isinstance(x, torch.Tensor):
+        return float(torch.linalg.norm(x - y) / torch.linalg.norm(y))
+    return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+
+
+def accuracy_report(w, test, ref):
+    """Errors of one mode against the complex128 GPU outputs."""
+    n = w.n_blocks
+    worst = {"lesser": 0.0, "greater": 0.0, "retarded_diag": 0.0}
+    for key, name in (("xl", "lesser"), ("xg", "greater")):
+        for i in range(n):
+            for j in (i - 1, i, i + 1):
+                if 0 <= j < n:
+                    worst[name] = max(worst[name], rel(test[key].blocks[i, j],
+                                                       ref[key].blocks[i, j]))
+    for i in range(n):
+        worst["retarded_diag"] = max(worst["retarded_diag"],
+                                     rel(test["xr"].blocks[i, i], ref["xr"].blocks[i, i]))
+    t, tr = test["transmission"], ref["transmission"]
+    return {"block_rel_max": {k: float(f"{v:.3e}") for k, v in worst.items()},
+            "density_rel": float(f"{rel(test['density'], ref['density']):.3e}"),
+            "transmission_rel_max": float(f"{float(((t - tr).abs() / tr.abs()).max()):.3e}")}
+
+
+def oracle_report(w, test, oref):
+    """Errors of one mode against the NumPy oracle on the CPU-sample energies."""
+    import numpy as np
+
+    from oracle.rgf import transmission as o_transmission
+
+    e0, e1 = oref["energies"]
+    n = w.n_blocks
+    worst = 0.0
+    for i in range(n):
+        worst = max(worst, rel(test["xl"].blocks[i, i][e0:e1].cpu().numpy(), oref["diag_l"][i]))
+    dens = test["density"][e0:e1].cpu().numpy()
+    gl = 1j * (w.sigma_b_left - w.sigma_b_left.conj().T)
+    gr = 1j * (w.sigma_b_right - w.sigma_b_right.conj().T)
+    t_ref = o_transmission(oref["corner"], gl, gr)
+    t = test["transmission"][e0:e1].cpu().numpy()
+    return {"energies": [e0, e1], "lesser_diag_block_rel_max": float(f"{worst:.3e}"),
+            "density_rel": float(f"{rel(dens, oref['density']):.3e}"),
+            "transmission_rel_max": float(f"{np.max(np.abs(t - t_ref) / np.abs(t_ref)):.3e}")}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="c128", choices=["c128", "c64"])
+    ap.add_argument("--precision", default="c128", choices=["c128", "c64", "mixed"])
+    ap.add_argument("--compare", default="mixed",
+                    help="comma-separated modes timed beside the primary one ('' for none)")
     ap.add_argument("--energies", type=int, default=512)
     ap.add_argument("--blocks", type=int, default=64)
     ap.add_argument("--block-size", type=int, default=256)
     ap.add_argument("--chunk", type=int, default=64)
+    ap.add_argument("--accuracy-energies", type=int, default=8,
+                    help="energies on which modes are compared (0 disables)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU work for the baseline sample (0 disables)")
@@ -129,10 +283,8 @@ def main():
               f"run one process per GPU on disjoint energy ranges.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
-    from quatrex_amd import _lib
     from quatrex_amd.datastructures import DSDBCOO, empty_btd
     from quatrex_amd.greens_function_solver import RGF
     from quatrex_amd.workloads import make_workload, solve_flops
@@ -141,7 +293,8 @@ def main():
     E, N, b, chunk = args.energies, args.blocks, args.block_size, args.chunk
     if E % chunk:
         raise SystemExit("--energies must be a multiple of --chunk")
-    log(f"workload: {E} energies, {N} blocks of {b}, precision {args.precision}, chunk {chunk}")
+    modes = [args.precision] + [m for m in args.compare.split(",") if m and m != args.precision]
+    log(f"workload: {E} energies, {N} blocks of {b}, modes {modes}, chunk {chunk}")
     w = make_workload(seed=args.seed, n_energies=E, n_blocks=N, block_size=b)
     t0 = time.perf_counter()
     a, sl, sg = w.assemble(dev)
@@ -151,94 +304,76 @@ def main():
     xl = DSDBCOO.zeros_like(a.stack_view(0, chunk))
     xg = DSDBCOO.zeros_like(a.stack_view(0, chunk))
     xr, _ = empty_btd(w.block_sizes, (chunk,), offdiag=False, device=dev)
-    rgf = RGF(max_batch_size=chunk, precision=args.precision)
     views = [(a.stack_view(c, c + chunk), sl.stack_view(c, c + chunk), sg.stack_view(c, c + chunk))
              for c in range(0, E, chunk)]
 
-    def step(lat):
-        for va, vl, vg in views:
-            t = time.perf_counter()
-            rgf.selected_solve(va, vl, vg, out=(xl, xg, xr), return_retarded=True)
-            lat.append(time.perf_counter() - t)
+    legs = {}
+    for m in modes:
+        rgf = RGF(max_batch_size=chunk, precision=m)
+        peak = PEAK_TFLOPS["c128" if m == "c128" else "c64"]
+        legs[m] = timed_leg(rgf, views, (xl, xg, xr), args.steps, args.warmup, dev,
+                            not args.no_timer, peak)
+        legs[m]["value"] = E * args.steps / legs[m]["elapsed"]
+        log(f"{m}: {legs[m]['value']:.1f} energies/s")
+    del xl, xg, xr
 
-    for i in range(args.warmup):
-        t = time.perf_counter()
-        step([])
-        log(f"warmup step {i} done ({time.perf_counter() - t:.2f} s)")
-    torch.cuda.synchronize()
-    lat = []
-    timer = None if args.no_timer else _lib.KernelTimer(dev)
-    if timer:
-        timer.__enter__()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(lat)
-        log(f"timed step {i} done")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    if timer:
-        timer.__exit__(None, None, None)
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = E * args.steps / elapsed
-    lat_ms = np.array(lat) * 1e3
+    n_acc = min(args.accuracy_energies, E)
+    acc = {}
+    if n_acc > 0:
+        for m in set(modes) | {"c128"}:
+            acc[m] = accuracy_outputs(w, RGF(max_batch_size=n_acc, precision=m), a, sl, sg, n_acc)
+        log("accuracy outputs done")
+    del a, sl, sg, views
+    torch.cuda.empty_cache()
 
-    roofline = None
-    kernels = None
-    if timer:
-        g, inv = timer.stats["gemm"], timer.stats["inv"]
-        # Sub-batches run on concurrent streams, so launches overlap: the GEMM
-        # throughput is its flops over the wall time during which at least one
-        # GEMM launch was in flight (union of the HIP-event intervals).
-        achieved = g["flops"] / (g["busy_ms"] / 1e3) / 1e12
-        peak = PEAK_TFLOPS[args.precision]
-        per_launch = g["flops"] / max(g["launches"], 1)
-        avg_ms = g["ms"] / max(g["launches"], 1)
-        roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-                    "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-                    "kernel": "zgemm_kernel (batched complex GEMM, 64x64 tiles)",
-                    "flops_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
-                    "per_launch_tflops": round(per_launch / (avg_ms / 1e3) / 1e12, 3),
-                    "busy_ms": round(g["busy_ms"], 2)}
-        kernels = {
-            "gemm": {"launches": g["launches"], "sum_ms": round(g["ms"], 2),
-                     "busy_ms": round(g["busy_ms"], 2), "tflops": round(achieved, 3)},
-            "inverse": {"launches": inv["launches"], "sum_ms": round(inv["ms"], 2),
-                        "busy_ms": round(inv["busy_ms"], 2),
-                        "tflops": round(inv["flops"] / max(inv["busy_ms"], 1e-9) / 1e9, 3)},
-            "wall_ms": round(elapsed * 1e3, 2),
-        }
+    oref, baseline = None, None
+    if args.cpu_budget > 0:
+        log("cpu baseline ...")
+        oref, baseline = cpu_baseline(w, args.cpu_budget, max(2, n_acc))
+        if n_acc < 2:
+            oref = None
+
+    prim = legs[args.precision]
     flops = solve_flops(N, b)
     result = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(prim["value"], 3),
         "unit": "energies/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 2),
+        "ms_per_step": round(1e3 * prim["elapsed"] / args.steps, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "c128" else "f32",
+        "dtype": {"c128": "f64", "c64": "f32", "mixed": "f32+f64"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"RGF selected_solve, {E} energies, {N} blocks of {b}, "
                                f"complex128 storage, {args.precision} arithmetic",
                    "energies": E, "n_blocks": N, "block_size": b, "precision": args.precision,
                    "energies_per_call": chunk},
-        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
-                       "p99": round(float(np.percentile(lat_ms, 99)), 2),
-                       "per": f"selected_solve call of {chunk} energies"},
-        "algorithmic_tflops": round(value * flops / 1e12, 3),
-        "roofline": roofline,
-        "kernels": kernels,
-        "cpu_baseline": None,
+        "latency_ms": dict(prim["latency_ms"], per=f"selected_solve call of {chunk} energies"),
+        "algorithmic_tflops": round(prim["value"] * flops / 1e12, 3),
+        "roofline": prim.get("roofline"),
+        "kernels": prim.get("kernels"),
+        "cpu_baseline": baseline,
     }
-    del a, sl, sg, xl, xg, xr, views
-    torch.cuda.empty_cache()
-    if args.cpu_budget > 0:
-        log("cpu baseline ...")
-        result["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+    modes_out = {}
+    for m in modes:
+        r = {"energies_per_s": round(legs[m]["value"], 3),
+             "speedup_vs_primary": round(legs[m]["value"] / prim["value"], 3),
+             "latency_ms": legs[m]["latency_ms"]}
+        if "roofline" in legs[m]:
+            r["gemm_tflops"] = legs[m]["roofline"]["achieved"]
+            r["inverse_busy_ms"] = legs[m]["kernels"]["inverse"]["busy_ms"]
+        if acc and m != "c128":
+            r["error_vs_c128_gpu"] = accuracy_report(w, acc[m], acc["c128"])
+        if acc and oref is not None:
+            r["error_vs_oracle"] = oracle_report(w, acc[m], oref)
+        modes_out[m] = r
+    result["modes"] = modes_out
+    if acc:
+        result["accuracy_energies"] = n_acc
     print(json.dumps(result), flush=True)
 
 

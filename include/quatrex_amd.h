@@ -39,6 +39,8 @@ extern "C" {
 /* arithmetic selector */
 #define QX_PREC_C128 0       /* complex128 on FP64 MFMA                      */
 #define QX_PREC_C64 1        /* complex64 on FP32 MFMA, FP64 in/out          */
+#define QX_PREC_MIXED 2      /* RGF only: complex64 products and inverses,   */
+                             /* one FP64 Newton-Schulz step per block inverse */
 
 /* GEMM operand ops */
 #define QX_OP_N 0

@@ -26,7 +26,11 @@ QX_ESINGULAR = -4
 
 PREC_C128 = 0
 PREC_C64 = 1
+PREC_MIXED = 2
 PRECISIONS = {"c128": PREC_C128, "complex128": PREC_C128, "c64": PREC_C64, "complex64": PREC_C64}
+# RGF also accepts the mixed mode (complex64 products, FP64 Newton-Schulz
+# step on every block inverse).
+RGF_PRECISIONS = dict(PRECISIONS, mixed=PREC_MIXED)
 
 OP_N = 0
 OP_C = 2

@@ -49,6 +49,20 @@ __global__ __launch_bounds__(NT) void axpby_kernel(AxpbyArgs g, int64_t total) {
   }
 }
 
+__global__ __launch_bounds__(NT) void ident_sub_kernel(z2* Y, int64_t sY, int ldy, int n,
+                                                       int64_t total) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * NT) {
+    const int64_t e = i / nn;
+    const int64_t rc = i - e * nn;
+    const int r = (int)(rc / n), c = (int)(rc - (int64_t)r * n);
+    z2* y = Y + e * sY + (int64_t)r * ldy + c;
+    const z2 v = *y;
+    *y = make_double2((r == c ? 1.0 : 0.0) - v.x, -v.y);
+  }
+}
+
 // Y = cy*Y + ca*A + cb*conj(B^T); Y, A are m x n, B is n x m.  Real coefs.
 struct HermArgs {
   int m, n;
@@ -277,6 +291,15 @@ void ew_axpby(hipStream_t s, int64_t batch, int m, int n, Cplx a, const Mat& A,
               Y.p, Y.stride, Y.ld};
   const int64_t total = batch * m * (int64_t)n;
   hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(total)), dim3(NT), 0, s, g, total);
+  QX_HIP_CHECK(hipGetLastError());
+}
+
+void ew_ident_sub(hipStream_t s, int64_t batch, int n, const Mat& Y) {
+  if (batch <= 0 || n <= 0) return;
+  QX_REQUIRE(!Y.zero(), "ident_sub: null matrix");
+  const int64_t total = batch * n * (int64_t)n;
+  hipLaunchKernelGGL(ident_sub_kernel, dim3(grid_for(total)), dim3(NT), 0, s, Y.p, Y.stride, Y.ld,
+                     n, total);
   QX_HIP_CHECK(hipGetLastError());
 }
 

@@ -7,12 +7,18 @@
 // Design
 // - One workgroup (256 threads = 4 waves of 64) owns a 64 x 64 complex tile
 //   of one matrix of the stack; each wave owns a 32 x 32 sub-tile made of
-//   2 x 2 MFMA 16x16 tiles, with separate real and imaginary accumulators.
-// - The complex product uses the 4M split on real MFMAs:
-//     Cr += Ar Br - Ai Bi ,  Ci += Ar Bi + Ai Br.
-// - Operands are staged through LDS split into real and imaginary planes,
-//   k-major, so that one MFMA operand fetch is 16 consecutive doubles.  The
-//   conjugate transpose is folded into the global->LDS copy (no extra pass).
+//   2 x 2 MFMA 16x16 tiles, with one accumulator set per real product.
+// - The complex product runs on real MFMAs through one of two splits:
+//     4M:  Cr += Ar Br - Ai Bi,  Ci += Ar Bi + Ai Br        (4 real MFMAs)
+//     3M:  P1 += Ar Br,  P2 += Ai Bi,  P3 += (Ar + Ai)(Br + Bi),
+//          Cr = P1 - P2,  Ci = P3 - P1 - P2                 (3 real MFMAs)
+//   3M saves a quarter of the matrix-core work; its imaginary part carries
+//   a cancellation, so its rounding error is a small constant factor above
+//   4M.  complex128 defaults to 4M, complex64 to 3M (QX_ZGEMM_SPLIT=3|4
+//   overrides both).
+// - Operands are staged through LDS as real planes (re, im, and re + im for
+//   3M), k-major, so that one MFMA operand fetch is 16 consecutive values.
+//   The conjugate transpose is folded into the global->LDS copy.
 // - The next K slice is prefetched into registers while the current one is
 //   multiplied (single LDS buffer, two barriers per slice).
 // - blockIdx is remapped so that the tiles of one matrix land on the same
@@ -26,6 +32,8 @@
 // A[row = l % 16][k = l / 16] and B[k = l / 16][col = l % 16].  Result
 // element e of lane l is D[l / 16 + 4 e][l % 16] for the f64 instruction and
 // D[4 (l / 16) + e][l % 16] for the f32 instruction.
+#include <cstdlib>
+
 #include "qx_internal.h"
 
 namespace qx {
@@ -83,12 +91,13 @@ struct MfmaTraits<float> {
   static __device__ inline int row(int lane, int e) { return 4 * (lane >> 4) + e; }
 };
 
-template <typename T, int OPA, int OPB>
-__global__ __launch_bounds__(NT) void zgemm_kernel(GemmArgs g) {
+template <typename T, int OPA, int OPB, int SPLIT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void zgemm_kernel(GemmArgs g) {
   typedef MfmaTraits<T> MT;
   typedef typename MT::acc_t acc_t;
-  __shared__ T As[2][BK][BM + PAD];
-  __shared__ T Bs[2][BK][BN + PAD];
+  constexpr int NP = SPLIT == 3 ? 3 : 2;  // LDS planes per operand
+  __shared__ T As[NP][BK][BM + PAD];
+  __shared__ T Bs[NP][BK][BN + PAD];
 
   const int id = xcd_remap(blockIdx.x, gridDim.x);
   const int tiles = g.tilesM * g.tilesN;
@@ -102,13 +111,15 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(GemmArgs g) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
 
-  acc_t accR[2][2], accI[2][2];
+  // 4M: acc1 = Cr, acc2 = Ci.  3M: acc1 = P1, acc2 = P2, acc3 = P3.
+  acc_t acc1[2][2], acc2[2][2], acc3[SPLIT == 3 ? 2 : 1][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      accR[i][j] = acc_t{0, 0, 0, 0};
-      accI[i][j] = acc_t{0, 0, 0, 0};
+      acc1[i][j] = acc_t{0, 0, 0, 0};
+      acc2[i][j] = acc_t{0, 0, 0, 0};
+      if constexpr (SPLIT == 3) acc3[i][j] = acc_t{0, 0, 0, 0};
     }
 
   z2 ra[4], rb[4];
@@ -148,21 +159,29 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(GemmArgs g) {
       const int idx = tid + NT * r;
       if (OPA == QX_OP_N) {
         const int row = idx >> 4, col = idx & 15;
-        As[0][col][row] = (T)ra[r].x;
-        As[1][col][row] = (T)ra[r].y;
+        const T re = (T)ra[r].x, im = (T)ra[r].y;
+        As[0][col][row] = re;
+        As[1][col][row] = im;
+        if constexpr (SPLIT == 3) As[NP - 1][col][row] = re + im;
       } else {
         const int kk = idx >> 6, mm = idx & 63;
-        As[0][kk][mm] = (T)ra[r].x;
-        As[1][kk][mm] = (T)(-ra[r].y);
+        const T re = (T)ra[r].x, im = (T)(-ra[r].y);
+        As[0][kk][mm] = re;
+        As[1][kk][mm] = im;
+        if constexpr (SPLIT == 3) As[NP - 1][kk][mm] = re + im;
       }
       if (OPB == QX_OP_N) {
         const int kk = idx >> 6, nn = idx & 63;
-        Bs[0][kk][nn] = (T)rb[r].x;
-        Bs[1][kk][nn] = (T)rb[r].y;
+        const T re = (T)rb[r].x, im = (T)rb[r].y;
+        Bs[0][kk][nn] = re;
+        Bs[1][kk][nn] = im;
+        if constexpr (SPLIT == 3) Bs[NP - 1][kk][nn] = re + im;
       } else {
         const int nn = idx >> 4, kk = idx & 15;
-        Bs[0][kk][nn] = (T)rb[r].x;
-        Bs[1][kk][nn] = (T)(-rb[r].y);
+        const T re = (T)rb[r].x, im = (T)(-rb[r].y);
+        Bs[0][kk][nn] = re;
+        Bs[1][kk][nn] = im;
+        if constexpr (SPLIT == 3) Bs[NP - 1][kk][nn] = re + im;
       }
     }
   };
@@ -176,30 +195,34 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(GemmArgs g) {
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
       const int kr = kk * 4 + (lane >> 4);
-      T a_r[2], a_i[2], na_i[2], b_r[2], b_i[2];
+      T a_r[2], a_i[2], a_s[2], b_r[2], b_i[2], b_s[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int mm = wm * 32 + i * 16 + (lane & 15);
         a_r[i] = As[0][kr][mm];
         a_i[i] = As[1][kr][mm];
-        na_i[i] = -a_i[i];
         const int nn = wn * 32 + i * 16 + (lane & 15);
         b_r[i] = Bs[0][kr][nn];
         b_i[i] = Bs[1][kr][nn];
+        if constexpr (SPLIT == 3) {
+          a_s[i] = As[NP - 1][kr][mm];
+          b_s[i] = Bs[NP - 1][kr][nn];
+        }
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          accR[i][j] = MT::mma(a_r[i], b_r[j], accR[i][j]);
-          accI[i][j] = MT::mma(a_r[i], b_i[j], accI[i][j]);
-        }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          accR[i][j] = MT::mma(na_i[i], b_i[j], accR[i][j]);
-          accI[i][j] = MT::mma(a_i[i], b_r[j], accI[i][j]);
+          if constexpr (SPLIT == 3) {
+            acc1[i][j] = MT::mma(a_r[i], b_r[j], acc1[i][j]);
+            acc2[i][j] = MT::mma(a_i[i], b_i[j], acc2[i][j]);
+            acc3[i][j] = MT::mma(a_s[i], b_s[j], acc3[i][j]);
+          } else {
+            acc1[i][j] = MT::mma(a_r[i], b_r[j], acc1[i][j]);
+            acc1[i][j] = MT::mma(-a_i[i], b_i[j], acc1[i][j]);
+            acc2[i][j] = MT::mma(a_r[i], b_i[j], acc2[i][j]);
+            acc2[i][j] = MT::mma(a_i[i], b_r[j], acc2[i][j]);
+          }
         }
     }
     __syncthreads();
@@ -219,7 +242,15 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(GemmArgs g) {
         const int row = m0 + wm * 32 + i * 16 + MT::row(lane, q);
         const int col = n0 + wn * 32 + j * 16 + (lane & 15);
         if (row < g.m && col < g.n) {
-          const double vr = (double)accR[i][j][q], vi = (double)accI[i][j][q];
+          double vr, vi;
+          if constexpr (SPLIT == 3) {
+            const double p1 = (double)acc1[i][j][q], p2 = (double)acc2[i][j][q];
+            vr = p1 - p2;
+            vi = (double)acc3[i][j][q] - p1 - p2;
+          } else {
+            vr = (double)acc1[i][j][q];
+            vi = (double)acc2[i][j][q];
+          }
           double outr = g.ar * vr - g.ai * vi, outi = g.ar * vi + g.ai * vr;
           z2* dst = C + (int64_t)row * g.ldc + col;
           if (!g.beta_zero) {
@@ -232,7 +263,7 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(GemmArgs g) {
       }
 }
 
-template <typename T>
+template <typename T, int SPLIT>
 void launch_typed(hipStream_t s, int64_t batch, int opA, int opB,
                   const GemmArgs& g) {
   const int64_t blocks = batch * g.tilesM * g.tilesN;
@@ -242,15 +273,28 @@ void launch_typed(hipStream_t s, int64_t batch, int opA, int opB,
   hipEvent_t ev0 = nullptr;
   if (timer) timer->begin(s, &ev0);
   if (opA == QX_OP_N && opB == QX_OP_N)
-    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_N, QX_OP_N>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_N, QX_OP_N, SPLIT>), grid, block, 0, s, g);
   else if (opA == QX_OP_N && opB == QX_OP_C)
-    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_N, QX_OP_C>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_N, QX_OP_C, SPLIT>), grid, block, 0, s, g);
   else if (opA == QX_OP_C && opB == QX_OP_N)
-    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_C, QX_OP_N>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_C, QX_OP_N, SPLIT>), grid, block, 0, s, g);
   else
-    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_C, QX_OP_C>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((zgemm_kernel<T, QX_OP_C, QX_OP_C, SPLIT>), grid, block, 0, s, g);
   QX_HIP_CHECK(hipGetLastError());
   if (timer) timer->end(s, ev0, KernelTimer::GEMM, 8.0 * g.m * g.n * (double)g.k * batch);
+}
+
+// Complex split of the GEMM: 4M for complex128, 3M for complex64, unless
+// QX_ZGEMM_SPLIT (3 or 4) is set.
+int gemm_split(int precision) {
+  static const int forced = []() {
+    const char* e = std::getenv("QX_ZGEMM_SPLIT");
+    if (!e) return 0;
+    const int v = std::atoi(e);
+    return (v == 3 || v == 4) ? v : 0;
+  }();
+  if (forced) return forced;
+  return precision == QX_PREC_C64 ? 3 : 4;
 }
 
 }  // namespace
@@ -288,10 +332,18 @@ void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
   g.tilesM = (m + BM - 1) / BM;
   g.tilesN = (n + BN - 1) / BN;
   g.beta_zero = beta_zero ? 1 : 0;
-  if (precision == QX_PREC_C64)
-    launch_typed<float>(s, batch, opA, opB, g);
-  else
-    launch_typed<double>(s, batch, opA, opB, g);
+  const int split = gemm_split(precision);
+  if (precision == QX_PREC_C64) {
+    if (split == 3)
+      launch_typed<float, 3>(s, batch, opA, opB, g);
+    else
+      launch_typed<float, 4>(s, batch, opA, opB, g);
+  } else {
+    if (split == 3)
+      launch_typed<double, 3>(s, batch, opA, opB, g);
+    else
+      launch_typed<double, 4>(s, batch, opA, opB, g);
+  }
 }
 
 }  // namespace qx

@@ -83,6 +83,33 @@ void fold(hipStream_t st, int64_t S, int m, int n, const Mat& a, double sign,
   ew_axpby(st, S, m, n, kOne, a, Cplx{sign, 0.0}, s, dst);
 }
 
+// Arithmetic of the block products: QX_PREC_MIXED runs them in complex64.
+int gemm_precision(int prec) {
+  QX_REQUIRE(prec == QX_PREC_C128 || prec == QX_PREC_C64 || prec == QX_PREC_MIXED,
+             "unknown precision selector");
+  return prec == QX_PREC_MIXED ? QX_PREC_C64 : prec;
+}
+
+// In-place block inverse X <- inv(X) in the arithmetic `prec` selects.
+// QX_PREC_MIXED inverts in complex64, then applies one Newton-Schulz step
+// X <- X + X (I - M X): the residual I - M X is formed by the FP64 GEMM (its
+// entries are of the order of the FP32 rounding error and would be lost in
+// FP32), the small correction X R by the FP32 GEMM.  Mc and R are n x n
+// temporaries.
+void block_inv(hipStream_t st, int64_t S, int n, const Mat& X, int* info, int prec,
+               const InvScratch& scr, const Mat& Mc, const Mat& R) {
+  if (prec != QX_PREC_MIXED) {
+    zinv(st, S, n, X, Mat(), X, info, prec, scr);
+    return;
+  }
+  ew_axpby(st, S, n, n, kOne, X, kZero, Mat(), Mc);  // M
+  zinv(st, S, n, X, Mat(), X, info, QX_PREC_C64, scr);
+  zgemm(st, S, n, n, n, QX_OP_N, QX_OP_N, kOne, Mc, X, kZero, R, QX_PREC_C128);
+  ew_ident_sub(st, S, n, R);                          // R = I - M X
+  ew_axpby(st, S, n, n, kOne, X, kZero, Mat(), Mc);  // X0
+  zgemm(st, S, n, n, n, QX_OP_N, QX_OP_N, kOne, Mc, R, kOne, X, QX_PREC_C64);
+}
+
 int block_limits(const qx_btd_layout* la, const char* what) {
   int bmax = 1;
   for (int i = 0; i < la->n_blocks; ++i) {
@@ -116,6 +143,7 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   int* info_all = h->ensure_info(Stot);
   const Btd A0(la, a), X0(lx, x);
 
+  const int gp = gemm_precision(prec);
   run_split(h, st0, Stot, [&](hipStream_t st, int64_t e0, int64_t S) {
     const Btd A = A0.sub(e0), X = X0.sub(e0);
     const Slots W{ws, Stot, (int64_t)bmax * bmax, e0};
@@ -125,20 +153,21 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     for (int i = 0; i < N; ++i) xd[i] = i;
     int spare = N;
     const int t1 = N + 1, t2 = N + 2, t3 = N + 3;
-    const InvScratch scr = W.inv(N + 4);
+    const InvScratch scr = W.inv(N + 4);  // N + 6, N + 7: Newton-Schulz temporaries
     auto bs = [&](int i) { return A.bs(i); };
 
     // Forward sweep (rgf.py:70-92).
     fold(st, S, bs(0), bs(0), A.diag(0), -1.0, OR.get(0, bs(0)), W.get(xd[0], bs(0)));
-    zinv(st, S, bs(0), W.get(xd[0], bs(0)), Mat(), W.get(xd[0], bs(0)), info, prec, scr);
+    block_inv(st, S, bs(0), W.get(xd[0], bs(0)), info, prec, scr, W.get(N + 6, bs(0)),
+              W.get(N + 7, bs(0)));
     for (int i = 0; i + 1 < N; ++i) {
       const int j = i + 1, bi = bs(i), bj = bs(j);
       const Mat T1 = W.get(t1, bi);
-      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get(xd[i], bi), kZero, T1, prec);
+      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get(xd[i], bi), kZero, T1, gp);
       const Mat Xj = W.get(xd[j], bj);
       fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xj);
-      zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xj, prec);
-      zinv(st, S, bj, Xj, Mat(), Xj, info, prec, scr);
+      zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xj, gp);
+      block_inv(st, S, bj, Xj, info, prec, scr, W.get(N + 6, bj), W.get(N + 7, bj));
     }
     // rgf.py:94-95
     if (!X.diag(N - 1).zero())
@@ -151,18 +180,18 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       const Mat Xii = W.get(xd[i], bi), Xjj = W.get(xd[j], bj);
       const Mat T1 = W.get(t1, bi), T2 = W.get(t2, bj);
       // x_ji = -x_jj a_ji x_ii
-      zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xjj, A.lower(i), kZero, T1, prec);
+      zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xjj, A.lower(i), kZero, T1, gp);
       Mat Xji = X.lower(i);
       if (Xji.zero()) Xji = W.get(t3, bi);
-      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, T1, Xii, kZero, Xji, prec);
+      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, T1, Xii, kZero, Xji, gp);
       // x_ij = -x_ii a_ij x_jj
-      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xii, A.upper(i), kZero, T2, prec);
+      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xii, A.upper(i), kZero, T2, gp);
       if (!X.upper(i).zero())
-        zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xjj, kZero, X.upper(i), prec);
+        zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xjj, kZero, X.upper(i), gp);
       // x_ii <- x_ii - x_ii a_ij x_ji
       const Mat Z = W.get(spare, bi);
       ew_axpby(st, S, bi, bi, kOne, Xii, kZero, Mat(), Z);
-      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xji, kOne, Z, prec);
+      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xji, kOne, Z, gp);
       std::swap(spare, xd[i]);
       if (!X.diag(i).zero())
         ew_axpby(st, S, bi, bi, kOne, W.get(xd[i], bi), kZero, Mat(), X.diag(i));
@@ -200,6 +229,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   const Btd A0(la, a), SL0(lsl, sl), SG0(lsg, sg), XL0(lxl, xl), XG0(lxg, xg),
       XR0(want_r ? lxr : nullptr, want_r ? xr : nullptr);
 
+  const int gp = gemm_precision(prec);
   run_split(h, st0, Stot, [&](hipStream_t st, int64_t e0, int64_t S) {
   Btd A = A0.sub(e0), SL = SL0.sub(e0), SG = SG0.sub(e0), XL = XL0.sub(e0), XG = XG0.sub(e0),
       XR = XR0.sub(e0);
@@ -225,6 +255,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
             T6s = 3 * N + 6, B1s = 3 * N + 7, B2s = 3 * N + 8, B3s = 3 * N + 9, B4s = 3 * N + 10,
             C1s = 3 * N + 11, C2s = 3 * N + 12, C3s = 3 * N + 13, C4s = 3 * N + 14;
   const InvScratch scr = W.inv(3 * N + 15);
+  const int NS1 = 3 * N + 20, NS2 = 3 * N + 21;  // Newton-Schulz temporaries
 
   struct Q {
     Btd* sig;
@@ -241,13 +272,13 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const int b0 = bs(0);
     const Mat X0 = W.get(sr[0], b0);
     fold(st, S, b0, b0, A.diag(0), -1.0, OR.get(0, b0), X0);
-    zinv(st, S, b0, X0, Mat(), X0, info, prec, scr);
+    block_inv(st, S, b0, X0, info, prec, scr, W.get(NS1, b0), W.get(NS2, b0));
     if (corner) ew_axpby(st, S, b0, b0, kOne, X0, kZero, Mat(), W.get(gc_cur, b0));
     for (auto& q : qs) {
       const Mat T3 = W.get(T3s, b0), T6 = W.get(T6s, b0);
       fold(st, S, b0, b0, q.sig->diag(0), 1.0, q.obc->get(0, b0), T3);
-      zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_N, kOne, X0, T3, kZero, T6, prec);
-      zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_C, kOne, T6, X0, kZero, W.get((*q.slot)[0], b0), prec);
+      zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_N, kOne, X0, T3, kZero, T6, gp);
+      zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_C, kOne, T6, X0, kZero, W.get((*q.slot)[0], b0), gp);
     }
   }
 
@@ -257,32 +288,32 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const Mat Xri = W.get(sr[i], bi), Xrj = W.get(sr[j], bj);
     const Mat T1 = W.get(T1s, bi);
     // a_ji xr_ii
-    zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), Xri, kZero, T1, prec);
+    zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), Xri, kZero, T1, gp);
     // xr_jj = inv(a_jj - obc - a_ji xr_ii a_ij)
     fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xrj);
-    zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xrj, prec);
-    zinv(st, S, bj, Xrj, Mat(), Xrj, info, prec, scr);
+    zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xrj, gp);
+    block_inv(st, S, bj, Xrj, info, prec, scr, W.get(NS1, bj), W.get(NS2, bj));
     if (corner) {
       // g_{j,0} = -x^R_jj (a_ji g_{i,0})
       zgemm(st, S, bj, b0, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get(gc_cur, b0), kZero,
-            W.get(gc_tmp, b0), prec);
+            W.get(gc_tmp, b0), gp);
       zgemm(st, S, bj, b0, bj, QX_OP_N, QX_OP_N, kMinus, Xrj, W.get(gc_tmp, b0), kZero,
-            W.get(gc_nxt, b0), prec);
+            W.get(gc_nxt, b0), gp);
       std::swap(gc_cur, gc_nxt);
     }
     for (auto& q : qs) {
       const Mat T3 = W.get(T3s, bj), T4 = W.get(T4s, bi), T5 = W.get(T5s, bj), T6 = W.get(T6s, bj);
       // sigma_jj + obc + a_ji x_ii a_ji^H + u^H - u,  u = a_ji xr_ii sigma_ij
       fold(st, S, bj, bj, q.sig->diag(j), 1.0, q.obc->get(j, bj), T3);
-      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get((*q.slot)[i], bi), kZero, T4, prec);
-      zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, A.lower(i), kOne, T3, prec);
+      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get((*q.slot)[i], bi), kZero, T4, gp);
+      zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, A.lower(i), kOne, T3, gp);
       const Mat sij = q.sig->upper(i);
       if (!sij.zero()) {
-        zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kOne, T1, sij, kZero, T5, prec);
+        zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kOne, T1, sij, kZero, T5, gp);
         ew_add_hdiff(st, S, bj, T5, T3);
       }
-      zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_N, kOne, Xrj, T3, kZero, T6, prec);
-      zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_C, kOne, T6, Xrj, kZero, W.get((*q.slot)[j], bj), prec);
+      zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_N, kOne, Xrj, T3, kZero, T6, gp);
+      zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_C, kOne, T6, Xrj, kZero, W.get((*q.slot)[j], bj), gp);
     }
   }
 
@@ -306,25 +337,25 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const Mat Xri = W.get(sr[i], bi), Xrj = W.get(sr[j], bj);
     const Mat Aij = A.upper(i), Aji = A.lower(i);
     const Mat B1 = W.get(B1s, bj), B2 = W.get(B2s, bi), B3 = W.get(B3s, bj), B4 = W.get(B4s, bi);
-    zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, Aij, kZero, B1, prec);  // xr_ii a_ij
-    zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xrj, Aji, kZero, B2, prec);  // xr_jj a_ji
-    zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kOne, B1, Xrj, kZero, B3, prec);   // xr_ii a_ij xr_jj
-    zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kOne, B1, B2, kZero, B4, prec);    // ... a_ji
+    zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, Aij, kZero, B1, gp);  // xr_ii a_ij
+    zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xrj, Aji, kZero, B2, gp);  // xr_jj a_ji
+    zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kOne, B1, Xrj, kZero, B3, gp);   // xr_ii a_ij xr_jj
+    zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kOne, B1, B2, kZero, B4, gp);    // ... a_ji
 
     if (want_cur) {
       // Meir-Wingreen layer current from the forward (left-connected) blocks
       // of layer i and the full blocks of layer j (rgf.py:381-400).
       const Mat C1 = W.get(C1s, bi), C2 = W.get(C2s, bj), C3 = W.get(C3s, bj), C4 = W.get(C4s, bj);
-      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, Aji, Xri, kZero, C1, prec);  // a_ji xr_ii
+      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, Aji, Xri, kZero, C1, gp);  // a_ji xr_ii
       Mat tl[2] = {C2, C3};
       for (int qi = 0; qi < 2; ++qi) {
         const Q& q = qs[qi];
         const Mat T4 = W.get(T4s, bi);
-        zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, Aji, W.get((*q.slot)[i], bi), kZero, T4, prec);
-        zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, Aji, kZero, tl[qi], prec);
+        zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, Aji, W.get((*q.slot)[i], bi), kZero, T4, gp);
+        zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, Aji, kZero, tl[qi], gp);
         const Mat sij = q.sig->upper(i);
         if (!sij.zero()) {
-          zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kOne, C1, sij, kZero, C4, prec);
+          zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kOne, C1, sij, kZero, C4, gp);
           ew_add_hdiff(st, S, bj, C4, tl[qi]);
         }
       }
@@ -338,26 +369,26 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       const Mat T1 = W.get(T1s, bi), T2 = W.get(T2s, bi), T3 = W.get(T3s, bj), T5 = W.get(T5s, bj);
       const Mat sij = q.sig->upper(i);
       // temp_1x = B4 x_ii - xr_ii s_ij B3^H ;  temp_1x -= temp_1x^H
-      zgemm(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xi, kZero, T1, prec);
+      zgemm(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xi, kZero, T1, gp);
       if (!sij.zero()) {
-        zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, sij, kZero, T5, prec);
-        zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, T5, B3, kOne, T1, prec);
+        zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, sij, kZero, T5, gp);
+        zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, T5, B3, kOne, T1, gp);
       }
       ew_antiherm(st, S, bi, 1.0, T1, T2);
       // temp_2x = B1 x_jj
-      zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kOne, B1, Xj, kZero, T3, prec);
+      zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kOne, B1, Xj, kZero, T3, gp);
       // x_ij = -temp_2x - x_ii B2^H + xr_ii s_ij xr_jj^H
       Mat Oij = q.out->upper(i);
       if (Oij.zero()) Oij = W.get(T4s, bj);
-      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, Xi, B2, kZero, Oij, prec);
+      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, Xi, B2, kZero, Oij, gp);
       ew_axpby(st, S, bi, bj, kOne, Oij, kMinus, T3, Oij);
       if (!sij.zero())
-        zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_C, kOne, T5, Xrj, kOne, Oij, prec);
+        zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_C, kOne, T5, Xrj, kOne, Oij, gp);
       if (!q.skip_lower && !q.out->lower(i).zero()) ew_neg_h(st, S, bi, bj, Oij, q.out->lower(i));
       // x_ii <- x_ii + temp_2x B1^H + temp_1x
       const Mat Z = W.get(spare, bi);
       ew_axpby(st, S, bi, bi, kOne, Xi, kOne, T2, Z);
-      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kOne, T3, B1, kOne, Z, prec);
+      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kOne, T3, B1, kOne, Z, gp);
       std::swap(spare, (*q.slot)[i]);
       if (!q.out->diag(i).zero()) ew_antiherm(st, S, bi, 0.5, Z, q.out->diag(i));
     }
@@ -366,11 +397,11 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     if (want_r && want_r_off) {
       if (!XR.upper(i).zero()) ew_axpby(st, S, bi, bj, kMinus, B3, kZero, Mat(), XR.upper(i));
       if (!XR.lower(i).zero())
-        zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, B2, Xri, kZero, XR.lower(i), prec);
+        zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, B2, Xri, kZero, XR.lower(i), gp);
     }
     const Mat Z = W.get(spare, bi);
     ew_axpby(st, S, bi, bi, kOne, Xri, kZero, Mat(), Z);
-    zgemm(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xri, kOne, Z, prec);
+    zgemm(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xri, kOne, Z, gp);
     std::swap(spare, sr[i]);
     if (want_r && !XR.diag(i).zero()) ew_axpby(st, S, bi, bi, kOne, Z, kZero, Mat(), XR.diag(i));
   }

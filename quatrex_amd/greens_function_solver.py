@@ -171,10 +171,11 @@ class RGF(GFSolver):
     max_batch_size : int
         Largest number of first-axis stack entries per batch (as the
         reference; batches may exceed it, see ``get_batches``).
-    precision : {"c128", "c64"}
+    precision : {"c128", "c64", "mixed"}
         Arithmetic of the block products and inverses: complex128 on FP64
-        matrix cores, or complex64 on FP32 matrix cores (inputs/outputs stay
-        complex128).
+        matrix cores; complex64 on FP32 matrix cores; or "mixed", complex64
+        products and inverses with one FP64 Newton-Schulz step on every block
+        inverse.  Inputs and outputs stay complex128 in every mode.
     retarded_offdiag : bool
         Also write the off-diagonal retarded blocks in ``selected_solve``.
     """
@@ -182,7 +183,7 @@ class RGF(GFSolver):
     def __init__(self, max_batch_size: int = 100, precision: str = "c128",
                  retarded_offdiag: bool = False) -> None:
         self.max_batch_size = max_batch_size
-        if precision not in _lib.PRECISIONS:
+        if precision not in _lib.RGF_PRECISIONS:
             raise ValueError(f"Unknown precision {precision!r}")
         self.precision = precision
         self.retarded_offdiag = retarded_offdiag
@@ -198,7 +199,7 @@ class RGF(GFSolver):
         x = out if out is not None else a.__class__.zeros_like(a)
         sizes, slices = get_batches(a.shape[0], self.max_batch_size)
         h, st = _lib.handle(dev), _lib.stream_ptr(dev)
-        prec = _lib.PRECISIONS[self.precision]
+        prec = _lib.RGF_PRECISIONS[self.precision]
         for b in range(len(sizes)):
             s0, s1 = int(slices[b]), int(slices[b + 1])
             keep: list = []
@@ -253,7 +254,7 @@ class RGF(GFSolver):
         if xg.symmetry:
             flags |= _lib.SOLVE_SKIP_LOWER_G
         h, st = _lib.handle(dev), _lib.stream_ptr(dev)
-        prec = _lib.PRECISIONS[self.precision]
+        prec = _lib.RGF_PRECISIONS[self.precision]
         inner = int(np.prod(a.stack_shape[1:], dtype=np.int64))
         for b in range(len(sizes)):
             s0, s1 = int(slices[b]), int(slices[b + 1])

@@ -146,6 +146,39 @@ def test_workload_c64_accuracy(device, workload):
         assert rel_err(xr.blocks[i, i].cpu().numpy(), rr.blocks[i, i]) < 1e-4
 
 
+def test_workload_mixed_accuracy(device, workload):
+    """Mixed mode (complex64 products, FP64 Newton-Schulz on the inverses)."""
+    w, (oa, osl, osg), (rl, rg, rr, _) = workload
+    a, sl, sg = to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device)
+    xl, xg, xr = RGF(16, precision="mixed").selected_solve(a, sl, sg, return_retarded=True)
+    for g, o in ((xl, rl), (xg, rg)):
+        assert _per_block_worst(w, g, o) < 1e-4
+    dens = np.stack([np.diagonal(xl.blocks[i, i].cpu().numpy(), axis1=-2, axis2=-1)
+                     for i in range(w.n_blocks)])
+    dens_ref = np.stack([np.diagonal(rl.blocks[i, i], axis1=-2, axis2=-1)
+                         for i in range(w.n_blocks)])
+    assert rel_err(dens, dens_ref) < 1e-5
+
+
+def test_mixed_newton_schulz_single_block(device):
+    """With one block the mixed mode is one complex64 inverse plus one FP64
+    Newton-Schulz step: the result must be FP64-accurate, where plain complex64
+    stays at FP32 accuracy."""
+    rng = np.random.default_rng(3)
+    n, ne = 96, 6
+    m = (rng.standard_normal((ne, n, n)) + 1j * rng.standard_normal((ne, n, n))) / np.sqrt(n)
+    m += 3.0 * np.eye(n)
+    ref = np.linalg.inv(m)
+    oa = ODSDBCOO.from_sparray(sps.coo_matrix(m[0]), [n], (ne,))
+    oa.data[:] = m.reshape(ne, -1)
+    errs = {}
+    for prec in ("c64", "mixed"):
+        x = RGF(precision=prec).selected_inv(to_gpu(oa, device))
+        errs[prec] = rel_err(x.blocks[0, 0].cpu().numpy(), ref)
+    assert errs["c64"] > 1e-9
+    assert errs["mixed"] < 1e-12, errs
+
+
 def test_obc_blocks_path(device, workload):
     w, (oa, osl, osg), _ = workload
     n, b, ne = w.n_blocks, w.block_sizes[0], oa.data.shape[0]
