@@ -9,9 +9,10 @@ in chunks of ``--chunk`` whose outputs (G^<, G^> with the BTD pattern of A,
 G^R block-diagonal) are written into one reused set of output stacks, since
 the outputs of all 512 energies (~240 GB) do not fit next to the inputs.
 
-The primary arithmetic (``--precision``, default complex128) gives
-``value``; every mode of ``--compare`` (default: mixed = complex64 products
-and inverses + one FP64 Newton-Schulz step per block inverse) is timed the
+The primary arithmetic (``--precision``, default complex128 with the 4M
+GEMM split) gives ``value``; every mode of ``--compare`` (default:
+c128-3m = complex128 with the 3M split, and mixed = complex64 products and
+inverses + one FP64 Newton-Schulz step per block inverse) is timed the
 same way and reported beside it with its speed-up and its error against the
 complex128 GPU result (per block, electron density diag G^<, transmission
 T(E) = Tr[Gamma_R G^R_{N-1,0} Gamma_L G^A_{0,N-1}]) on the first
@@ -37,7 +38,8 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 METRIC = "RGF energies/s at 64 blocks×256, mixed vs c128 GPU; p50/p99 solve latency"
-PEAK_TFLOPS = {"c128": 78.6, "c64": 157.3}  # MI355X dense FP64 / FP32 matrix peaks
+# MI355X dense matrix peaks of the GEMM arithmetic each mode runs on
+PEAK_TFLOPS = {"c128": 78.6, "c128-3m": 78.6, "c64": 157.3, "mixed": 157.3}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -168,7 +170,7 @@ def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak):
         rec["roofline"] = {
             "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-            "kernel": "zgemm_kernel (batched complex GEMM, 64x64 tiles, 3M split)",
+            "kernel": "zgemm_kernel (batched complex GEMM, 64x64 tiles)",
             "flops_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
             "per_launch_tflops": round(per_launch / (avg_ms / 1e3) / 1e12, 3),
             "busy_ms": round(g["busy_ms"], 2)}
@@ -256,13 +258,99 @@ def oracle_report(w, test, oref):
             "transmission_rel_max": float(f"{np.max(np.abs(t - t_ref) / np.abs(t_ref)):.3e}")}
 
 
+def obc_workload(args):
+    """Open-boundary lead solves batched over energies (one JSON line).
+
+    Lead principal layer of b x b (H00 Hermitian, H01 ~ CN / sqrt(b)),
+    M = (E + i eta) I - H over ``--energies`` energies.  Times the surface
+    Green's function by Sancho-Rubio (FP64, tolerance 1e-12) and by the
+    Spectral OBC on Beyn's contour method (annulus r_i = 1 / r_o, 64 nodes,
+    32 probe vectors), compares the two, and checks Sancho-Rubio against
+    the NumPy oracle on a few energies.
+    """
+    import numpy as np
+    import torch
+
+    from quatrex_amd import _lib
+    from quatrex_amd.boundary_conditions import SanchoRubio, Spectral
+    from quatrex_amd.nevp import Beyn
+    from quatrex_amd.workloads import lead_blocks
+
+    dev = torch.device("cuda:0")
+    ne, b = args.energies, args.block_size
+    m_ji, m_ii, m_ij, _ = lead_blocks(seed=args.seed, n_energies=ne, block_size=b)
+    a_ji, a_ii, a_ij = (torch.as_tensor(x, device=dev) for x in (m_ji, m_ii, m_ij))
+    torch.cuda.synchronize()
+    log(f"obc workload: {ne} energies, lead layer {b}, resident")
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - t0) / args.steps
+
+    sr = SanchoRubio(max_iterations=200, convergence_tol=1e-12)
+    timer = _lib.KernelTimer(dev)
+    with timer:
+        x_sr, t_sr = timed(lambda: sr(a_ii, a_ij, a_ji))
+    g = timer.stats["gemm"]
+    log(f"sancho-rubio: {t_sr * 1e3:.1f} ms per batch, {sr.iterations} iterations")
+    r_o = args.beyn_radius
+    spec = Spectral(Beyn(r_o=r_o, r_i=1.0 / r_o, m_0=32, num_quad_points=64))
+    x_sp, t_sp = timed(lambda: spec(a_ii, a_ij, a_ji))
+    log(f"spectral/beyn: {t_sp * 1e3:.1f} ms per batch")
+    from quatrex_amd.ops import inv, mm
+
+    res_sr = torch.linalg.norm(x_sr - inv(a_ii - mm(a_ji, x_sr, a_ij)), dim=(-2, -1))
+    res_sr = (res_sr / torch.linalg.norm(x_sr, dim=(-2, -1))).max().item()
+    diff = torch.linalg.norm(x_sp - x_sr, dim=(-2, -1)) / torch.linalg.norm(x_sr, dim=(-2, -1))
+    oracle = None
+    if args.cpu_budget > 0:
+        import oracle.obc as oobc
+
+        k = 2
+        t0 = time.perf_counter()
+        x_o = oobc.SanchoRubio(max_iterations=200, convergence_tol=1e-12)(
+            m_ii[:k], m_ij[:k], m_ji[:k])
+        t_o = time.perf_counter() - t0
+        oracle = {"energies": k, "sancho_rubio_rel_err": float(
+            np.linalg.norm(x_sr[:k].cpu().numpy() - x_o) / np.linalg.norm(x_o)),
+            "cpu_s_per_energy": round(t_o / k, 3), "cores": host_cores()}
+    flops_sr = sr.iterations * 7 * 8.0 * b ** 3 * ne
+    result = {
+        "metric": "OBC lead solves: energies/s at 1024 energies x 256 (Sancho-Rubio, Beyn)",
+        "value": round(ne / t_sr, 3),
+        "unit": "energies/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(t_sr * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"lead surface Green's function, {ne} energies, layer {b}",
+                   "energies": ne, "block_size": b, "sancho_rubio_tol": 1e-12,
+                   "beyn": {"r_o": r_o, "r_i": 1.0 / r_o, "quad_points": 64, "m_0": 32}},
+        "sancho_rubio": {"energies_per_s": round(ne / t_sr, 3), "iterations": sr.iterations,
+                         "max_fixed_point_residual": float(f"{res_sr:.3e}"),
+                         "algorithmic_tflops": round(flops_sr / t_sr / 1e12, 3),
+                         "gemm_tflops": round(g["flops"] / max(g["busy_ms"], 1e-9) / 1e9, 3)},
+        "spectral_beyn": {"energies_per_s": round(ne / t_sp, 3),
+                          "rel_diff_vs_sancho_rubio": {
+                              "median": float(f"{diff.median().item():.3e}"),
+                              "max": float(f"{diff.max().item():.3e}")}},
+        "oracle": oracle,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="c128", choices=["c128", "c64", "mixed"])
-    ap.add_argument("--compare", default="mixed",
+    ap.add_argument("--precision", default="c128", choices=list(PEAK_TFLOPS))
+    ap.add_argument("--compare", default="c128-3m,mixed",
                     help="comma-separated modes timed beside the primary one ('' for none)")
     ap.add_argument("--energies", type=int, default=512)
     ap.add_argument("--blocks", type=int, default=64)
@@ -275,6 +363,11 @@ def main():
                     help="seconds of CPU work for the baseline sample (0 disables)")
     ap.add_argument("--no-timer", action="store_true",
                     help="do not bracket kernels with HIP events (no roofline)")
+    ap.add_argument("--workload", default="rgf", choices=["rgf", "obc"],
+                    help="rgf: the selected solve (default); obc: lead surface Green's "
+                         "functions (use --energies 1024 --block-size 256)")
+    ap.add_argument("--beyn-radius", type=float, default=2.0,
+                    help="outer contour radius of the Beyn leg (inner = 1 / radius)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -282,6 +375,10 @@ def main():
               f"(--gpus {args.gpus} requested). Energies shard with no communication; "
               f"run one process per GPU on disjoint energy ranges.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "obc":
+        obc_workload(args)
+        return
 
     import torch
 
@@ -310,7 +407,7 @@ def main():
     legs = {}
     for m in modes:
         rgf = RGF(max_batch_size=chunk, precision=m)
-        peak = PEAK_TFLOPS["c128" if m == "c128" else "c64"]
+        peak = PEAK_TFLOPS[m]
         legs[m] = timed_leg(rgf, views, (xl, xg, xr), args.steps, args.warmup, dev,
                             not args.no_timer, peak)
         legs[m]["value"] = E * args.steps / legs[m]["elapsed"]
@@ -346,7 +443,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"c128": "f64", "c64": "f32", "mixed": "f32+f64"}[args.precision],
+        "dtype": {"c128": "f64", "c128-3m": "f64", "c64": "f32",
+                  "mixed": "f32+f64"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"RGF selected_solve, {E} energies, {N} blocks of {b}, "
                                f"complex128 storage, {args.precision} arithmetic",

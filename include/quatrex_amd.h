@@ -41,6 +41,9 @@ extern "C" {
 #define QX_PREC_C64 1        /* complex64 on FP32 MFMA, FP64 in/out          */
 #define QX_PREC_MIXED 2      /* RGF only: complex64 products and inverses,   */
                              /* one FP64 Newton-Schulz step per block inverse */
+#define QX_PREC_C128_3M 3    /* complex128 on FP64 MFMA with the 3M product   */
+                             /* split (3/4 of the matrix-core work, rounding  */
+                             /* error a small factor above the 4M split)      */
 
 /* GEMM operand ops */
 #define QX_OP_N 0
@@ -164,6 +167,18 @@ int qx_beyn_moments(qx_handle* handle, void* stream, int64_t n, int b,
                     int n_quad, double r_o, double r_i, const void* a_ji,
                     const void* a_ii, const void* a_ij, void* q0, void* q1,
                     int precision);
+
+/* Contact (boundary) self-energies of one lead.  Replaces the assembly of
+ * src/quatrex/electron/solver.py:324-337 (left) and :370-386 (right):
+ *   sigma_r = m10 g m01,  Gamma = i (sigma_r - sigma_r^H),
+ *   sigma_l = i f Gamma,  sigma_g = i (f - 1) Gamma
+ * (scale_stack, src/qttools/utils/stack_utils.py:6-30), with one occupation
+ * f per matrix (occupancies: n doubles in device memory).  All matrices
+ * (n, b, b) contiguous. */
+int qx_contact_self_energy(qx_handle* handle, void* stream, int64_t n, int b,
+                           const void* m10, const void* g, const void* m01,
+                           const double* occupancies, void* sigma_r,
+                           void* sigma_l, void* sigma_g);
 
 #ifdef __cplusplus
 }

@@ -4,6 +4,7 @@ Drop-in replacements for the reference's solver API (qttools):
 
 - ``quatrex_amd.datastructures.DSDBCOO``          block-sorted COO stacks
 - ``quatrex_amd.greens_function_solver.RGF``      selected_inv / selected_solve
+- ``quatrex_amd.greens_function_solver.Inv``      dense reference solver (small systems)
 - ``quatrex_amd.boundary_conditions.SanchoRubio`` / ``Spectral`` / ``OBCSystem``
 - ``quatrex_amd.nevp.Beyn`` / ``Full``
 

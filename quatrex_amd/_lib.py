@@ -27,7 +27,9 @@ QX_ESINGULAR = -4
 PREC_C128 = 0
 PREC_C64 = 1
 PREC_MIXED = 2
-PRECISIONS = {"c128": PREC_C128, "complex128": PREC_C128, "c64": PREC_C64, "complex64": PREC_C64}
+PREC_C128_3M = 3
+PRECISIONS = {"c128": PREC_C128, "complex128": PREC_C128, "c64": PREC_C64, "complex64": PREC_C64,
+              "c128-3m": PREC_C128_3M}
 # RGF also accepts the mixed mode (complex64 products, FP64 Newton-Schulz
 # step on every block inverse).
 RGF_PRECISIONS = dict(PRECISIONS, mixed=PREC_MIXED)
@@ -45,6 +47,7 @@ EXPORTED_SYMBOLS = (
     "qx_abi_version", "qx_create", "qx_destroy", "qx_last_error", "qx_workspace_query",
     "qx_sync_check", "qx_rgf_selected_inv", "qx_rgf_selected_solve", "qx_zgemm_batched",
     "qx_inv_batched", "qx_sancho_rubio", "qx_beyn_moments", "qx_timer_start", "qx_timer_stop",
+    "qx_contact_self_energy",
 )
 
 
@@ -114,6 +117,7 @@ def load() -> ctypes.CDLL:
         lib.qx_sancho_rubio.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, i32, f64,
                                         ctypes.POINTER(ctypes.c_int)]
         lib.qx_beyn_moments.argtypes = [vp, vp, i64, i32, i32, f64, f64, vp, vp, vp, vp, vp, i32]
+        lib.qx_contact_self_energy.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, vp, vp, vp]
         lib.qx_timer_start.argtypes = [vp, vp]
         lib.qx_timer_stop.argtypes = [vp, vp, ctypes.POINTER(f64)]
         for name in EXPORTED_SYMBOLS:

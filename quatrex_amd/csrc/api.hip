@@ -481,4 +481,27 @@ int qx_beyn_moments(qx_handle* h, void* stream, int64_t n, int b, int n_quad, do
   });
 }
 
+int qx_contact_self_energy(qx_handle* h, void* stream, int64_t n, int b, const void* m10,
+                           const void* g, const void* m01, const double* occupancies,
+                           void* sigma_r, void* sigma_l, void* sigma_g) {
+  return guarded(h, [&]() {
+    QX_REQUIRE(h, "null handle");
+    QX_REQUIRE(n >= 0 && b >= 1, "contact_self_energy: bad sizes");
+    QX_REQUIRE(m10 && g && m01 && occupancies && sigma_r && sigma_l && sigma_g,
+               "contact_self_energy: null argument");
+    if (n == 0) return QX_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t bb = (int64_t)b * b;
+    qx::z2* t = static_cast<qx::z2*>(h->ensure_ws((size_t)(n * bb) * sizeof(qx::z2)));
+    const qx::Cplx one{1.0, 0.0}, zero{0.0, 0.0};
+    qx::zgemm(st, n, b, b, b, QX_OP_N, QX_OP_N, one, qx::Mat(g, bb, b), qx::Mat(m01, bb, b), zero,
+              qx::Mat(t, bb, b), QX_PREC_C128);
+    qx::zgemm(st, n, b, b, b, QX_OP_N, QX_OP_N, one, qx::Mat(m10, bb, b), qx::Mat(t, bb, b),
+              zero, qx::Mat(sigma_r, bb, b), QX_PREC_C128);
+    qx::ew_contact_sigma(st, n, b, qx::Mat(sigma_r, bb, b), occupancies,
+                         qx::Mat(sigma_l, bb, b), qx::Mat(sigma_g, bb, b));
+    return QX_OK;
+  });
+}
+
 }  // extern "C"

@@ -132,6 +132,39 @@ void herm(hipStream_t s, int64_t batch, int m, int n, double cy, double ca,
   QX_HIP_CHECK(hipGetLastError());
 }
 
+// Contact self-energies from Sigma^R (one 32 x 32 tile per workgroup, the
+// transposed tile through LDS): D = S - S^H, L = -f D, G = (1 - f) D with
+// one occupation f per matrix (i f Gamma and i (f - 1) Gamma for
+// Gamma = i (S - S^H)).
+__global__ __launch_bounds__(NT) void contact_sigma_kernel(const z2* S, int64_t sS, int n,
+                                                           const double* f, z2* L, int64_t sL,
+                                                           z2* G, int64_t sG, int tn) {
+  __shared__ z2 tile[TS][TS + 1];
+  const int tiles = tn * tn;
+  const int64_t e = blockIdx.x / tiles;
+  const int t = blockIdx.x - (int)(e * tiles);
+  const int r0 = (t / tn) * TS, c0 = (t % tn) * TS;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const z2* Se = S + e * sS;
+  for (int j = ty; j < TS; j += 8) {
+    const int sr = c0 + j, sc = r0 + tx;
+    z2 v = make_double2(0.0, 0.0);
+    if (sr < n && sc < n) v = Se[(int64_t)sr * n + sc];
+    tile[j][tx] = v;  // S[c0 + j][r0 + tx]
+  }
+  __syncthreads();
+  const double fe = f[e];
+  for (int i = ty; i < TS; i += 8) {
+    const int r = r0 + i, c = c0 + tx;
+    if (r >= n || c >= n) continue;
+    const z2 a = Se[(int64_t)r * n + c];
+    const z2 b = tile[tx][i];  // S[c][r]
+    const double dr = a.x - b.x, di = a.y + b.y;
+    L[e * sL + (int64_t)r * n + c] = make_double2(-fe * dr, -fe * di);
+    G[e * sG + (int64_t)r * n + c] = make_double2((1.0 - fe) * dr, (1.0 - fe) * di);
+  }
+}
+
 // Block reduction of NT doubles in a fixed order.
 __device__ inline double block_sum(double v, double* red) {
 #pragma unroll
@@ -291,6 +324,19 @@ void ew_axpby(hipStream_t s, int64_t batch, int m, int n, Cplx a, const Mat& A,
               Y.p, Y.stride, Y.ld};
   const int64_t total = batch * m * (int64_t)n;
   hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(total)), dim3(NT), 0, s, g, total);
+  QX_HIP_CHECK(hipGetLastError());
+}
+
+void ew_contact_sigma(hipStream_t s, int64_t batch, int n, const Mat& S, const double* f,
+                      const Mat& L, const Mat& G) {
+  if (batch <= 0 || n <= 0) return;
+  QX_REQUIRE(!S.zero() && !L.zero() && !G.zero() && f, "contact_sigma: null argument");
+  QX_REQUIRE(S.ld == n && L.ld == n && G.ld == n, "contact_sigma: matrices must be dense n x n");
+  const int tn = (n + TS - 1) / TS;
+  const int64_t blocks = batch * tn * tn;
+  QX_REQUIRE(blocks < (int64_t(1) << 31), "contact_sigma: grid too large");
+  hipLaunchKernelGGL(contact_sigma_kernel, dim3((unsigned)blocks), dim3(NT), 0, s, S.p, S.stride,
+                     n, f, L.p, L.stride, G.p, G.stride, tn);
   QX_HIP_CHECK(hipGetLastError());
 }
 

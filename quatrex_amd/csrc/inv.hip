@@ -462,6 +462,9 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const
       (A.p == start.p) && S.zero() && A.ld == start.ld && A.stride == start.stride;
   if (!in_place) ew_axpby(s, batch, n, n, Cplx{1.0, 0.0}, A, Cplx{-1.0, 0.0}, S, start);
 
+  QX_REQUIRE(precision == QX_PREC_C128 || precision == QX_PREC_C64 ||
+                 precision == QX_PREC_C128_3M,
+             "inv: precision must be QX_PREC_C128, QX_PREC_C64 or QX_PREC_C128_3M");
   if (precision == QX_PREC_C64) {
     if (big) run_inverse<float, 16, 2>(s, batch, n, X, scr, s0, info);
     else run_inverse<float, 32, 1>(s, batch, n, X, scr, s0, info);

@@ -78,6 +78,10 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S,
 void ew_axpby(hipStream_t s, int64_t batch, int m, int n, Cplx a, const Mat& A,
               Cplx b, const Mat& B, const Mat& Y);
 // Y = scale * (X - X^H)    (square)
+// Contact self-energies: L = -f (S - S^H), G = (1 - f)(S - S^H), one f per
+// matrix (device array).
+void ew_contact_sigma(hipStream_t s, int64_t batch, int n, const Mat& S, const double* f,
+                      const Mat& L, const Mat& G);
 // Y = I - Y for a stack of n x n matrices.
 void ew_ident_sub(hipStream_t s, int64_t batch, int n, const Mat& Y);
 void ew_antiherm(hipStream_t s, int64_t batch, int n, double scale,

@@ -85,7 +85,8 @@ void fold(hipStream_t st, int64_t S, int m, int n, const Mat& a, double sign,
 
 // Arithmetic of the block products: QX_PREC_MIXED runs them in complex64.
 int gemm_precision(int prec) {
-  QX_REQUIRE(prec == QX_PREC_C128 || prec == QX_PREC_C64 || prec == QX_PREC_MIXED,
+  QX_REQUIRE(prec == QX_PREC_C128 || prec == QX_PREC_C64 || prec == QX_PREC_MIXED ||
+                 prec == QX_PREC_C128_3M,
              "unknown precision selector");
   return prec == QX_PREC_MIXED ? QX_PREC_C64 : prec;
 }
@@ -99,7 +100,7 @@ int gemm_precision(int prec) {
 void block_inv(hipStream_t st, int64_t S, int n, const Mat& X, int* info, int prec,
                const InvScratch& scr, const Mat& Mc, const Mat& R) {
   if (prec != QX_PREC_MIXED) {
-    zinv(st, S, n, X, Mat(), X, info, prec, scr);
+    zinv(st, S, n, X, Mat(), X, info, prec == QX_PREC_C64 ? QX_PREC_C64 : QX_PREC_C128, scr);
     return;
   }
   ew_axpby(st, S, n, n, kOne, X, kZero, Mat(), Mc);  // M

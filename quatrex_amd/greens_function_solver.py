@@ -171,11 +171,13 @@ class RGF(GFSolver):
     max_batch_size : int
         Largest number of first-axis stack entries per batch (as the
         reference; batches may exceed it, see ``get_batches``).
-    precision : {"c128", "c64", "mixed"}
+    precision : {"c128", "c128-3m", "c64", "mixed"}
         Arithmetic of the block products and inverses: complex128 on FP64
-        matrix cores; complex64 on FP32 matrix cores; or "mixed", complex64
-        products and inverses with one FP64 Newton-Schulz step on every block
-        inverse.  Inputs and outputs stay complex128 in every mode.
+        matrix cores (4M complex split; "c128-3m": 3M split, a quarter less
+        matrix-core work, rounding error a small factor larger); complex64
+        on FP32 matrix cores; or "mixed", complex64 products and inverses
+        with one FP64 Newton-Schulz step on every block inverse.  Inputs and
+        outputs stay complex128 in every mode.
     retarded_offdiag : bool
         Also write the off-diagonal retarded blocks in ``selected_solve``.
     """
@@ -303,6 +305,117 @@ class RGF(GFSolver):
         if return_current:
             return current
         return None
+
+
+class Inv(GFSolver):
+    """Dense selected inversion / solve on the HIP path (inv.py:10-227).
+
+    Densifies the stack, folds the boundary blocks into the dense matrices,
+    inverts them with the batched HIP inverse, forms X^<> = X Sigma^<> X^H
+    with the HIP GEMM and gathers the entries of the output pattern.  As the
+    reference, it is meant for checks on small systems, not production:
+    the dense size (sum of the block sizes) is limited to 512 here.
+
+    Reference behaviour kept: ``selected_solve`` subtracts the lesser /
+    greater boundary blocks (inv.py:199-202, where RGF adds them), always
+    returns its outputs (inv.py:224-227), raises
+    ``ValueError("Missing output for the retarded Green's function.")``
+    (inv.py:172-176) and ``NotImplementedError`` for ``return_current``
+    (inv.py:148-151).  Boundary blocks fold on the last two axes, so stacks
+    of any dimension work.
+    """
+
+    MAX_DENSE = 512
+
+    def __init__(self, max_batch_size: int = 100) -> None:
+        self.max_batch_size = max_batch_size
+
+    def _dense(self, m: DSDBCOO, sl: slice) -> torch.Tensor:
+        if not isinstance(m, DSDBCOO):
+            raise TypeError("quatrex_amd solvers take quatrex_amd.datastructures.DSDBCOO")
+        _lib.require_device_tensor(m.data, "DSDBCOO.data")
+        return m.stack_view(sl.start, sl.stop).to_dense()
+
+    def _check(self, a: DSDBCOO) -> None:
+        n = int(np.sum(a.block_sizes))
+        if n > self.MAX_DENSE:
+            raise ValueError(f"Inv densifies the matrix: dense size {n} exceeds "
+                             f"{self.MAX_DENSE}; use RGF")
+
+    @staticmethod
+    def _fold(dense, blocks, offsets, sl, sign):
+        for j, blk in enumerate(blocks):
+            if blk is None:
+                continue
+            r = slice(int(offsets[j]), int(offsets[j + 1]))
+            dense[..., r, r] += sign * torch.as_tensor(blk)[sl].to(dense.device, dense.dtype)
+
+    @staticmethod
+    def _gather(out: DSDBCOO, dense: torch.Tensor, sl: slice) -> None:
+        r = torch.as_tensor(out.rows, dtype=torch.long, device=dense.device)
+        c = torch.as_tensor(out.cols, dtype=torch.long, device=dense.device)
+        out.data[sl] = dense[..., r, c]
+
+    # inv.py:32-97
+    def selected_inv(self, a: DSDBCOO, obc_blocks: OBCBlocks | None = None,
+                     out: DSDBCOO | None = None):
+        from quatrex_amd.ops import inv
+
+        self._check(a)
+        sizes, slices = get_batches(a.shape[0], self.max_batch_size)
+        if obc_blocks is None:
+            obc_blocks = OBCBlocks(num_blocks=a.num_blocks)
+        return_out = out is None
+        if out is None:
+            out = a.__class__.zeros_like(a)
+        offs = np.concatenate([[0], np.cumsum(a.block_sizes)])
+        for k in range(len(sizes)):
+            sl = slice(int(slices[k]), int(slices[k + 1]))
+            ad = self._dense(a, sl)
+            self._fold(ad, obc_blocks.retarded, offs, sl, -1.0)
+            self._gather(out, inv(ad), sl)
+        if return_out:
+            return out
+        return None
+
+    # inv.py:99-227
+    def selected_solve(self, a: DSDBCOO, sigma_lesser: DSDBCOO, sigma_greater: DSDBCOO,
+                       obc_blocks: OBCBlocks | None = None, out=None,
+                       return_retarded: bool = False, return_current: bool = False):
+        from quatrex_amd.ops import gemm, inv
+
+        if return_current:
+            raise NotImplementedError("The computation of the current is not implemented.")
+        self._check(a)
+        sizes, slices = get_batches(a.shape[0], self.max_batch_size)
+        if obc_blocks is None:
+            obc_blocks = OBCBlocks(num_blocks=a.num_blocks)
+        if out is None:
+            xl, xg = a.__class__.zeros_like(a), a.__class__.zeros_like(a)
+            xr = a.__class__.zeros_like(a) if return_retarded else None
+        else:
+            xl, xg, *rest = out
+            xr = None
+            if return_retarded:
+                if len(rest) == 0:
+                    raise ValueError("Missing output for the retarded Green's function.")
+                xr = rest[0]
+        offs = np.concatenate([[0], np.cumsum(a.block_sizes)])
+        for k in range(len(sizes)):
+            sl = slice(int(slices[k]), int(slices[k + 1]))
+            ad = self._dense(a, sl)
+            sld, sgd = self._dense(sigma_lesser, sl), self._dense(sigma_greater, sl)
+            self._fold(ad, obc_blocks.retarded, offs, sl, -1.0)
+            self._fold(sld, obc_blocks.lesser, offs, sl, -1.0)
+            self._fold(sgd, obc_blocks.greater, offs, sl, -1.0)
+            x = inv(ad)
+            self._gather(xl, gemm(gemm(x, sld), x, op_b="C"), sl)
+            self._gather(xg, gemm(gemm(x, sgd), x, op_b="C"), sl)
+            if return_retarded:
+                self._gather(xr, x, sl)
+        if return_retarded:
+            return xl, xg, xr
+        return xl, xg
 
 
 def transmission(corner, gamma_l, gamma_r) -> torch.Tensor:

@@ -48,7 +48,7 @@ SHAPES = [(1, 1, 1), (5, 7, 3), (16, 16, 16), (64, 64, 64), (100, 37, 129), (130
 
 @pytest.mark.parametrize("mnk", SHAPES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("ops", [(0, 0), (0, 2), (2, 0), (2, 2)], ids=["NN", "NC", "CN", "CC"])
-@pytest.mark.parametrize("prec", [0, 1], ids=["c128", "c64"])
+@pytest.mark.parametrize("prec", [0, 1, 3], ids=["c128", "c64", "c128_3m"])
 def test_zgemm(device, mnk, ops, prec):
     m, n, k = mnk
     rng = np.random.default_rng(m * 7 + n * 3 + k)
@@ -69,12 +69,38 @@ def test_zgemm(device, mnk, ops, prec):
     opA = a if opa == 0 else np.conj(np.swapaxes(a, 1, 2))
     opB = b if opb == 0 else np.conj(np.swapaxes(b, 1, 2))
     ref = alpha * opA @ opB + (beta * c if beta != 0.0 else 0)
-    tol = 1e-12 if prec == 0 else 2e-5
+    tol = {0: 1e-12, 1: 2e-5, 3: 1e-12}[prec]
     got = C.cpu().numpy()
     for e in range(batch):
         assert rel_err(got[e], ref[e]) < tol
     # the padding columns of A are untouched
     assert torch.all(A[..., a_shape[2]:] == 0)
+
+
+BIG = [(256, 256, 256, 64), (200, 136, 72, 130), (384, 130, 33, 100)]
+
+
+@pytest.mark.parametrize("mnkb", BIG, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("ops", [(0, 0), (0, 2), (2, 0), (2, 2)], ids=["NN", "NC", "CN", "CC"])
+@pytest.mark.parametrize("prec", [0, 1, 3], ids=["c128", "c64", "c128_3m"])
+def test_zgemm_large_stack(device, mnkb, ops, prec):
+    """Stacks with >= 256 tiles of 128 x 128 (the large-tile kernel shape)."""
+    m, n, k, batch = mnkb
+    rng = np.random.default_rng(m + n + k + batch)
+    opa, opb = ops
+    a_shape = (batch, m, k) if opa == 0 else (batch, k, m)
+    b_shape = (batch, k, n) if opb == 0 else (batch, n, k)
+    a, b, c = _cplx(rng, a_shape), _cplx(rng, b_shape), _cplx(rng, (batch, m, n))
+    alpha, beta = 1.1 + 0.2j, -0.5 + 0.25j
+    A, B = torch.as_tensor(a, device=device), torch.as_tensor(b, device=device)
+    C = torch.as_tensor(c, device=device)
+    _zgemm(device, A, B, C, opa, opb, alpha, beta, prec)
+    got = C.cpu().numpy()
+    tol = {0: 1e-12, 1: 2e-5, 3: 1e-12}[prec]
+    for e in (0, batch // 2, batch - 1):
+        opA = a[e] if opa == 0 else np.conj(a[e].T)
+        opB = b[e] if opb == 0 else np.conj(b[e].T)
+        assert rel_err(got[e], alpha * opA @ opB + beta * c[e]) < tol
 
 
 def _inv(dev, A, X, prec, S=None):

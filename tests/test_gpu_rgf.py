@@ -127,6 +127,17 @@ def test_workload_parity_c128(device, workload, max_batch_size):
     assert rel_err(cur.cpu().numpy(), rcur) < 1e-10
 
 
+def test_workload_parity_c128_3m(device, workload):
+    """complex128 with the 3M product split: same bar on this workload."""
+    w, (oa, osl, osg), (rl, rg, rr, rcur) = workload
+    a, sl, sg = to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device)
+    xl, xg, xr, cur = RGF(16, precision="c128-3m", retarded_offdiag=True).selected_solve(
+        a, sl, sg, return_retarded=True, return_current=True)
+    for g, o in ((xl, rl), (xg, rg), (xr, rr)):
+        assert _per_block_worst(w, g, o) < 1e-10
+    assert rel_err(cur.cpu().numpy(), rcur) < 1e-10
+
+
 def test_workload_device_assembly_matches(device, workload):
     """Workload.assemble (device, dense-block layout) equals the oracle containers."""
     w, (oa, osl, osg), _ = workload
