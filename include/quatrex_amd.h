@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define QX_ABI_VERSION 1
+#define QX_ABI_VERSION 2
 
 /* status codes */
 #define QX_OK 0
@@ -116,7 +116,11 @@ int qx_rgf_selected_inv(qx_handle* handle, void* stream,
  * corner (nullable, no counterpart in the reference): receives the retarded
  * corner block G^R_{N-1,0}, (n_stack, b_{N-1}, b_0) contiguous, from the
  * forward recursion g_{j,0} = -x^R_jj A_{j,j-1} g_{j-1,0}; it gives the
- * transmission Tr[Gamma_R G^R_{N-1,0} Gamma_L (G^R_{N-1,0})^H]. */
+ * transmission Tr[Gamma_R G^R_{N-1,0} Gamma_L (G^R_{N-1,0})^H].
+ * density (nullable): (n_stack, sum of block sizes) doubles, receives
+ * Im diag(G^<) as each diagonal block is finished (the quantity of
+ * quatrex/core/observables.py:57-86 without overlap), without a second pass
+ * over the output. */
 int qx_rgf_selected_solve(qx_handle* handle, void* stream,
                           const qx_btd_layout* la, const void* a,
                           const qx_btd_layout* lsl, const void* sigma_lesser,
@@ -126,8 +130,8 @@ int qx_rgf_selected_solve(qx_handle* handle, void* stream,
                           const qx_btd_layout* lxl, void* x_lesser,
                           const qx_btd_layout* lxg, void* x_greater,
                           const qx_btd_layout* lxr, void* x_retarded,
-                          void* current, void* corner, uint32_t flags,
-                          int precision);
+                          void* current, void* corner, double* density,
+                          uint32_t flags, int precision);
 
 /* Batched complex GEMM C = alpha op(A) op(B) + beta C, op in {N, C}.
  * Replaces the batched `@` of the reference (cuBLAS zgemm strided-batched).
@@ -179,6 +183,19 @@ int qx_contact_self_energy(qx_handle* handle, void* stream, int64_t n, int b,
                            const void* m10, const void* g, const void* m01,
                            const double* occupancies, void* sigma_r,
                            void* sigma_l, void* sigma_g);
+
+/* System matrix on the DSDBCOO nnz pattern.  Replaces
+ * ElectronSolver._assemble_system_matrix (src/quatrex/electron/solver.py:
+ * 390-407 with _btd_subtract, :26-50) for stacks whose operands share the
+ * pattern of A:  a[e, k] = z_e s[k] + t[k] - sigma[e, k], where z_e = E_e +
+ * i eta (n_stack complex), s the overlap (or identity) values and t the
+ * static part -H - diag(V) on the pattern (nnz complex each, nullable = 0),
+ * sigma the retarded scattering self-energy (nullable), stack strides
+ * sigma_stride / a_stride. */
+int qx_assemble_system_matrix(qx_handle* handle, void* stream, int64_t n_stack,
+                              int64_t nnz, const void* z, const void* s,
+                              const void* t, const void* sigma,
+                              int64_t sigma_stride, void* a, int64_t a_stride);
 
 #ifdef __cplusplus
 }

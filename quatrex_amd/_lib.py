@@ -24,6 +24,7 @@ QX_ENOMEM = -2
 QX_EHIP = -3
 QX_ESINGULAR = -4
 
+ABI_VERSION = 2  # QX_ABI_VERSION of include/quatrex_amd.h
 PREC_C128 = 0
 PREC_C64 = 1
 PREC_MIXED = 2
@@ -47,7 +48,7 @@ EXPORTED_SYMBOLS = (
     "qx_abi_version", "qx_create", "qx_destroy", "qx_last_error", "qx_workspace_query",
     "qx_sync_check", "qx_rgf_selected_inv", "qx_rgf_selected_solve", "qx_zgemm_batched",
     "qx_inv_batched", "qx_sancho_rubio", "qx_beyn_moments", "qx_timer_start", "qx_timer_stop",
-    "qx_contact_self_energy",
+    "qx_contact_self_energy", "qx_assemble_system_matrix",
 )
 
 
@@ -108,7 +109,7 @@ def load() -> ctypes.CDLL:
         lib.qx_rgf_selected_inv.argtypes = [vp, vp, lay_p, vp, vp, lay_p, vp, i32]
         lib.qx_rgf_selected_solve.argtypes = [
             vp, vp, lay_p, vp, lay_p, vp, lay_p, vp, vp, vp, vp, lay_p, vp, lay_p, vp,
-            lay_p, vp, vp, vp, ctypes.c_uint32, i32]
+            lay_p, vp, vp, vp, vp, ctypes.c_uint32, i32]
         lib.qx_zgemm_batched.argtypes = [
             vp, vp, i64, i32, i32, i32, i32, i32, ctypes.POINTER(f64), vp, i32, i64, vp, i32,
             i64, ctypes.POINTER(f64), vp, i32, i64, i32]
@@ -118,11 +119,13 @@ def load() -> ctypes.CDLL:
                                         ctypes.POINTER(ctypes.c_int)]
         lib.qx_beyn_moments.argtypes = [vp, vp, i64, i32, i32, f64, f64, vp, vp, vp, vp, vp, i32]
         lib.qx_contact_self_energy.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, vp, vp, vp]
+        lib.qx_assemble_system_matrix.argtypes = [vp, vp, i64, i64, vp, vp, vp, vp, i64, vp,
+                                                   i64]
         lib.qx_timer_start.argtypes = [vp, vp]
         lib.qx_timer_stop.argtypes = [vp, vp, ctypes.POINTER(f64)]
         for name in EXPORTED_SYMBOLS:
             getattr(lib, name).restype = getattr(lib, name).restype or i32
-        if lib.qx_abi_version() != 1:
+        if lib.qx_abi_version() != ABI_VERSION:
             raise ImportError("libquatrex_amd ABI version mismatch; rebuild the library")
         _lib = lib
         return lib

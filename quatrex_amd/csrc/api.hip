@@ -18,7 +18,8 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, c
                         const void* sg, const void* const* obc_r, const void* const* obc_l,
                         const void* const* obc_g, const qx_btd_layout* lxl, void* xl,
                         const qx_btd_layout* lxg, void* xg, const qx_btd_layout* lxr, void* xr,
-                        void* current, void* corner, uint32_t flags, int prec);
+                        void* current, void* corner, double* density, uint32_t flags,
+                        int prec);
 
 namespace {
 
@@ -421,11 +422,12 @@ int qx_rgf_selected_solve(qx_handle* h, void* stream, const qx_btd_layout* la, c
                           const void* sg, const void* const* obc_r, const void* const* obc_l,
                           const void* const* obc_g, const qx_btd_layout* lxl, void* xl,
                           const qx_btd_layout* lxg, void* xg, const qx_btd_layout* lxr, void* xr,
-                          void* current, void* corner, uint32_t flags, int precision) {
+                          void* current, void* corner, double* density, uint32_t flags,
+                          int precision) {
   return guarded(h, [&]() {
     QX_REQUIRE(h, "null handle");
     qx::rgf_selected_solve(h, static_cast<hipStream_t>(stream), la, a, lsl, sl, lsg, sg, obc_r,
-                           obc_l, obc_g, lxl, xl, lxg, xg, lxr, xr, current, corner, flags,
+                           obc_l, obc_g, lxl, xl, lxg, xg, lxr, xr, current, corner, density, flags,
                            precision);
     return QX_OK;
   });
@@ -500,6 +502,21 @@ int qx_contact_self_energy(qx_handle* h, void* stream, int64_t n, int b, const v
               zero, qx::Mat(sigma_r, bb, b), QX_PREC_C128);
     qx::ew_contact_sigma(st, n, b, qx::Mat(sigma_r, bb, b), occupancies,
                          qx::Mat(sigma_l, bb, b), qx::Mat(sigma_g, bb, b));
+    return QX_OK;
+  });
+}
+
+int qx_assemble_system_matrix(qx_handle* h, void* stream, int64_t n_stack, int64_t nnz,
+                              const void* z, const void* s, const void* t, const void* sigma,
+                              int64_t sigma_stride, void* a, int64_t a_stride) {
+  return guarded(h, [&]() {
+    QX_REQUIRE(h, "null handle");
+    QX_REQUIRE(n_stack >= 0 && nnz >= 0, "assemble_system_matrix: bad sizes");
+    QX_REQUIRE(a && (z || !s), "assemble_system_matrix: null argument");
+    qx::ew_assemble(static_cast<hipStream_t>(stream), n_stack, nnz,
+                    static_cast<const qx::z2*>(z), static_cast<const qx::z2*>(s),
+                    static_cast<const qx::z2*>(t), static_cast<const qx::z2*>(sigma),
+                    sigma_stride, static_cast<qx::z2*>(a), a_stride);
     return QX_OK;
   });
 }

@@ -165,6 +165,44 @@ __global__ __launch_bounds__(NT) void contact_sigma_kernel(const z2* S, int64_t 
   }
 }
 
+__global__ __launch_bounds__(NT) void diag_imag_kernel(const z2* X, int64_t sX, int ldx, int n,
+                                                       double* out, int64_t so, int64_t total) {
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * NT) {
+    const int64_t e = i / n;
+    const int r = (int)(i - e * n);
+    out[e * so + r] = X[e * sX + (int64_t)r * ldx + r].y;
+  }
+}
+
+// System matrix on the nnz pattern: A[e, k] = z_e S_k + T_k - Sig[e, k].
+__global__ __launch_bounds__(NT) void assemble_kernel(const z2* z, const z2* S, const z2* T,
+                                                      const z2* Sig, int64_t sSig, z2* A,
+                                                      int64_t sA, int64_t nnz, int64_t total) {
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * NT) {
+    const int64_t e = i / nnz;
+    const int64_t k = i - e * nnz;
+    double vr = 0.0, vi = 0.0;
+    if (S) {
+      const z2 ze = z[e], sk = S[k];
+      vr = ze.x * sk.x - ze.y * sk.y;
+      vi = ze.x * sk.y + ze.y * sk.x;
+    }
+    if (T) {
+      const z2 tk = T[k];
+      vr += tk.x;
+      vi += tk.y;
+    }
+    if (Sig) {
+      const z2 sg = Sig[e * sSig + k];
+      vr -= sg.x;
+      vi -= sg.y;
+    }
+    A[e * sA + k] = make_double2(vr, vi);
+  }
+}
+
 // Block reduction of NT doubles in a fixed order.
 __device__ inline double block_sum(double v, double* red) {
 #pragma unroll
@@ -337,6 +375,26 @@ void ew_contact_sigma(hipStream_t s, int64_t batch, int n, const Mat& S, const d
   QX_REQUIRE(blocks < (int64_t(1) << 31), "contact_sigma: grid too large");
   hipLaunchKernelGGL(contact_sigma_kernel, dim3((unsigned)blocks), dim3(NT), 0, s, S.p, S.stride,
                      n, f, L.p, L.stride, G.p, G.stride, tn);
+  QX_HIP_CHECK(hipGetLastError());
+}
+
+void ew_diag_imag(hipStream_t s, int64_t batch, int n, const Mat& X, double* out,
+                  int64_t out_stride) {
+  if (batch <= 0 || n <= 0) return;
+  QX_REQUIRE(!X.zero() && out, "diag_imag: null argument");
+  const int64_t total = batch * n;
+  hipLaunchKernelGGL(diag_imag_kernel, dim3(grid_for(total)), dim3(NT), 0, s, X.p, X.stride, X.ld,
+                     n, out, out_stride, total);
+  QX_HIP_CHECK(hipGetLastError());
+}
+
+void ew_assemble(hipStream_t s, int64_t batch, int64_t nnz, const z2* z, const z2* S,
+                 const z2* T, const z2* Sig, int64_t sSig, z2* A, int64_t sA) {
+  if (batch <= 0 || nnz <= 0) return;
+  QX_REQUIRE(A && (z || !S), "assemble: null argument");
+  const int64_t total = batch * nnz;
+  hipLaunchKernelGGL(assemble_kernel, dim3(grid_for(total)), dim3(NT), 0, s, z, S, T, Sig, sSig, A,
+                     sA, nnz, total);
   QX_HIP_CHECK(hipGetLastError());
 }
 

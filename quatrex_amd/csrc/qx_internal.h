@@ -82,6 +82,12 @@ void ew_axpby(hipStream_t s, int64_t batch, int m, int n, Cplx a, const Mat& A,
 // matrix (device array).
 void ew_contact_sigma(hipStream_t s, int64_t batch, int n, const Mat& S, const double* f,
                       const Mat& L, const Mat& G);
+// A[e * sA + k] = z_e S_k + T_k - Sig[e * sSig + k] (S, T, Sig nullable = 0).
+void ew_assemble(hipStream_t s, int64_t batch, int64_t nnz, const z2* z, const z2* S,
+                 const z2* T, const z2* Sig, int64_t sSig, z2* A, int64_t sA);
+// out[e * out_stride + r] = Im X_e[r][r], r < n.
+void ew_diag_imag(hipStream_t s, int64_t batch, int n, const Mat& X, double* out,
+                  int64_t out_stride);
 // Y = I - Y for a stack of n x n matrices.
 void ew_ident_sub(hipStream_t s, int64_t batch, int n, const Mat& Y);
 void ew_antiherm(hipStream_t s, int64_t batch, int n, double scale,

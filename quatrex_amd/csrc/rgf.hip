@@ -207,7 +207,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
                         const void* const* obc_g, const qx_btd_layout* lxl,
                         void* xl, const qx_btd_layout* lxg, void* xg,
                         const qx_btd_layout* lxr, void* xr, void* current,
-                        void* corner, uint32_t flags, int prec) {
+                        void* corner, double* density, uint32_t flags, int prec) {
   QX_REQUIRE(la && a, "selected_solve: null A");
   QX_REQUIRE(lxl && xl && lxg && xg, "selected_solve: null lesser/greater output");
   const bool want_r = (flags & QX_SOLVE_RETARDED) != 0;
@@ -238,6 +238,11 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   const Obc OR{obc_r, e0}, OL{obc_l, e0}, OG{obc_g, e0};
   int* info = info_all + e0;
   z2* cur = current ? static_cast<z2*>(current) + e0 * (N - 1) : nullptr;
+  // density Im diag(G^<): row of n_orb per matrix, block i at orb_off[i]
+  std::vector<int64_t> orb_off(N + 1, 0);
+  for (int i = 0; i < N; ++i) orb_off[i + 1] = orb_off[i] + A.bs(i);
+  const int64_t n_orb = orb_off[N];
+  double* dens = density ? density + e0 * n_orb : nullptr;
   // corner block G^R_{N-1,0}: ping-pong slots + product temp
   const int b0 = A.bs(0), bl = A.bs(N - 1);
   int gc_cur = 3 * N + 17, gc_nxt = 3 * N + 18;
@@ -325,6 +330,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   // ---- last diagonal blocks (rgf.py:297-305) ----
   {
     const int bn = bs(N - 1);
+    if (dens) ew_diag_imag(st, S, bn, W.get(sl_[N - 1], bn), dens + orb_off[N - 1], n_orb);
     for (auto& q : qs)
       if (!q.out->diag(N - 1).zero())
         ew_antiherm(st, S, bn, 0.5, W.get((*q.slot)[N - 1], bn), q.out->diag(N - 1));
@@ -392,6 +398,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kOne, T3, B1, kOne, Z, gp);
       std::swap(spare, (*q.slot)[i]);
       if (!q.out->diag(i).zero()) ew_antiherm(st, S, bi, 0.5, Z, q.out->diag(i));
+      if (dens && &q == &qs[0]) ew_diag_imag(st, S, bi, Z, dens + orb_off[i], n_orb);
     }
 
     // retarded: xr_ii <- xr_ii + B4 xr_ii ; off-diagonal blocks on request

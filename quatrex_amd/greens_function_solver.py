@@ -220,9 +220,17 @@ class RGF(GFSolver):
     def selected_solve(self, a: DSDBCOO, sigma_lesser: DSDBCOO, sigma_greater: DSDBCOO,
                        obc_blocks: OBCBlocks | None = None, out=None,
                        return_retarded: bool = False, return_current: bool = False,
-                       *, corner: torch.Tensor | None = None):
-        """rgf.py:115-414.  Extension: ``corner`` (complex128 device tensor of
-        shape (*stack, b_{N-1}, b_0)) receives G^R_{N-1,0} for the transmission."""
+                       *, corner: torch.Tensor | None = None,
+                       density: torch.Tensor | None = None):
+        """rgf.py:115-414.
+
+        Extensions (keyword-only, filled in place):
+        ``corner`` (complex128 device tensor (*stack, b_{N-1}, b_0)) receives
+        G^R_{N-1,0} for the transmission; ``density`` (float64 device tensor
+        (*stack, sum of block sizes)) receives Im diag(G^<), the electron
+        density of quatrex/core/observables.py:57-86 (no overlap), written as
+        each diagonal block is finished.
+        """
         lib = _lib.load()
         _lib.require_device_tensor(a.data, "a.data")
         dev = a.device
@@ -282,6 +290,15 @@ class RGF(GFSolver):
                                      "(*stack, b_{N-1}, b_0)")
                 corner_ptr = ctypes.c_void_p(corner.data_ptr()
                                              + s0 * inner * nl * n0 * corner.element_size())
+            dens_ptr = None
+            if density is not None:
+                n_orb = int(np.sum(a.block_sizes))
+                if (density.dtype != torch.float64 or not density.is_contiguous()
+                        or tuple(density.shape) != tuple(a.shape[:-2]) + (n_orb,)):
+                    raise ValueError("density must be a contiguous float64 tensor of shape "
+                                     "(*stack, sum of block sizes)")
+                dens_ptr = ctypes.c_void_p(density.data_ptr()
+                                           + s0 * inner * n_orb * density.element_size())
             cur_ptr = None
             if current is not None:
                 cur_ptr = ctypes.c_void_p(current.data_ptr()
@@ -290,7 +307,7 @@ class RGF(GFSolver):
                 h, st, A.lay(), A.ptr(), SL.lay(), SL.ptr(), SG.lay(), SG.ptr(), o_r, o_l, o_g,
                 XL.lay(), XL.ptr(), XG.lay(), XG.ptr(),
                 XR.lay() if XR else None, XR.ptr() if XR else None, cur_ptr, corner_ptr,
-                flags, prec)
+                dens_ptr, flags, prec)
             _lib.check(rc, h, "RGF.selected_solve")
             for X in (XL, XG, XR):
                 if X is not None:
