@@ -344,6 +344,99 @@ def obc_workload(args):
     print(json.dumps(result), flush=True)
 
 
+def uneven_workload(args):
+    """Devices of mixed block size / block count / energy count in one graph.
+
+    24 devices, block size from {64, 128, 256, 416}, block count uniform in
+    [16, 128], energies uniform in [64, 1024] scaled by ``--energy-scale``
+    (the full mix is ~1.8e16 flops, minutes of GPU time).  Each device keeps
+    H, Sigma and its results resident; lanes solve energy chunks side by
+    side; the whole schedule is captured into one HIP graph and replayed.
+    Tile shape and chunk per block size come from the autotuner.  complex128
+    gives ``value``; the mixed mode is timed the same way and compared.
+    """
+    import numpy as np
+    import torch
+
+    from quatrex_amd.engine import Device, UnevenBatch, autotune, uneven_mix
+    from quatrex_amd.workloads import make_workload
+
+    dev = torch.device("cuda:0")
+    mix = uneven_mix(args.seed, n_devices=args.devices, energy_scale=args.energy_scale)
+    t0 = time.perf_counter()
+    devices = [Device(make_workload(seed=100 + k, n_energies=e, n_blocks=n, block_size=b), dev,
+                      f"dev{k}") for k, (b, n, e) in enumerate(mix)]
+    torch.cuda.synchronize()
+    log(f"uneven: {len(devices)} devices, {sum(d.n_energies for d in devices)} energies "
+        f"resident ({time.perf_counter() - t0:.1f} s)")
+    results = {}
+    tables = {}
+    for prec in ["c128"] + [m for m in args.compare.split(",") if m == "mixed"]:
+        t0 = time.perf_counter()
+        tables[prec] = autotune([d.b for d in devices], dev, precision=prec,
+                                budget_bytes=args.lane_budget << 30, verbose=log)
+        log(f"{prec}: autotune {time.perf_counter() - t0:.1f} s: {tables[prec]}")
+        ub = UnevenBatch(devices, n_lanes=args.lanes, precision=prec, tuning=tables[prec],
+                         lane_budget_bytes=args.lane_budget << 30)
+        t0 = time.perf_counter()
+        ub.run_eager()
+        torch.cuda.synchronize()
+        t_eager = time.perf_counter() - t0
+        log(f"{prec}: eager {t_eager:.2f} s")
+        t0 = time.perf_counter()
+        ub.capture()
+        torch.cuda.synchronize()
+        log(f"{prec}: captured ({time.perf_counter() - t0:.1f} s)")
+        ub.replay()
+        ub.check()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ub.replay()
+        torch.cuda.synchronize()
+        t_graph = (time.perf_counter() - t0) / args.steps
+        ub.check()
+        log(f"{prec}: graph replay {t_graph:.2f} s")
+        results[prec] = {
+            "energies_per_s": round(ub.total_energies / t_graph, 3),
+            "eager_energies_per_s": round(ub.total_energies / t_eager, 3),
+            "graph_s": round(t_graph, 3), "eager_s": round(t_eager, 3),
+            "algorithmic_tflops": round(ub.total_flops / t_graph / 1e12, 3),
+            "lane_load_balance": round(min(ub.lane_flops) / max(ub.lane_flops), 3),
+            "items": sum(len(l.items) for l in ub.lanes),
+            "density": [d.density.clone() for d in devices],
+            "transmission": [d.transmission.clone() for d in devices],
+        }
+        del ub
+        torch.cuda.empty_cache()
+    prim = results["c128"]
+    out_modes = {}
+    for prec, r in results.items():
+        rec = {k: v for k, v in r.items() if k not in ("density", "transmission")}
+        if prec != "c128":
+            dn = sum(float(torch.linalg.norm(a - b) ** 2)
+                     for a, b in zip(r["density"], prim["density"])) ** 0.5
+            dd = sum(float(torch.linalg.norm(b) ** 2) for b in prim["density"]) ** 0.5
+            tr = max(float(((a - b).abs() / b.abs().clamp_min(1e-300)).max())
+                     for a, b in zip(r["transmission"], prim["transmission"]))
+            rec["error_vs_c128"] = {"density_rel": float(f"{dn / dd:.3e}"),
+                                    "transmission_rel_max": float(f"{tr:.3e}")}
+            rec["speedup_vs_c128"] = round(r["energies_per_s"] / prim["energies_per_s"], 3)
+        out_modes[prec] = rec
+    result = {
+        "metric": "uneven batched RGF: energies/s over a 24-device size mix, one graph launch",
+        "value": prim["energies_per_s"], "unit": "energies/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(prim["graph_s"] * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "uneven device mix", "devices": [list(m) for m in mix],
+                   "energy_scale": args.energy_scale, "lanes": args.lanes,
+                   "total_energies": sum(d.n_energies for d in devices)},
+        "modes": out_modes, "autotune": {p: {str(k): v for k, v in t.items()}
+                                         for p, t in tables.items()},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -363,9 +456,16 @@ def main():
                     help="seconds of CPU work for the baseline sample (0 disables)")
     ap.add_argument("--no-timer", action="store_true",
                     help="do not bracket kernels with HIP events (no roofline)")
-    ap.add_argument("--workload", default="rgf", choices=["rgf", "obc"],
+    ap.add_argument("--workload", default="rgf", choices=["rgf", "obc", "uneven"],
                     help="rgf: the selected solve (default); obc: lead surface Green's "
-                         "functions (use --energies 1024 --block-size 256)")
+                         "functions (use --energies 1024 --block-size 256); uneven: mixed "
+                         "device sizes in one graph launch")
+    ap.add_argument("--devices", type=int, default=24, help="uneven: number of devices")
+    ap.add_argument("--energy-scale", type=float, default=1 / 16,
+                    help="uneven: energies per device scaled from [64, 1024]")
+    ap.add_argument("--lanes", type=int, default=4, help="uneven: concurrent lanes")
+    ap.add_argument("--lane-budget", type=int, default=24,
+                    help="uneven: HBM per lane in GiB (bounds energies per chunk)")
     ap.add_argument("--beyn-radius", type=float, default=2.0,
                     help="outer contour radius of the Beyn leg (inner = 1 / radius)")
     args = ap.parse_args()
@@ -378,6 +478,9 @@ def main():
 
     if args.workload == "obc":
         obc_workload(args)
+        return
+    if args.workload == "uneven":
+        uneven_workload(args)
         return
 
     import torch

@@ -184,6 +184,13 @@ int qx_contact_self_energy(qx_handle* handle, void* stream, int64_t n, int b,
                            const double* occupancies, void* sigma_r,
                            void* sigma_l, void* sigma_g);
 
+/* Execution tuning of a handle (no counterpart in the reference): streams =
+ * concurrent sub-batches of one RGF call (1..8, default 4, env
+ * QX_RGF_STREAMS); gemm_tile = GEMM tile shape, 0 automatic, 64 or 128
+ * (env QX_ZGEMM_TILE overrides).  Used by the autotuner and by callers that
+ * run several handles side by side. */
+int qx_set_tuning(qx_handle* handle, int streams, int gemm_tile);
+
 /* System matrix on the DSDBCOO nnz pattern.  Replaces
  * ElectronSolver._assemble_system_matrix (src/quatrex/electron/solver.py:
  * 390-407 with _btd_subtract, :26-50) for stacks whose operands share the

@@ -48,7 +48,7 @@ EXPORTED_SYMBOLS = (
     "qx_abi_version", "qx_create", "qx_destroy", "qx_last_error", "qx_workspace_query",
     "qx_sync_check", "qx_rgf_selected_inv", "qx_rgf_selected_solve", "qx_zgemm_batched",
     "qx_inv_batched", "qx_sancho_rubio", "qx_beyn_moments", "qx_timer_start", "qx_timer_stop",
-    "qx_contact_self_energy", "qx_assemble_system_matrix",
+    "qx_contact_self_energy", "qx_assemble_system_matrix", "qx_set_tuning",
 )
 
 
@@ -121,6 +121,7 @@ def load() -> ctypes.CDLL:
         lib.qx_contact_self_energy.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, vp, vp, vp]
         lib.qx_assemble_system_matrix.argtypes = [vp, vp, i64, i64, vp, vp, vp, vp, i64, vp,
                                                    i64]
+        lib.qx_set_tuning.argtypes = [vp, i32, i32]
         lib.qx_timer_start.argtypes = [vp, vp]
         lib.qx_timer_stop.argtypes = [vp, vp, ctypes.POINTER(f64)]
         for name in EXPORTED_SYMBOLS:
@@ -137,8 +138,52 @@ def require_device_tensor(t: torch.Tensor, what: str) -> None:
                          "quatrex_amd has no CPU fallback")
 
 
+_override = threading.local()
+
+
+def new_handle(device: torch.device) -> ctypes.c_void_p:
+    """A separate library handle (own workspace and helper streams)."""
+    lib = load()
+    device = torch.device(device)
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    h = ctypes.c_void_p()
+    rc = lib.qx_create(ctypes.byref(h), idx)
+    if rc != QX_OK:
+        raise RuntimeError(f"qx_create failed on device {idx} (code {rc})")
+    return h
+
+
+def set_tuning(h, streams: int, gemm_tile: int = 0) -> None:
+    """Sub-batch streams of one RGF call and GEMM tile shape (0 automatic)."""
+    check(load().qx_set_tuning(h, int(streams), int(gemm_tile)), h, "set_tuning")
+
+
+class use_handle:
+    """Route this thread's library calls through handle ``h`` (context manager)."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def __enter__(self):
+        self.prev = getattr(_override, "h", None)
+        _override.h = self.h
+        return self.h
+
+    def __exit__(self, *exc):
+        _override.h = self.prev
+
+
+def capturing(device=None) -> bool:
+    """True while the current stream records a graph (no host syncs then)."""
+    return torch.cuda.is_current_stream_capturing()
+
+
 def handle(device: torch.device) -> ctypes.c_void_p:
-    """Per-device library handle (created on first use)."""
+    """This thread's library handle: the ``use_handle`` override if any, else
+    the per-device default handle (created on first use)."""
+    h = getattr(_override, "h", None)
+    if h is not None:
+        return h
     lib = load()
     idx = device.index if device.index is not None else torch.cuda.current_device()
     h = _handles.get(idx)
@@ -172,7 +217,12 @@ def check(rc: int, h, what: str) -> int:
 
 
 def sync_check(device: torch.device, what: str) -> None:
-    """Wait for the device stream and raise on recorded device errors."""
+    """Wait for the device stream and raise on recorded device errors.
+
+    Skipped while a graph is being captured; the caller checks after replay.
+    """
+    if capturing():
+        return
     h = handle(device)
     check(load().qx_sync_check(h, stream_ptr(device)), h, what)
 

@@ -145,6 +145,7 @@ int beyn_moments(qx_handle* h, hipStream_t st, int64_t n, int b, int nq, double 
 }  // namespace
 
 thread_local KernelTimer* active_timer = nullptr;
+thread_local int active_gemm_tile = 0;
 
 void run_split(qx_handle* h, hipStream_t st, int64_t S,
                const std::function<void(hipStream_t, int64_t, int64_t)>& f) {
@@ -168,11 +169,13 @@ void run_split(qx_handle* h, hipStream_t st, int64_t S,
   for (int k = 0; k < P; ++k) e0[k + 1] = e0[k] + S / P + (k < S % P ? 1 : 0);
   std::vector<std::exception_ptr> err(P);
   KernelTimer* timer = active_timer;
+  const int tile = active_gemm_tile;
   std::vector<std::thread> threads;
   for (int k = 1; k < P; ++k) {
     threads.emplace_back([&, k]() {
       try {
         active_timer = timer;
+        active_gemm_tile = tile;
         QX_HIP_CHECK(hipSetDevice(h->device));
         f(h->streams[k - 1], e0[k], e0[k + 1] - e0[k]);
       } catch (...) {
@@ -266,9 +269,17 @@ int* qx_handle::ensure_info(size_t n) {
 
 namespace {
 
+// Per-call thread-local state taken from the handle: kernel timer and GEMM
+// tile policy (helper threads of run_split inherit both).
 struct TimerScope {
-  explicit TimerScope(qx_handle* h) { qx::active_timer = h ? h->timer : nullptr; }
-  ~TimerScope() { qx::active_timer = nullptr; }
+  explicit TimerScope(qx_handle* h) {
+    qx::active_timer = h ? h->timer : nullptr;
+    qx::active_gemm_tile = h ? h->gemm_tile : 0;
+  }
+  ~TimerScope() {
+    qx::active_timer = nullptr;
+    qx::active_gemm_tile = 0;
+  }
 };
 
 template <typename F>
@@ -517,6 +528,18 @@ int qx_assemble_system_matrix(qx_handle* h, void* stream, int64_t n_stack, int64
                     static_cast<const qx::z2*>(z), static_cast<const qx::z2*>(s),
                     static_cast<const qx::z2*>(t), static_cast<const qx::z2*>(sigma),
                     sigma_stride, static_cast<qx::z2*>(a), a_stride);
+    return QX_OK;
+  });
+}
+
+int qx_set_tuning(qx_handle* h, int streams, int gemm_tile) {
+  return guarded(h, [&]() {
+    QX_REQUIRE(h, "null handle");
+    QX_REQUIRE(streams >= 1 && streams <= 8, "set_tuning: streams must be in [1, 8]");
+    QX_REQUIRE(gemm_tile == 0 || gemm_tile == 64 || gemm_tile == 128,
+               "set_tuning: gemm_tile must be 0 (automatic), 64 or 128");
+    h->parts = streams;
+    h->gemm_tile = gemm_tile;
     return QX_OK;
   });
 }

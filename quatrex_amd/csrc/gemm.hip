@@ -303,8 +303,10 @@ void launch_shape(hipStream_t s, int64_t batch, int opA, int opB, GemmArgs g) {
   if (timer) timer->end(s, ev0, KernelTimer::GEMM, 8.0 * g.m * g.n * (double)g.k * batch);
 }
 
-// Tile shape: 128 x 128 when both dimensions fill it and the stack has at
-// least one such tile per CU, else 64 x 64.
+// Tile shape: QX_ZGEMM_TILE if set, else the calling handle's setting
+// (qx_set_tuning, e.g. from an autotuner), else 128 x 128 when both
+// dimensions fill it and the stack has at least one such tile per CU, else
+// 64 x 64.
 int gemm_tile(int m, int n, int64_t batch) {
   static const int forced = []() {
     const char* e = std::getenv("QX_ZGEMM_TILE");
@@ -313,6 +315,7 @@ int gemm_tile(int m, int n, int64_t batch) {
     return (v == 64 || v == 128) ? v : 0;
   }();
   if (forced) return forced;
+  if (active_gemm_tile == 64 || active_gemm_tile == 128) return active_gemm_tile;
   const int64_t big = batch * ((m + 127) / 128) * ((n + 127) / 128);
   return (m >= 128 && n >= 128 && big >= 256) ? 128 : 64;
 }

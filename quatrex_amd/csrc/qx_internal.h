@@ -135,6 +135,8 @@ struct KernelTimer {
   ~KernelTimer();
 };
 extern thread_local KernelTimer* active_timer;
+// GEMM tile shape of the calling handle: 0 automatic, 64 or 128.
+extern thread_local int active_gemm_tile;
 
 // Splits a batch of S matrices into up to `handle->parts` contiguous
 // sub-batches and runs f(stream, e0, n) for each concurrently: sub-batch 0 on
@@ -156,6 +158,7 @@ struct qx_handle {
   std::string last_error;
   qx::KernelTimer* timer = nullptr;
   int parts = 4;                      // concurrent sub-batches of one RGF call
+  int gemm_tile = 0;                  // GEMM tile shape: 0 automatic, 64, 128
   std::vector<hipStream_t> streams;   // helper streams (default priority)
   std::vector<hipEvent_t> join;       // one per helper stream
   hipEvent_t fork = nullptr;

@@ -97,7 +97,11 @@ class DSDBCOO:
             global_stack_shape = (global_stack_shape,)
         if not isinstance(data, torch.Tensor):
             data = torch.as_tensor(np.asarray(data))
-        self._data = data.contiguous()
+        # A read-only stack may be one entry broadcast over the first axis
+        # (stride 0); it is kept as is.  Everything else is made contiguous.
+        bcast = data.dim() >= 2 and data.shape[0] > 1 and data.stride(0) == 0 and \
+            data[0].is_contiguous()
+        self._data = data if bcast else data.contiguous()
         self.rows = np.asarray(rows, dtype=np.int32)
         self.cols = np.asarray(cols, dtype=np.int32)
         if self.rows.shape != self.cols.shape or self.rows.shape[0] != data.shape[-1]:

@@ -100,11 +100,23 @@ class _Bound:
         upper = np.full(max(n - 1, 0), -1, np.int64)
         lower = np.full(max(n - 1, 0), -1, np.int64)
         arrs = {"d": diag, "u": upper, "l": lower}
+        # A read-only stack broadcast over its first axis (stride 0, e.g. an
+        # energy-independent self-energy expanded to the batch) is passed
+        # with stack stride 0: every matrix of the batch reads the same data.
+        bcast = (not output and mat.data.dim() == 2 and mat.data.shape[0] > 1
+                 and mat.data.stride(0) == 0)
         if not staged:
             for (kind, i), off in direct.items():
                 arrs[kind][i] = off
-            self.base = mat.data.data_ptr() + s0 * inner * mat.nnz * mat.data.element_size()
-            stride = mat.nnz
+            if bcast:
+                self.base, stride = mat.data.data_ptr(), 0
+            else:
+                if mat.data.stride(-1) != 1 or (mat.data.dim() >= 2
+                                                and mat.data.stride(-2) != mat.nnz):
+                    raise ValueError("DSDBCOO.data must be contiguous (or a stride-0 "
+                                     "broadcast of one stack entry for inputs)")
+                self.base = mat.data.data_ptr() + s0 * inner * mat.nnz * mat.data.element_size()
+                stride = mat.nnz
         else:
             # all blocks of this container go through one dense staging buffer
             keys = sorted(set(direct) | set(staged), key=lambda k: (k[1], k[0]))
