@@ -63,7 +63,7 @@ __global__ __launch_bounds__(NT) void ident_sub_kernel(z2* Y, int64_t sY, int ld
   }
 }
 
-// Y = cy*Y + ca*A + cb*conj(B^T); Y, A are m x n, B is n x m.  Real coefs.
+// Y = cy*Y + ca*A + cb*conj(B^T) + cc*D; Y, A, D are m x n, B is n x m.  Real coefs.
 struct HermArgs {
   int m, n;
   double cy, ca, cb;
@@ -77,6 +77,10 @@ struct HermArgs {
   int64_t sY;
   int ldy;
   int tm, tn;
+  double cc;
+  const z2* D;
+  int64_t sD;
+  int ldd;
 };
 
 __global__ __launch_bounds__(NT) void herm_kernel(HermArgs g) {
@@ -117,15 +121,22 @@ __global__ __launch_bounds__(NT) void herm_kernel(HermArgs g) {
       yr += g.cb * b.x;
       yi -= g.cb * b.y;
     }
+    if (g.D) {
+      const z2 d = g.D[e * g.sD + (int64_t)r * g.ldd + c];
+      yr += g.cc * d.x;
+      yi += g.cc * d.y;
+    }
     *y = make_double2(yr, yi);
   }
 }
 
 void herm(hipStream_t s, int64_t batch, int m, int n, double cy, double ca,
-          const Mat& A, double cb, const Mat& B, const Mat& Y) {
+          const Mat& A, double cb, const Mat& B, const Mat& Y, double cc = 0.0,
+          const Mat& D = Mat()) {
   if (batch <= 0 || m <= 0 || n <= 0) return;
   HermArgs g{m, n, cy, ca, cb, A.p, A.stride, A.ld, B.p, B.stride, B.ld,
-             Y.p, Y.stride, Y.ld, (m + TS - 1) / TS, (n + TS - 1) / TS};
+             Y.p, Y.stride, Y.ld, (m + TS - 1) / TS, (n + TS - 1) / TS,
+             cc, D.p, D.stride, D.ld};
   const int64_t blocks = batch * g.tm * g.tn;
   QX_REQUIRE(blocks < (int64_t(1) << 31), "herm: grid too large");
   hipLaunchKernelGGL(herm_kernel, dim3((unsigned)blocks), dim3(NT), 0, s, g);
@@ -411,6 +422,12 @@ void ew_antiherm(hipStream_t s, int64_t batch, int n, double scale,
                  const Mat& X, const Mat& Y) {
   QX_REQUIRE(X.p != Y.p, "antiherm: in-place use is not supported");
   herm(s, batch, n, n, 0.0, scale, X, -scale, X, Y);
+}
+
+void ew_herm_add(hipStream_t s, int64_t batch, int n, const Mat& D, double scale,
+                 const Mat& X, const Mat& Y) {
+  QX_REQUIRE(X.p != Y.p, "herm_add: Y must not alias X");
+  herm(s, batch, n, n, 0.0, scale, X, -scale, X, Y, 1.0, D);
 }
 
 void ew_add_hdiff(hipStream_t s, int64_t batch, int n, const Mat& T,

@@ -60,6 +60,9 @@ struct GemmArgs {
   z2* C;
   int64_t sC;
   int ldc;
+  const z2* Cin;  // beta term source (== C for the in-place form)
+  int64_t sCin;
+  int ldcin;
   int tilesM, tilesN;
   int beta_zero;
 };
@@ -115,7 +118,8 @@ void zgemm_kernel(GemmArgs g) {
   const int m0 = (t / g.tilesN) * BM, n0 = (t % g.tilesN) * BN;
   const z2* __restrict__ A = g.A + e * g.sA;
   const z2* __restrict__ B = g.B + e * g.sB;
-  z2* __restrict__ C = g.C + e * g.sC;
+  z2* C = g.C + e * g.sC;
+  const z2* Ci = g.Cin + e * g.sCin;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w / WN, wn = w % WN;
@@ -272,7 +276,7 @@ void zgemm_kernel(GemmArgs g) {
           double outr = g.ar * vr - g.ai * vi, outi = g.ar * vi + g.ai * vr;
           z2* dst = C + (int64_t)row * g.ldc + col;
           if (!g.beta_zero) {
-            const z2 c = *dst;
+            const z2 c = Ci[(int64_t)row * g.ldcin + col];
             outr += g.br * c.x - g.bi * c.y;
             outi += g.br * c.y + g.bi * c.x;
           }
@@ -352,6 +356,12 @@ int gemm_split(int precision) {
 void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
            Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& C,
            int precision) {
+  zgemm_in(s, batch, m, n, k, opA, opB, alpha, A, B, beta, C, C, precision);
+}
+
+void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+              Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
+              const Mat& C, int precision) {
   QX_REQUIRE(!C.zero(), "zgemm: output is a zero matrix");
   QX_REQUIRE(opA == QX_OP_N || opA == QX_OP_C, "zgemm: bad op_a");
   QX_REQUIRE(opB == QX_OP_N || opB == QX_OP_C, "zgemm: bad op_b");
@@ -359,10 +369,10 @@ void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
                  precision == QX_PREC_C128_3M,
              "zgemm: precision must be QX_PREC_C128, QX_PREC_C64 or QX_PREC_C128_3M");
   if (batch <= 0 || m <= 0 || n <= 0) return;
-  const bool beta_zero = beta.re == 0.0 && beta.im == 0.0;
+  const bool beta_zero = (beta.re == 0.0 && beta.im == 0.0) || Cin.zero();
   if (A.zero() || B.zero() || k <= 0) {
-    // op(A) op(B) == 0: C = beta C.
-    ew_axpby(s, batch, m, n, beta, C, Cplx{0, 0}, Mat(), C);
+    // op(A) op(B) == 0: C = beta Cin.
+    ew_axpby(s, batch, m, n, beta_zero ? Cplx{0, 0} : beta, Cin, Cplx{0, 0}, Mat(), C);
     return;
   }
   GemmArgs g;
@@ -382,6 +392,9 @@ void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
   g.C = C.p;
   g.sC = C.stride;
   g.ldc = C.ld;
+  g.Cin = Cin.zero() ? C.p : Cin.p;
+  g.sCin = Cin.zero() ? C.stride : Cin.stride;
+  g.ldcin = Cin.zero() ? C.ld : Cin.ld;
   g.beta_zero = beta_zero ? 1 : 0;
   const int split = gemm_split(precision);
   if (precision == QX_PREC_C64) {

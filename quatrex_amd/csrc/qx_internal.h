@@ -58,6 +58,10 @@ struct Error {
 void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
            Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& C,
            int precision);
+// C = alpha op(A) op(B) + beta Cin (Cin may equal C; a zero Cin counts as 0).
+void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+              Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
+              const Mat& C, int precision);
 
 // X = inv(A - S) with partial pivoting (S may be zero).  info[e] is set to a
 // nonzero value for an exactly singular matrix.  n <= kInvMaxN.  The scratch
@@ -90,6 +94,9 @@ void ew_diag_imag(hipStream_t s, int64_t batch, int n, const Mat& X, double* out
                   int64_t out_stride);
 // Y = I - Y for a stack of n x n matrices.
 void ew_ident_sub(hipStream_t s, int64_t batch, int n, const Mat& Y);
+// Y = D + scale (X - X^H) (Y may alias D, not X).
+void ew_herm_add(hipStream_t s, int64_t batch, int n, const Mat& D, double scale,
+                 const Mat& X, const Mat& Y);
 void ew_antiherm(hipStream_t s, int64_t batch, int n, double scale,
                  const Mat& X, const Mat& Y);
 // Y += T^H - T             (square)

@@ -166,8 +166,12 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       const Mat T1 = W.get(t1, bi);
       zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get(xd[i], bi), kZero, T1, gp);
       const Mat Xj = W.get(xd[j], bj);
-      fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xj);
-      zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xj, gp);
+      Mat Ajj = A.diag(j);
+      if (!OR.get(j, bj).zero()) {
+        fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xj);
+        Ajj = Xj;
+      }
+      zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Ajj, Xj, gp);
       block_inv(st, S, bj, Xj, info, prec, scr, W.get(N + 6, bj), W.get(N + 7, bj));
     }
     // rgf.py:94-95
@@ -191,8 +195,7 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
         zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xjj, kZero, X.upper(i), gp);
       // x_ii <- x_ii - x_ii a_ij x_ji
       const Mat Z = W.get(spare, bi);
-      ew_axpby(st, S, bi, bi, kOne, Xii, kZero, Mat(), Z);
-      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xji, kOne, Z, gp);
+      zgemm_in(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xji, kOne, Xii, Z, gp);
       std::swap(spare, xd[i]);
       if (!X.diag(i).zero())
         ew_axpby(st, S, bi, bi, kOne, W.get(xd[i], bi), kZero, Mat(), X.diag(i));
@@ -257,7 +260,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     sg_[i] = 2 * N + i;
   }
   int spare = 3 * N;
-  const int T1s = 3 * N + 1, T2s = 3 * N + 2, T3s = 3 * N + 3, T4s = 3 * N + 4, T5s = 3 * N + 5,
+  const int T1s = 3 * N + 1, T3s = 3 * N + 3, T4s = 3 * N + 4, T5s = 3 * N + 5,
             T6s = 3 * N + 6, B1s = 3 * N + 7, B2s = 3 * N + 8, B3s = 3 * N + 9, B4s = 3 * N + 10,
             C1s = 3 * N + 11, C2s = 3 * N + 12, C3s = 3 * N + 13, C4s = 3 * N + 14;
   const InvScratch scr = W.inv(3 * N + 15);
@@ -296,8 +299,12 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     // a_ji xr_ii
     zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), Xri, kZero, T1, gp);
     // xr_jj = inv(a_jj - obc - a_ji xr_ii a_ij)
-    fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xrj);
-    zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Xrj, gp);
+    Mat Ajj = A.diag(j);
+    if (!OR.get(j, bj).zero()) {
+      fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xrj);
+      Ajj = Xrj;
+    }
+    zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Ajj, Xrj, gp);
     block_inv(st, S, bj, Xrj, info, prec, scr, W.get(NS1, bj), W.get(NS2, bj));
     if (corner) {
       // g_{j,0} = -x^R_jj (a_ji g_{i,0})
@@ -310,9 +317,13 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     for (auto& q : qs) {
       const Mat T3 = W.get(T3s, bj), T4 = W.get(T4s, bi), T5 = W.get(T5s, bj), T6 = W.get(T6s, bj);
       // sigma_jj + obc + a_ji x_ii a_ji^H + u^H - u,  u = a_ji xr_ii sigma_ij
-      fold(st, S, bj, bj, q.sig->diag(j), 1.0, q.obc->get(j, bj), T3);
+      Mat Sjj = q.sig->diag(j);
+      if (!q.obc->get(j, bj).zero()) {
+        fold(st, S, bj, bj, q.sig->diag(j), 1.0, q.obc->get(j, bj), T3);
+        Sjj = T3;
+      }
       zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get((*q.slot)[i], bi), kZero, T4, gp);
-      zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, A.lower(i), kOne, T3, gp);
+      zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, A.lower(i), kOne, Sjj, T3, gp);
       const Mat sij = q.sig->upper(i);
       if (!sij.zero()) {
         zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kOne, T1, sij, kZero, T5, gp);
@@ -373,29 +384,26 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
 
     for (auto& q : qs) {
       const Mat Xi = W.get((*q.slot)[i], bi), Xj = W.get((*q.slot)[j], bj);
-      const Mat T1 = W.get(T1s, bi), T2 = W.get(T2s, bi), T3 = W.get(T3s, bj), T5 = W.get(T5s, bj);
+      const Mat T1 = W.get(T1s, bi), T5 = W.get(T5s, bj);
       const Mat sij = q.sig->upper(i);
-      // temp_1x = B4 x_ii - xr_ii s_ij B3^H ;  temp_1x -= temp_1x^H
+      // temp_1x = B4 x_ii - xr_ii s_ij B3^H
       zgemm(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xi, kZero, T1, gp);
       if (!sij.zero()) {
         zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, sij, kZero, T5, gp);
         zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, T5, B3, kOne, T1, gp);
       }
-      ew_antiherm(st, S, bi, 1.0, T1, T2);
-      // temp_2x = B1 x_jj
-      zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kOne, B1, Xj, kZero, T3, gp);
-      // x_ij = -temp_2x - x_ii B2^H + xr_ii s_ij xr_jj^H
+      // x_ii' = x_ii + (temp_1x - temp_1x^H) [+ temp_2x B1^H below]
+      const Mat Z = W.get(spare, bi);
+      ew_herm_add(st, S, bi, Xi, 1.0, T1, Z);
+      // x_ij = -temp_2x - x_ii B2^H + xr_ii s_ij xr_jj^H,  temp_2x = B1 x_jj
       Mat Oij = q.out->upper(i);
       if (Oij.zero()) Oij = W.get(T4s, bj);
-      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, Xi, B2, kZero, Oij, gp);
-      ew_axpby(st, S, bi, bj, kOne, Oij, kMinus, T3, Oij);
+      zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, B1, Xj, kZero, Oij, gp);  // -temp_2x
+      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, Oij, B1, kOne, Z, gp);    // + temp_2x B1^H
+      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, Xi, B2, kOne, Oij, gp);
       if (!sij.zero())
         zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_C, kOne, T5, Xrj, kOne, Oij, gp);
       if (!q.skip_lower && !q.out->lower(i).zero()) ew_neg_h(st, S, bi, bj, Oij, q.out->lower(i));
-      // x_ii <- x_ii + temp_2x B1^H + temp_1x
-      const Mat Z = W.get(spare, bi);
-      ew_axpby(st, S, bi, bi, kOne, Xi, kOne, T2, Z);
-      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kOne, T3, B1, kOne, Z, gp);
       std::swap(spare, (*q.slot)[i]);
       if (!q.out->diag(i).zero()) ew_antiherm(st, S, bi, 0.5, Z, q.out->diag(i));
       if (dens && &q == &qs[0]) ew_diag_imag(st, S, bi, Z, dens + orb_off[i], n_orb);
@@ -408,8 +416,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
         zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, B2, Xri, kZero, XR.lower(i), gp);
     }
     const Mat Z = W.get(spare, bi);
-    ew_axpby(st, S, bi, bi, kOne, Xri, kZero, Mat(), Z);
-    zgemm(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xri, kOne, Z, gp);
+    zgemm_in(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xri, kOne, Xri, Z, gp);
     std::swap(spare, sr[i]);
     if (want_r && !XR.diag(i).zero()) ew_axpby(st, S, bi, bi, kOne, Z, kZero, Mat(), XR.diag(i));
   }

@@ -182,25 +182,79 @@ class Lane:
             _lib.sync_check(self.device, what)
 
 
+class Job:
+    """Energies, self-energies and result buffers of one solve of a device.
+
+    The device's own grid by default; a serving request brings its energies
+    and (optionally) updated Sigma^< / Sigma^> blocks.
+    """
+
+    def __init__(self, d: Device, energies=None, sigma_lesser=None, sigma_greater=None):
+        dev = d.device
+        self.device = d
+        if energies is None:
+            self.energies, self.density, self.transmission = (d.energies, d.density,
+                                                              d.transmission)
+        else:
+            self.energies = torch.as_tensor(np.asarray(energies, dtype=np.complex128), device=dev)
+            n = self.energies.shape[0]
+            self.density = torch.zeros((n, d.n_orb), dtype=torch.float64, device=dev)
+            self.transmission = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.sl, self.sg = d.sl, d.sg
+        if sigma_lesser is not None:
+            self.sl = btd_from_blocks([_f(x, dev) for x in sigma_lesser], None, None,
+                                      global_stack_shape=(), device=dev).data.reshape(1, -1)
+        if sigma_greater is not None:
+            self.sg = btd_from_blocks([_f(x, dev) for x in sigma_greater], None, None,
+                                      global_stack_shape=(), device=dev).data.reshape(1, -1)
+        self._views: dict = {}
+
+    @property
+    def n_energies(self) -> int:
+        return int(self.energies.shape[0])
+
+    def sigma_views(self, c: int):
+        if self.sl is self.device.sl and self.sg is self.device.sg:
+            return self.device.sigma_views(c)
+        v = self._views.get(c)
+        if v is None:
+            d = self.device
+            v = (d.container(self.sl.expand(c, d.nnz_d), True),
+                 d.container(self.sg.expand(c, d.nnz_d), True))
+            self._views[c] = v
+        return v
+
+    def extra_bytes(self) -> int:
+        d = self.device
+        own = 0 if self.energies is d.energies else 24 * self.n_energies * (d.n_orb + 1)
+        if self.sl is not d.sl:
+            own += 32 * d.nnz_d
+        return own
+
+
 def solve_item(lane: Lane, d: Device, e0: int, e1: int, precision: str = "c128",
-               tune: dict | None = None) -> None:
-    """Assemble and solve energies [e0, e1) of d on the lane (current stream = lane)."""
+               tune: dict | None = None, job: Job | None = None) -> None:
+    """Assemble and solve energies [e0, e1) of d (or of ``job`` on d) on the
+    lane; the current stream must be the lane's."""
     c = e1 - e0
     a, xl, xg, corner = lane.views(d, c)
     h = lane.h
     if tune:
         _lib.set_tuning(h, 1, int(tune.get("gemm_tile", 0)))
+    energies = job.energies if job else d.energies
     st = ctypes_stream(lane)
     rc = _lib.load().qx_assemble_system_matrix(
-        h, st, c, d.nnz, d.energies[e0:e1].data_ptr(), d.s_static.data_ptr(),
+        h, st, c, d.nnz, energies[e0:e1].data_ptr(), d.s_static.data_ptr(),
         d.t_static.data_ptr(), None, 0, a.data.data_ptr(), d.nnz)
     _lib.check(rc, h, "assemble_system_matrix")
-    sl, sg = d.sigma_views(c)
+    sl, sg = job.sigma_views(c) if job else d.sigma_views(c)
+    density = job.density if job else d.density
+    trans = job.transmission if job else d.transmission
     RGF(max_batch_size=c, precision=precision).selected_solve(
-        a, sl, sg, out=(xl, xg), corner=corner, density=d.density[e0:e1])
+        a, sl, sg, out=(xl, xg), corner=corner, density=density[e0:e1])
     gl = d.gamma_l.expand(c, *d.gamma_l.shape).contiguous()
     gr = d.gamma_r.expand(c, *d.gamma_r.shape).contiguous()
-    d.transmission[e0:e1].copy_(transmission(corner, gl, gr))
+    trans[e0:e1].copy_(transmission(corner, gl, gr))
 
 
 def ctypes_stream(lane: Lane):
@@ -239,9 +293,17 @@ class UnevenBatch:
         return max(1, min(c, cap, d.n_energies))
 
     def plan(self) -> None:
+        """Split every device's energies into chunks and assign them to lanes.
+
+        A chunk is at most the tuned size and at most a quarter of one
+        lane's fair share of the flops, so a device that dominates the mix
+        is spread over several lanes.
+        """
         items = []
+        total = sum(d.n_energies * d.flops_per_energy() for d in self.devices)
+        share = total / (4 * len(self.lanes))
         for k, d in enumerate(self.devices):
-            c = self.chunk_for(d)
+            c = min(self.chunk_for(d), max(1, int(share // d.flops_per_energy())))
             for e0 in range(0, d.n_energies, c):
                 e1 = min(e0 + c, d.n_energies)
                 items.append(Item(k, e0, e1, (e1 - e0) * d.flops_per_energy()))
