@@ -19,10 +19,10 @@ Execution model (MI355X-first):
 - one item = system-matrix assembly kernel (A = E I - H - Sigma_B on the
   nnz pattern) -> RGF selected solve with fused density and corner-block
   outputs -> transmission;
-- ``UnevenBatch.capture()`` records the whole schedule of all lanes (forked
-  from and joined to one capture stream) into one HIP graph;
-  ``replay()`` launches it with a single call, so the thousands of small
-  kernel launches of small devices cost no host time;
+- ``UnevenBatch.capture()`` records each lane's whole schedule into one HIP
+  graph; ``replay()`` launches the lanes' graphs on their streams (one
+  launch per lane), so the thousands of small kernel launches of small
+  devices cost no host time;
 - ``autotune`` picks, per block size, the GEMM tile shape (64 or 128) and
   the energies per call by timing probe solves; its table is cached.
 
@@ -350,29 +350,45 @@ class UnevenBatch:
             lane.sync_check("UnevenBatch")
 
     def capture(self) -> None:
-        """Record the schedule of all lanes into one HIP graph.
+        """Record each lane's schedule into its own HIP graph.
 
         Runs the schedule eagerly first (sizes every workspace and buffer so
-        nothing is allocated while recording), then records it.
+        nothing is allocated while recording).  One graph per lane, replayed
+        on the lane's stream: graph launches on different streams run
+        concurrently, where the branches of a single multi-stream graph are
+        not guaranteed to.
         """
         self.run_eager()
         torch.cuda.synchronize(self.lanes[0].device)
-        g = torch.cuda.CUDAGraph()
-        cap_stream = torch.cuda.Stream(self.lanes[0].device)
-        with torch.cuda.graph(g, stream=cap_stream):
-            cap = torch.cuda.current_stream(self.lanes[0].device)
-            for lane in self.lanes:
-                lane.stream.wait_stream(cap)
-            for lane in self.lanes:
+        self.graph = []
+        for lane in self.lanes:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=lane.stream):
                 self._run_lane(lane)
-            for lane in self.lanes:
-                cap.wait_stream(lane.stream)
-        self.graph = g
+            self.graph.append(g)
 
     def replay(self) -> None:
+        """Launch every lane's graph on its stream, one launch per lane, each
+        from its own host thread (a graph launch enqueues its kernel nodes
+        from the host, so concurrent launches keep the lanes' queues fed)."""
         if self.graph is None:
             raise RuntimeError("capture() first")
-        self.graph.replay()
+        cur = torch.cuda.current_stream(self.lanes[0].device)
+        for lane in self.lanes:
+            lane.stream.wait_stream(cur)
+
+        def launch(lane, g):
+            with torch.cuda.stream(lane.stream):
+                g.replay()
+
+        threads = [threading.Thread(target=launch, args=(lane, g))
+                   for lane, g in zip(self.lanes, self.graph)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        for lane in self.lanes:
+            cur.wait_stream(lane.stream)
 
     def check(self) -> None:
         torch.cuda.synchronize(self.lanes[0].device)
@@ -392,7 +408,7 @@ class UnevenBatch:
 # autotuner
 
 TILES = (64, 128)
-CHUNKS = (8, 16, 32, 64)
+CHUNKS = (8, 16, 32, 64, 128, 256)
 
 
 def _cache_path() -> Path:

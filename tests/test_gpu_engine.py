@@ -47,7 +47,12 @@ def _check(devs, refs, tol):
 
 def test_eager_matches_direct(batch):
     ws, devs, ub, refs = batch
-    assert sorted(it.device for lane in ub.lanes for it in lane.items).count(1) == 3
+    # the plan covers every energy of every device exactly once
+    for k, d in enumerate(devs):
+        spans = sorted((it.e0, it.e1) for lane in ub.lanes for it in lane.items
+                       if it.device == k)
+        assert spans[0][0] == 0 and spans[-1][1] == d.n_energies
+        assert all(a[1] == b[0] for a, b in zip(spans, spans[1:]))
     ub.run_eager()
     _check(devs, refs, 1e-12)
 
