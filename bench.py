@@ -124,6 +124,24 @@ def cpu_baseline(w, budget_s: float, max_energies: int):
     }
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "zgemm_256_hbm_traffic.json"
+
+
+def measured_traffic(flops_per_launch: float):
+    """HBM bytes per launch of the dominant GEMM from the committed counter
+    measurement (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated on a copy), when
+    it was taken at this launch shape; else None."""
+    try:
+        rec = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    for name, k in rec.get("kernels", {}).items():
+        if name.startswith("zgemm_kernel<double,N,N,4M,64x64>") and \
+                abs(8.0 * 256 ** 3 * 16 - flops_per_launch) < 1.0:
+            return k["traffic_bytes"]
+    return None
+
+
 def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak):
     """Warm-up + timed steps of one arithmetic mode; returns its record."""
     import numpy as np
@@ -167,10 +185,14 @@ def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak):
         achieved = g["flops"] / (g["busy_ms"] / 1e3) / 1e12
         per_launch = g["flops"] / max(g["launches"], 1)
         avg_ms = g["ms"] / max(g["launches"], 1)
+        traffic = measured_traffic(per_launch) if rgf.precision == "c128" else None
         rec["roofline"] = {
             "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
             "kernel": "zgemm_kernel (batched complex GEMM, 64x64 tiles)",
+            "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, calibrated "
+                               "on a 1 GiB copy: profiles/zgemm_256_hbm_traffic.json")
+            if traffic else None,
             "flops_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
             "per_launch_tflops": round(per_launch / (avg_ms / 1e3) / 1e12, 3),
             "busy_ms": round(g["busy_ms"], 2)}
@@ -513,6 +535,42 @@ def serve_workload(args):
     print(json.dumps(result), flush=True)
 
 
+def kernel_probe(args):
+    """Standalone launches for counter runs: a plain device copy of known
+    size (the calibration for HBM byte counters), then the dominant GEMM and
+    the batched inverse at the benchmark's sub-batch shape."""
+    import ctypes
+
+    import torch
+
+    from quatrex_amd import _lib
+    from quatrex_amd.ops import gemm, inv
+
+    dev = torch.device("cuda:0")
+    copy_bytes = 1 << 30
+    src = torch.ones(copy_bytes // 8, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    b, batch = args.block_size, args.probe_batch
+    g = torch.Generator(device=dev).manual_seed(0)
+    a = torch.randn((batch, b, b), dtype=torch.complex128, device=dev, generator=g)
+    bm = torch.randn((batch, b, b), dtype=torch.complex128, device=dev, generator=g)
+    c = torch.empty_like(a)
+    for _ in range(5):
+        gemm(a, bm, out=c)
+    a = a + b * torch.eye(b, dtype=torch.complex128, device=dev)
+    for _ in range(3):
+        inv(a, check=False)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": copy_bytes, "copy_launches": 3,
+                      "gemm": {"m": b, "n": b, "k": b, "batch": batch, "launches": 5,
+                               "compulsory_bytes": 16 * 4 * b * b * batch,
+                               "flops": 8.0 * b ** 3 * batch},
+                      "inverse": {"n": b, "batch": batch, "launches": 3}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -532,7 +590,10 @@ def main():
                     help="seconds of CPU work for the baseline sample (0 disables)")
     ap.add_argument("--no-timer", action="store_true",
                     help="do not bracket kernels with HIP events (no roofline)")
-    ap.add_argument("--workload", default="rgf", choices=["rgf", "obc", "uneven", "serve"],
+    ap.add_argument("--probe-batch", type=int, default=16,
+                    help="probe: matrices per launch (the RGF sub-batch of 64 energies / 4)")
+    ap.add_argument("--workload", default="rgf",
+                    choices=["rgf", "obc", "uneven", "serve", "probe"],
                     help="rgf: the selected solve (default); obc: lead surface Green's "
                          "functions (use --energies 1024 --block-size 256); uneven: mixed "
                          "device sizes in one graph launch")
@@ -563,6 +624,9 @@ def main():
         return
     if args.workload == "serve":
         serve_workload(args)
+        return
+    if args.workload == "probe":
+        kernel_probe(args)
         return
 
     import torch

@@ -30,6 +30,8 @@
 // MFMA 16x16x4 layouts as in gemm.hip: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; result element e of lane l is D[l/16 + 4e][l%16] (f64) or
 // D[4(l/16) + e][l%16] (f32).
+#include <cstdlib>
+
 #include "qx_internal.h"
 
 namespace qx {
@@ -452,7 +454,14 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const
   if (timer) timer->begin(s, &ev0);
 
   const bool big = n > 256;
-  const int NB = big ? 16 : 32;
+  // Panel width: 32 columns up to n = 256 (16 above, where two rows per
+  // thread double the per-column work); QX_INV_NB=16|32 overrides for n <= 256.
+  static const int forced_nb = []() {
+    const char* e = std::getenv("QX_INV_NB");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 16 || v == 32) ? v : 0;
+  }();
+  const int NB = big ? 16 : (forced_nb ? forced_nb : 32);
   const int npan = (n + NB - 1) / NB;
   // start in the buffer that makes the last panel write the scratch buffer
   const int s0 = 1 ^ (npan % 2);
@@ -467,9 +476,11 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const
              "inv: precision must be QX_PREC_C128, QX_PREC_C64 or QX_PREC_C128_3M");
   if (precision == QX_PREC_C64) {
     if (big) run_inverse<float, 16, 2>(s, batch, n, X, scr, s0, info);
+    else if (NB == 16) run_inverse<float, 16, 1>(s, batch, n, X, scr, s0, info);
     else run_inverse<float, 32, 1>(s, batch, n, X, scr, s0, info);
   } else {
     if (big) run_inverse<double, 16, 2>(s, batch, n, X, scr, s0, info);
+    else if (NB == 16) run_inverse<double, 16, 1>(s, batch, n, X, scr, s0, info);
     else run_inverse<double, 32, 1>(s, batch, n, X, scr, s0, info);
   }
   QX_HIP_CHECK(hipGetLastError());
