@@ -214,10 +214,12 @@ def _block_view(arr: np.ndarray, axis: int, num_blocks: int) -> np.ndarray:
 class Spectral(OBCSolver):
     """Spectral OBC (spectral.py:15-526).
 
-    The NEVP moments run on the GPU (``quatrex_amd.nevp.Beyn``); mode
-    classification (residuals, group velocities, decay) is evaluated from
-    device products; the per-energy pseudo-inverse and the refinement
-    inversions run on the GPU.
+    Device-resident after the NEVP: mode upscaling, residuals, group
+    velocities and the decay classification are device tensors (products on
+    the MFMA GEMM), the per-energy pseudo-inverse of the reference's Python
+    loop is one batched masked normal-equation solve, and the refinement
+    inversions run on the GPU.  Beyn's small SVD/eig stays on the host, as
+    the reference's default ``eig_compute_location="numpy"``.
     """
 
     def __init__(self, nevp, block_sections: int = 1, min_decay: float = 1e-3,
@@ -262,66 +264,91 @@ class Spectral(OBCSolver):
         return tuple(np.ascontiguousarray(b) for b in sel)
 
     @staticmethod
-    def _compute_dE_dk(ws, vs, a_xx):
-        """spectral.py:160-196 on host arrays."""
-        b = len(a_xx) // 2
-        with np.errstate(divide="ignore", invalid="ignore"):
-            return -sum((1j * n) * np.diagonal(np.conj(np.swapaxes(vs, -1, -2)) @ a @ vs,
-                                               axis1=-2, axis2=-1) * ws ** n
-                        for a, n in zip(a_xx, range(-b, b + 1)))
+    def _compute_dE_dk(ws, vs, a_vs):
+        """spectral.py:160-196: dE/dk_m = -sum_n i n (v_m^H a_n v_m) w_m^n, with
+        the products a_n v precomputed (device tensors)."""
+        b = len(a_vs) // 2
+        out = torch.zeros_like(ws)
+        for av, n in zip(a_vs, range(-b, b + 1)):
+            if n == 0:
+                continue
+            quad = (vs.conj() * av).sum(-2)  # diag(V^H a_n V)
+            out = out - (1j * n) * quad * ws ** n
+        return out
 
     def _upscale_eigenmodes(self, ws, vs):
-        """spectral.py:322-377 on host arrays."""
-        with np.errstate(divide="ignore", invalid="ignore"):
-            if self.block_sections == 1:
-                return ws, vs / np.linalg.norm(vs, axis=-2, keepdims=True)
-            batch = vs.shape[:-2]
-            nb = len(batch)
-            sub, modes = vs.shape[-2], vs.shape[-1]
-            wsu = np.moveaxis(np.array([ws ** n for n in range(self.block_sections)]), 0, nb)
-            vsu = (wsu[..., :, np.newaxis, :] * vs[..., np.newaxis, :, :]).reshape(
-                *batch, sub * self.block_sections, modes)
-            vsu = vsu / np.linalg.norm(vsu, axis=-2, keepdims=True)
+        """spectral.py:322-377 on device tensors."""
+        if self.block_sections == 1:
+            return ws, vs / torch.linalg.norm(vs, dim=-2, keepdim=True)
+        batch = vs.shape[:-2]
+        sub, modes = vs.shape[-2], vs.shape[-1]
+        wsu = torch.stack([ws ** n for n in range(self.block_sections)], dim=len(batch))
+        vsu = (wsu[..., :, None, :] * vs[..., None, :, :]).reshape(
+            *batch, sub * self.block_sections, modes)
+        vsu = vsu / torch.linalg.norm(vsu, dim=-2, keepdim=True)
         return ws ** self.block_sections, vsu
 
     def _find_reflected_modes(self, ws, vs, a_xx):
-        """spectral.py:198-320 (no injection vectors), host arrays."""
-        with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
-            prod = sum(a @ vs * ws[..., np.newaxis, :] ** (i - len(a_xx) // 2)
-                       for i, a in enumerate(a_xx))
-            res = np.linalg.norm(prod, axis=-2)
-            res /= np.linalg.norm(vs, axis=-2)
-            if self.residual_normalization:
-                res /= np.abs(ws)
-            dk = self._compute_dE_dk(ws, vs, a_xx)
-            ks = -1j * np.log(ws)
-        dk = np.nan_to_num(dk, nan=0, posinf=0, neginf=0)
-        ks = np.nan_to_num(ks, nan=0, posinf=0, neginf=0)
-        eta = np.finfo(dk.dtype).eps
-        ratio = np.abs(dk.real) / (np.abs(dk.imag) + eta)
-        prop = (np.abs(ks.imag) < self.min_decay) & (self.min_propagation < ratio) & (dk.real < 0)
+        """spectral.py:198-320 (no injection vectors) on device tensors.
+
+        The products a_n V run on the MFMA GEMM; residuals, group velocities
+        and the decay classification are elementwise on the device.
+        """
+        nb = len(a_xx) // 2
+        a_vs = [gemm(a.contiguous(), vs.contiguous()) for a in a_xx]
+        prod = sum(av * ws[..., None, :] ** (i - nb) for i, av in enumerate(a_vs))
+        res = torch.linalg.norm(prod, dim=-2) / torch.linalg.norm(vs, dim=-2)
+        if self.residual_normalization:
+            res = res / ws.abs()
+        dk = torch.nan_to_num(self._compute_dE_dk(ws, vs, a_vs), nan=0.0, posinf=0.0,
+                              neginf=0.0)
+        ks = torch.nan_to_num(-1j * torch.log(ws), nan=0.0, posinf=0.0, neginf=0.0)
+        eta = torch.finfo(torch.float64).eps
+        ratio = dk.real.abs() / (dk.imag.abs() + eta)
+        prop = (ks.imag.abs() < self.min_decay) & (self.min_propagation < ratio) & (dk.real < 0)
         dec = ks.imag < -self.min_decay
-        dec |= (self.min_propagation >= ratio) & (ks.imag < -self.eta_decay)
-        dec &= ks.imag > -self.max_decay
+        dec = dec | ((self.min_propagation >= ratio) & (ks.imag < -self.eta_decay))
+        dec = dec & (ks.imag > -self.max_decay)
         return (prop | dec) & (res < self.residual_tolerance)
+
+    @staticmethod
+    def _x_ii_a_ij(ws, vs, mask):
+        """x_ii a_ij = V_r diag(1/w_r) V_r^+ for every energy at once
+        (spectral.py:412-421).
+
+        The selected modes of each energy are packed to the front (at most b
+        of them); the padding columns are zero, and the identity on their
+        diagonal of V^H V leaves the pseudo-inverse of the selected columns
+        unchanged and its padding rows zero.
+        """
+        n, b = vs.shape[0], vs.shape[-2]
+        k_max = int(mask.sum(-1).max().item()) if mask.numel() else 0
+        if k_max == 0:
+            return torch.zeros((n, b, b), dtype=vs.dtype, device=vs.device)
+        order = torch.argsort(mask.to(torch.int8), dim=-1, descending=True, stable=True)
+        idx = order[..., :k_max]
+        sel = torch.gather(mask, -1, idx)
+        vr = torch.gather(vs, -1, idx[..., None, :].expand(n, b, k_max))
+        vr = torch.where(sel[..., None, :], vr, torch.zeros_like(vr)).contiguous()
+        w = torch.gather(ws, -1, idx)
+        w_inv = torch.where(sel, 1.0 / torch.where(sel, w, torch.ones_like(w)),
+                            torch.zeros_like(w))
+        gram = gemm(vr, vr, op_a="C") + torch.diag_embed((~sel).to(vs.dtype))
+        pinv = gemm(inv(gram), vr, op_b="C")
+        return gemm((vr * w_inv[..., None, :]).contiguous(), pinv)
 
     def __call__(self, a_ii, a_ij, a_ji, contact: str = ""):
         (aii, aij, aji), shape, squeeze = _batched3(a_ii, a_ij, a_ji)
         blocks = self._extract_subblocks(aji, aii, aij)
         ws, vs = self.nevp(blocks)
-        ws = ws.cpu().numpy() if isinstance(ws, torch.Tensor) else np.asarray(ws)
-        vs = vs.cpu().numpy() if isinstance(vs, torch.Tensor) else np.asarray(vs)
+        dev = aii.device
+        ws = torch.as_tensor(ws).to(dev, torch.complex128)
+        vs = torch.as_tensor(vs).to(dev, torch.complex128)
         ws, vs = self._upscale_eigenmodes(ws, vs)
-        host = [t.cpu().numpy() for t in (aji, aii, aij)]
-        mask = self._find_reflected_modes(ws, vs, host)
-        # x_ii a_ij = V_r diag(1/w) V_r^+ per energy (spectral.py:412-421)
+        mask = self._find_reflected_modes(ws, vs, [aji, aii, aij])
         b = aii.shape[-1]
-        xa = np.zeros((aii.shape[0], b, b), dtype=np.complex128)
-        for i in range(aii.shape[0]):
-            vr, w = vs[i][:, mask[i]], ws[i][mask[i]]
-            v_inv = np.linalg.inv(vr.conj().T @ vr) @ vr.conj().T
-            xa[i] = (vr / w) @ v_inv
-        x = inv(aii + gemm(aji, as_device(xa, aii.device)))
+        xa = self._x_ii_a_ij(ws, vs, mask)
+        x = inv(aii + gemm(aji, xa))
         for _ in range(self.num_ref_iterations - 1):
             x = inv(aii - mm(aji, x, aij))
         x = inv(aii - mm(aji, x, aij))

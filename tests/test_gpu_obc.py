@@ -134,3 +134,52 @@ def test_spectral_full_matches_sancho_rubio(device):
     assert rel_err(np_(x), x_ref) < 1e-8
     x_sr = SanchoRubio(max_iterations=500, convergence_tol=1e-13)(a_ii, a_ij, a_ji)
     assert rel_err(np_(x), np_(x_sr)) < 1e-7
+
+
+def _dev(x, device):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.complex128, device=device)
+
+
+@pytest.mark.parametrize("b", [8, 40])
+def test_spectral_mode_classification_matches_oracle(device, b):
+    """Same eigenpairs in, same reflected-mode mask and x_ii a_ij out."""
+    a_xx = lead(b=b, eta=1e-3, seed=b)
+    ws, vs = onevp.Full()(a_xx)
+    osp = oobc.Spectral(onevp.Full(), residual_tolerance=1e-6)
+    ows, ovs = osp._upscale_eigenmodes(ws, vs)
+    omask = osp._find_reflected_modes(ows, ovs, list(a_xx))
+    gsp = Spectral(Full(), residual_tolerance=1e-6)
+    gws, gvs = gsp._upscale_eigenmodes(_dev(ws, device), _dev(vs, device))
+    assert rel_err(np_(gvs), ovs) < 1e-13
+    gmask = gsp._find_reflected_modes(gws, gvs, [_dev(a, device) for a in a_xx])
+    assert np.array_equal(np_(gmask), omask)
+    assert np.all(omask.sum(-1) == b)  # one reflected mode per orbital
+    x_a = np.zeros((ws.shape[0], b, b), complex)
+    for i in range(ws.shape[0]):
+        vr, w = ovs[i][:, omask[i]], ows[i][omask[i]]
+        x_a[i] = vr / w @ np.linalg.inv(vr.conj().T @ vr) @ vr.conj().T
+    got = np_(gsp._x_ii_a_ij(gws, gvs, gmask))
+    assert rel_err(got, x_a) < 1e-10
+
+
+def test_spectral_beyn_matches_oracle(device):
+    a_ji, a_ii, a_ij = lead(b=8, eta=1e-3, rank=4)
+    kw = dict(r_o=10.0, r_i=0.99, m_0=8, num_quad_points=64)
+    ref = oobc.Spectral(onevp.Beyn(**kw), residual_tolerance=1e-1, max_decay=20)(a_ii, a_ij, a_ji)
+    got = Spectral(Beyn(**kw), residual_tolerance=1e-1, max_decay=20)(a_ii, a_ij, a_ji)
+    assert rel_err(np_(got), ref) < 1e-9
+
+
+@pytest.mark.parametrize("block_sections", [2])
+def test_spectral_block_sections_matches_oracle(device, block_sections):
+    from tests.test_oracle_leads import _periodic_layer
+
+    rng = np.random.default_rng(9)
+    a_ii, a_ij, a_ji = _periodic_layer(rng, block_sections, 4, 3)
+    z = (np.array([-0.3, 0.2, 0.5]) + 1e-3j)[:, None, None]
+    m_ii = z * np.eye(a_ii.shape[-1]) - a_ii
+    ref = oobc.Spectral(onevp.Full(), block_sections=block_sections, residual_tolerance=1e-6)(
+        m_ii, -a_ij, -a_ji)
+    got = Spectral(Full(), block_sections=block_sections, residual_tolerance=1e-6)(
+        m_ii, -a_ij, -a_ji)
+    assert rel_err(np_(got), ref) < 1e-9
