@@ -5,9 +5,10 @@ One step = ``RGF.selected_solve`` (G^R diagonal, G^<, G^> blocks) for all
 512 energies of the synthetic production workload (N = 64 blocks of
 256 x 256, complex128 storage).  A, Sigma^< and Sigma^> of all energies are
 resident in HBM before timing (102 GB + 2 x 34 GB); the energies are solved
-in chunks of ``--chunk`` whose outputs (G^<, G^> with the BTD pattern of A,
-G^R block-diagonal) are written into one reused set of output stacks, since
-the outputs of all 512 energies (~240 GB) do not fit next to the inputs.
+in chunks of ``--chunk`` (default 128) whose outputs (G^<, G^> with the BTD
+pattern of A, G^R block-diagonal) are written into one reused set of output
+stacks, since the outputs of all 512 energies (~240 GB) do not fit next to
+the inputs.
 
 The primary arithmetic (``--precision``, default complex128 with the 4M
 GEMM split) gives ``value``; every mode of ``--compare`` (default:
@@ -137,7 +138,7 @@ def measured_traffic(flops_per_launch: float):
         return None
     for name, k in rec.get("kernels", {}).items():
         if name.startswith("zgemm_kernel<double,N,N,4M,64x64>") and \
-                abs(8.0 * 256 ** 3 * 16 - flops_per_launch) < 1.0:
+                abs(k.get("flops_per_launch", -1.0) - flops_per_launch) < 1.0:
             return k["traffic_bytes"]
     return None
 
@@ -582,7 +583,7 @@ def main():
     ap.add_argument("--energies", type=int, default=512)
     ap.add_argument("--blocks", type=int, default=64)
     ap.add_argument("--block-size", type=int, default=256)
-    ap.add_argument("--chunk", type=int, default=64)
+    ap.add_argument("--chunk", type=int, default=128)
     ap.add_argument("--accuracy-energies", type=int, default=8,
                     help="energies on which modes are compared (0 disables)")
     ap.add_argument("--seed", type=int, default=1)
@@ -590,8 +591,8 @@ def main():
                     help="seconds of CPU work for the baseline sample (0 disables)")
     ap.add_argument("--no-timer", action="store_true",
                     help="do not bracket kernels with HIP events (no roofline)")
-    ap.add_argument("--probe-batch", type=int, default=16,
-                    help="probe: matrices per launch (the RGF sub-batch of 64 energies / 4)")
+    ap.add_argument("--probe-batch", type=int, default=32,
+                    help="probe: matrices per launch (the RGF sub-batch: 128 energies / 4)")
     ap.add_argument("--workload", default="rgf",
                     choices=["rgf", "obc", "uneven", "serve", "probe"],
                     help="rgf: the selected solve (default); obc: lead surface Green's "

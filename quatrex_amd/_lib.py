@@ -15,7 +15,9 @@ from pathlib import Path
 import numpy as np
 import torch
 
-LIB_PATH = Path(__file__).resolve().parent / "libquatrex_amd.so"
+# QX_LIBRARY selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = Path(os.environ.get("QX_LIBRARY") or
+                Path(__file__).resolve().parent / "libquatrex_amd.so")
 
 QX_OK = 0
 QX_NOT_CONVERGED = 1

@@ -43,7 +43,13 @@
 namespace qx {
 namespace {
 
-constexpr int BK = 16, PAD = 1;
+#ifndef QX_GEMM_BK
+#define QX_GEMM_BK 16
+#endif
+#ifndef QX_GEMM_PAD
+#define QX_GEMM_PAD 1
+#endif
+constexpr int BK = QX_GEMM_BK, PAD = QX_GEMM_PAD;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
