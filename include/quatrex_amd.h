@@ -184,6 +184,20 @@ int qx_contact_self_energy(qx_handle* handle, void* stream, int64_t n, int b,
                            const double* occupancies, void* sigma_r,
                            void* sigma_l, void* sigma_g);
 
+/* Ozaki-style split GEMM (no counterpart in the reference; an alternative
+ * arithmetic for the products of rgf.py): C = alpha op(A) op(B) + beta C for
+ * complex128 data, computed on bf16 matrix cores from `slices` (2..7)
+ * 7-bit digit slices of every row of op(A) and column of op(B), with exact
+ * fp32 accumulation of each digit product; the relative error is a small
+ * multiple of 2^(-7 slices) (6e-6 at 3 slices, 5e-14 at 7).
+ * Same argument conventions as qx_zgemm_batched; k <= 4096.  Uses the
+ * handle's workspace for the digits. */
+int qx_zgemm_ozaki(qx_handle* handle, void* stream, int64_t batch, int m, int n,
+                   int k, int op_a, int op_b, const double* alpha,
+                   const void* a, int lda, int64_t stride_a, const void* b,
+                   int ldb, int64_t stride_b, const double* beta, void* c,
+                   int ldc, int64_t stride_c, int slices);
+
 /* Execution tuning of a handle (no counterpart in the reference): streams =
  * concurrent sub-batches of one RGF call (1..8, default 4, env
  * QX_RGF_STREAMS); gemm_tile = GEMM tile shape, 0 automatic, 64 or 128

@@ -50,7 +50,7 @@ EXPORTED_SYMBOLS = (
     "qx_abi_version", "qx_create", "qx_destroy", "qx_last_error", "qx_workspace_query",
     "qx_sync_check", "qx_rgf_selected_inv", "qx_rgf_selected_solve", "qx_zgemm_batched",
     "qx_inv_batched", "qx_sancho_rubio", "qx_beyn_moments", "qx_timer_start", "qx_timer_stop",
-    "qx_contact_self_energy", "qx_assemble_system_matrix", "qx_set_tuning",
+    "qx_contact_self_energy", "qx_assemble_system_matrix", "qx_set_tuning", "qx_zgemm_ozaki",
 )
 
 
@@ -124,6 +124,9 @@ def load() -> ctypes.CDLL:
         lib.qx_assemble_system_matrix.argtypes = [vp, vp, i64, i64, vp, vp, vp, vp, i64, vp,
                                                    i64]
         lib.qx_set_tuning.argtypes = [vp, i32, i32]
+        lib.qx_zgemm_ozaki.argtypes = [
+            vp, vp, i64, i32, i32, i32, i32, i32, ctypes.POINTER(f64), vp, i32, i64, vp, i32,
+            i64, ctypes.POINTER(f64), vp, i32, i64, i32]
         lib.qx_timer_start.argtypes = [vp, vp]
         lib.qx_timer_stop.argtypes = [vp, vp, ctypes.POINTER(f64)]
         for name in EXPORTED_SYMBOLS:

@@ -544,4 +544,22 @@ int qx_set_tuning(qx_handle* h, int streams, int gemm_tile) {
   });
 }
 
+int qx_zgemm_ozaki(qx_handle* h, void* stream, int64_t batch, int m, int n, int k, int op_a,
+                   int op_b, const double* alpha, const void* a, int lda, int64_t stride_a,
+                   const void* b, int ldb, int64_t stride_b, const double* beta, void* c,
+                   int ldc, int64_t stride_c, int slices) {
+  return guarded(h, [&]() {
+    QX_REQUIRE(h, "null handle");
+    QX_REQUIRE(alpha && beta, "zgemm_ozaki: null alpha/beta");
+    QX_REQUIRE(batch >= 0 && m >= 0 && n >= 0 && k >= 1, "zgemm_ozaki: bad sizes");
+    const size_t bytes = qx::ozaki_workspace_bytes(batch, m, n, k, slices < 2 ? 2 : slices);
+    void* ws = h->ensure_ws(bytes);
+    qx::zgemm_ozaki(static_cast<hipStream_t>(stream), batch, m, n, k, op_a, op_b,
+                    qx::Cplx{alpha[0], alpha[1]}, qx::Mat(a, stride_a, lda),
+                    qx::Mat(b, stride_b, ldb), qx::Cplx{beta[0], beta[1]},
+                    qx::Mat(c, stride_c, ldc), qx::Mat(c, stride_c, ldc), slices, ws, h->ws_bytes);
+    return QX_OK;
+  });
+}
+
 }  // extern "C"

@@ -63,6 +63,13 @@ void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int op
               Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
               const Mat& C, int precision);
 
+// Ozaki-split GEMM on bf16 matrix cores: C = alpha op(A) op(B) + beta Cin
+// from `slices` 7-bit digit slices per operand (2..7); ws holds the digits.
+size_t ozaki_workspace_bytes(int64_t batch, int m, int n, int k, int slices);
+void zgemm_ozaki(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+                 Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
+                 const Mat& C, int slices, void* ws, size_t ws_bytes);
+
 // X = inv(A - S) with partial pivoting (S may be zero).  info[e] is set to a
 // nonzero value for an exactly singular matrix.  n <= kInvMaxN.  The scratch
 // holds, per matrix, an n x n complex buffer (buf + e * buf_stride) and 3 n

@@ -66,6 +66,37 @@ def gemm(a, b, op_a: str = "N", op_b: str = "N", alpha=1.0, beta=0.0, out=None,
     return out
 
 
+def gemm_ozaki(a, b, op_a: str = "N", op_b: str = "N", alpha=1.0, beta=0.0, out=None,
+               slices: int = 7) -> torch.Tensor:
+    """Like ``gemm`` (complex128 in and out), computed on bf16 matrix cores from
+    ``slices`` 7-bit digit slices of the operands (Ozaki-style split)."""
+    a = as_device(a)
+    b = as_device(b, a.device)
+    ops = {"N": _lib.OP_N, "C": _lib.OP_C}
+    oa, ob = ops[op_a], ops[op_b]
+    m, k = (a.shape[-2], a.shape[-1]) if oa == _lib.OP_N else (a.shape[-1], a.shape[-2])
+    kb, n = (b.shape[-2], b.shape[-1]) if ob == _lib.OP_N else (b.shape[-1], b.shape[-2])
+    if k != kb:
+        raise ValueError(f"inner dimensions differ: {k} != {kb}")
+    if a.shape[:-2] != b.shape[:-2]:
+        raise ValueError("batch shapes of a and b differ")
+    a3, b3 = _flat(a), _flat(b)
+    if out is None:
+        if beta != 0:
+            raise ValueError("beta != 0 needs out")
+        out = torch.empty(a.shape[:-2] + (m, n), dtype=torch.complex128, device=a.device)
+    c3 = _flat(out)
+    al = (ctypes.c_double * 2)(complex(alpha).real, complex(alpha).imag)
+    be = (ctypes.c_double * 2)(complex(beta).real, complex(beta).imag)
+    h, st = _lib.handle(a.device), _lib.stream_ptr(a.device)
+    rc = _lib.load().qx_zgemm_ozaki(
+        h, st, a3.shape[0], m, n, k, oa, ob, al, a3.data_ptr(), a3.shape[-1], a3.stride(0),
+        b3.data_ptr(), b3.shape[-1], b3.stride(0), be, c3.data_ptr(), c3.shape[-1],
+        c3.stride(0), int(slices))
+    _lib.check(rc, h, "gemm_ozaki")
+    return out
+
+
 def mm(*factors, precision: str = "c128") -> torch.Tensor:
     """Left-to-right product of several stacks (as ``a @ b @ c`` in NumPy)."""
     out = factors[0]
