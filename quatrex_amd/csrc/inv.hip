@@ -7,8 +7,9 @@
 // that the O(n^3) part runs on every CU of the chip while the sequential
 // pivoting chain stays short:
 //   for each panel of NB columns (NB = 32 for n <= 256, 16 up to n = 512)
-//     panel kernel  (one workgroup per matrix): each thread holds one or two
-//        rows of the n x NB panel in registers; per column the pivot (first
+//     panel kernel  (one workgroup per matrix): for n <= 256 four threads
+//        hold one row of the 32-column panel (8 columns each, inv_panel4);
+//        above, each thread holds one or two rows (inv_panel); per column the pivot (first
 //        row of maximal |re| + |im| among the unreduced rows, compared through
 //        a 32-bit key: float magnitude with 9 low mantissa bits replaced by the
 //        row index) is found with wave shuffles plus one LDS exchange, the
@@ -285,6 +286,179 @@ __global__ __launch_bounds__(NT) void inv_panel(PanelArgs g) {
   }
 }
 
+// Panel kernel for n <= 256 with four threads per row (1024 threads): thread
+// t holds columns 8 q .. 8 q + 7 (q = t % 4) of row t / 4 of the 32-column
+// panel, so each elimination step costs a thread 8 complex updates instead
+// of 32, and the four threads of a row are neighbouring lanes of one wave
+// (the active column's value reaches them by a shuffle).  The column loop is
+// unrolled, so every register index is a compile-time constant and nothing
+// is rotated.  Pivot choice, outputs and the row / column gathers are those
+// of inv_panel.
+constexpr int NT4 = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
+  constexpr int NB = 32, CQ = 8;
+  __shared__ T prow_r[NB], prow_i[NB], krow_r[NB], krow_i[NB];
+  __shared__ unsigned red[NT4 / 64];
+  __shared__ int perm_l[kInvMaxN];
+  __shared__ int sig_l[kInvMaxN];
+  const int64_t e = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r = tid >> 2, q = tid & 3, base = lane & ~3;
+  const int n = g.n, k0 = g.k0, nbw = g.nbw;
+  const z2* Xc = g.Xc + e * g.sXc;
+  z2* Xn = g.Xn + e * g.sXn;
+  int* perm = g.ints + e * g.sI;
+  int* sig = perm + n;
+  int* idx = sig + n;
+
+  T rr[CQ], ri[CQ];
+#pragma unroll
+  for (int c = 0; c < CQ; ++c) {
+    z2 v = make_double2(0.0, 0.0);
+    const int col = q * CQ + c;
+    if (r < n && col < nbw) v = Xc[(int64_t)r * g.ldc + k0 + col];
+    rr[c] = (T)v.x;
+    ri[c] = (T)v.y;
+  }
+  // pivot candidates of the first panel column (quarter 0, register 0)
+  unsigned key = 0;
+  if (q == 0 && r >= k0 && r < n) key = pivot_key((double)rr[0], (double)ri[0], r);
+  key = wave_max(key);
+  if (lane == 0) red[w] = key;
+  __syncthreads();
+
+#pragma unroll
+  for (int qq = 0; qq < 4; ++qq) {
+#pragma unroll
+    for (int c = 0; c < CQ; ++c) {
+      const int kk = qq * CQ + c;
+      if (kk < nbw) {
+        const int k = k0 + kk;
+        unsigned kmax = red[0];
+#pragma unroll
+        for (int u = 1; u < NT4 / 64; ++u) kmax = red[u] > kmax ? red[u] : kmax;
+        int p = 511 - (int)(kmax & 0x1FFu);
+        if (p < k || p >= n) p = k;
+        // pivot row: 1 / x_pk to all four threads of the row, scaled row to LDS
+        const double pr = __shfl((double)rr[c], base + qq, 64);
+        const double pi = __shfl((double)ri[c], base + qq, 64);
+        if (r == p) {
+          double vr, vi;
+          crecip(pr, pi, vr, vi);
+#pragma unroll
+          for (int cc = 0; cc < CQ; ++cc) {
+            if (q == qq && cc == c) {
+              prow_r[kk] = (T)vr;
+              prow_i[kk] = (T)vi;
+            } else {
+              const double a = (double)rr[cc], b = (double)ri[cc];
+              prow_r[q * CQ + cc] = (T)(a * vr - b * vi);
+              prow_i[q * CQ + cc] = (T)(a * vi + b * vr);
+            }
+          }
+        }
+        if (r == k && p != k) {
+#pragma unroll
+          for (int cc = 0; cc < CQ; ++cc) {
+            krow_r[q * CQ + cc] = rr[cc];
+            krow_i[q * CQ + cc] = ri[cc];
+          }
+        }
+        if (tid == 0) {
+          perm_l[kk] = p;
+          if ((kmax >> 9) == 0u) g.info[e] = 1;  // no nonzero pivot: singular
+        }
+        __syncthreads();
+
+        if (r < n) {
+          if (r == k) {
+#pragma unroll
+            for (int cc = 0; cc < CQ; ++cc) {
+              rr[cc] = prow_r[q * CQ + cc];
+              ri[cc] = prow_i[q * CQ + cc];
+            }
+          } else if (r == p) {
+#pragma unroll
+            for (int cc = 0; cc < CQ; ++cc) {
+              rr[cc] = krow_r[q * CQ + cc];
+              ri[cc] = krow_i[q * CQ + cc];
+            }
+          }
+        }
+        // multiplier of this row: its (swapped-in) value in the pivot column
+        const T fr = (T)__shfl((double)rr[c], base + qq, 64);
+        const T fi = (T)__shfl((double)ri[c], base + qq, 64);
+        if (r < n && r != k) {
+#pragma unroll
+          for (int cc = 0; cc < CQ; ++cc) {
+            const T br = prow_r[q * CQ + cc], bi = prow_i[q * CQ + cc];
+            if (q == qq && cc == c) {
+              rr[cc] = -(fr * br - fi * bi);
+              ri[cc] = -(fr * bi + fi * br);
+            } else {
+              rr[cc] -= fr * br - fi * bi;
+              ri[cc] -= fr * bi + fi * br;
+            }
+          }
+        }
+        // pivot candidates of the next column
+        key = 0;
+        const int kn = kk + 1;
+        if (kn < nbw && r > k && r < n && q == kn / CQ) {
+          const int cn = kn % CQ;  // compile-time after unrolling
+          key = pivot_key((double)rr[cn], (double)ri[cn], r);
+        }
+        key = wave_max(key);
+        if (lane == 0) red[w] = key;
+        __syncthreads();
+      }
+    }
+  }
+
+  if (r < n) {
+#pragma unroll
+    for (int c = 0; c < CQ; ++c) {
+      const int col = q * CQ + c;
+      if (col < nbw)
+        Xn[(int64_t)r * g.ldn + k0 + col] = make_double2((double)rr[c], (double)ri[c]);
+    }
+  }
+  // composed row gather of this panel, pivot record
+  for (int u = tid; u < n; u += NT4) sig_l[u] = u;
+  if (g.last)
+    for (int u = tid; u < k0; u += NT4) perm_l[NB + u] = perm[u];  // earlier panels
+  __syncthreads();
+  if (tid == 0) {
+    for (int kk = 0; kk < nbw; ++kk) {
+      const int pq = perm_l[kk];
+      const int t = sig_l[k0 + kk];
+      sig_l[k0 + kk] = sig_l[pq];
+      sig_l[pq] = t;
+    }
+  }
+  __syncthreads();
+  for (int u = tid; u < n; u += NT4) sig[u] = sig_l[u];
+  for (int u = tid; u < nbw; u += NT4) perm[k0 + u] = perm_l[u];
+  if (g.last) {
+    // column gather of the whole inverse: reverse order of all pivots
+    __syncthreads();
+    for (int u = tid; u < n; u += NT4) sig_l[u] = u;
+    __syncthreads();
+    if (tid == 0) {
+      for (int k = n - 1; k >= 0; --k) {
+        const int pq = (k >= k0) ? perm_l[k - k0] : perm_l[NB + k];
+        const int t = sig_l[k];
+        sig_l[k] = sig_l[pq];
+        sig_l[pq] = t;
+      }
+    }
+    __syncthreads();
+    for (int u = tid; u < n; u += NT4) idx[u] = sig_l[u];
+  }
+}
+
 struct UpdArgs {
   int n, k0, nbw, tiles;
   const z2* Xc;
@@ -408,6 +582,16 @@ __global__ __launch_bounds__(NT) void inv_gather(int n, const z2* Y, int64_t sY,
   }
 }
 
+// Four threads per row for 32-column panels of n <= 256 (QX_INV_PANEL=1
+// selects the one-thread-per-row kernel).
+bool use_panel4() {
+  static const bool on = []() {
+    const char* e = std::getenv("QX_INV_PANEL");
+    return !(e && std::atoi(e) == 1);
+  }();
+  return on;
+}
+
 template <typename T, int NB, int RPT>
 void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScratch& scr,
                  int s0, int* info) {
@@ -419,7 +603,10 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
     const int k0 = p * NB, nbw = min(NB, n - k0), nxt = 1 - cur;
     PanelArgs pa{n, k0, nbw, p == npan - 1 ? 1 : 0, B[cur].p, B[cur].stride, B[cur].ld,
                  B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, info};
-    hipLaunchKernelGGL((inv_panel<T, NB, RPT>), dim3((unsigned)batch), dim3(NT), 0, s, pa);
+    if (NB == 32 && RPT == 1 && use_panel4())
+      hipLaunchKernelGGL((inv_panel4<T>), dim3((unsigned)batch), dim3(NT4), 0, s, pa);
+    else
+      hipLaunchKernelGGL((inv_panel<T, NB, RPT>), dim3((unsigned)batch), dim3(NT), 0, s, pa);
     if (n > nbw) {
       UpdArgs ua{n, k0, nbw, tn * tn, B[cur].p, B[cur].stride, B[cur].ld,
                  B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride};
