@@ -90,6 +90,38 @@ __device__ inline unsigned wave_max(unsigned v) {
   return v;
 }
 
+// Wave-wide unsigned max without LDS: DPP butterflies inside each quad and
+// each row of 16 lanes, then the four row maxima read into scalars.  Every
+// lane of the wave must be active.
+__device__ inline unsigned wave_max_dpp(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, false));   // quad xor 1
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xf, 0xf, false));   // quad xor 2
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xf, 0xf, false));  // row_ror 4
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xf, 0xf, false));  // row_ror 8
+  const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)v, 0);
+  const unsigned b = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
+  const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)v, 32);
+  const unsigned d = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+  return max(max(a, b), max(c, d));
+}
+
+// Value of lane QQ of each quad, to all four lanes of the quad (DPP).
+template <int QQ>
+__device__ inline double quad_bcast(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), QQ * 0x55, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), QQ * 0x55, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline double quad_bcast(double x, int qq) {
+  switch (qq) {
+    case 0: return quad_bcast<0>(x);
+    case 1: return quad_bcast<1>(x);
+    case 2: return quad_bcast<2>(x);
+    default: return quad_bcast<3>(x);
+  }
+}
+
 struct PanelArgs {
   int n, k0, nbw, last;
   const z2* Xc;
@@ -290,7 +322,7 @@ __global__ __launch_bounds__(NT) void inv_panel(PanelArgs g) {
 // t holds columns 8 q .. 8 q + 7 (q = t % 4) of row t / 4 of the 32-column
 // panel, so each elimination step costs a thread 8 complex updates instead
 // of 32, and the four threads of a row are neighbouring lanes of one wave
-// (the active column's value reaches them by a shuffle).  The column loop is
+// (the active column's value reaches them by a DPP quad broadcast).  The column loop is
 // unrolled, so every register index is a compile-time constant and nothing
 // is rotated.  Pivot choice, outputs and the row / column gathers are those
 // of inv_panel.
@@ -305,7 +337,7 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
   __shared__ int sig_l[kInvMaxN];
   const int64_t e = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int r = tid >> 2, q = tid & 3, base = lane & ~3;
+  const int r = tid >> 2, q = tid & 3;
   const int n = g.n, k0 = g.k0, nbw = g.nbw;
   const z2* Xc = g.Xc + e * g.sXc;
   z2* Xn = g.Xn + e * g.sXn;
@@ -325,7 +357,7 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
   // pivot candidates of the first panel column (quarter 0, register 0)
   unsigned key = 0;
   if (q == 0 && r >= k0 && r < n) key = pivot_key((double)rr[0], (double)ri[0], r);
-  key = wave_max(key);
+  key = wave_max_dpp(key);
   if (lane == 0) red[w] = key;
   __syncthreads();
 
@@ -342,8 +374,8 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
         int p = 511 - (int)(kmax & 0x1FFu);
         if (p < k || p >= n) p = k;
         // pivot row: 1 / x_pk to all four threads of the row, scaled row to LDS
-        const double pr = __shfl((double)rr[c], base + qq, 64);
-        const double pi = __shfl((double)ri[c], base + qq, 64);
+        const double pr = quad_bcast((double)rr[c], qq);
+        const double pi = quad_bcast((double)ri[c], qq);
         if (r == p) {
           double vr, vi;
           crecip(pr, pi, vr, vi);
@@ -388,8 +420,8 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
           }
         }
         // multiplier of this row: its (swapped-in) value in the pivot column
-        const T fr = (T)__shfl((double)rr[c], base + qq, 64);
-        const T fi = (T)__shfl((double)ri[c], base + qq, 64);
+        const T fr = (T)quad_bcast((double)rr[c], qq);
+        const T fi = (T)quad_bcast((double)ri[c], qq);
         if (r < n && r != k) {
 #pragma unroll
           for (int cc = 0; cc < CQ; ++cc) {
@@ -410,7 +442,7 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
           const int cn = kn % CQ;  // compile-time after unrolling
           key = pivot_key((double)rr[cn], (double)ri[cn], r);
         }
-        key = wave_max(key);
+        key = wave_max_dpp(key);
         if (lane == 0) red[w] = key;
         __syncthreads();
       }
