@@ -16,7 +16,7 @@ c128-3m = complex128 with the 3M split, and mixed = complex64 products and
 inverses + one FP64 Newton-Schulz step per block inverse) is timed the
 same way and reported beside it with its speed-up and its error against the
 complex128 GPU result (per block, electron density diag G^<, transmission
-T(E) = Tr[Gamma_R G^R_{N-1,0} Gamma_L G^A_{0,N-1}]) on the first
+T(E) = Tr[Gamma_L G^R_{0,N-1} Gamma_R G^A_{N-1,0}]) on the first
 ``--accuracy-energies`` energies.  The CPU baseline leg times the NumPy
 oracle on a bounded energy sample and compares both GPU modes with it on
 those energies.
@@ -105,7 +105,7 @@ def cpu_baseline(w, budget_s: float, max_energies: int):
         t1 = time.perf_counter() - t0
         n = int(max(1, min(max_energies - 1, budget_s // max(t1, 1e-3))))
         a, sl, sg = containers(w.energies[1:1 + n])
-        corner = np.zeros((n, bs[-1], bs[0]), complex)
+        corner = np.zeros((n, bs[0], bs[-1]), complex)
         t0 = time.perf_counter()
         xl, xg, xr = solver.selected_solve(a, sl, sg, return_retarded=True, corner=corner)
         dt = time.perf_counter() - t0
@@ -220,7 +220,7 @@ def accuracy_outputs(w, rgf, a, sl, sg, n_acc):
     va, vl, vg = a.stack_view(0, n_acc), sl.stack_view(0, n_acc), sg.stack_view(0, n_acc)
     xl, xg = DSDBCOO.zeros_like(va), DSDBCOO.zeros_like(va)
     xr, _ = empty_btd(bs, (n_acc,), offdiag=False, device=a.device)
-    corner = torch.zeros((n_acc, bs[-1], bs[0]), dtype=torch.complex128, device=a.device)
+    corner = torch.zeros((n_acc, bs[0], bs[-1]), dtype=torch.complex128, device=a.device)
     rgf.selected_solve(va, vl, vg, out=(xl, xg, xr), return_retarded=True, corner=corner)
     gl = 1j * (w.sigma_b_left - w.sigma_b_left.conj().T)
     gr = 1j * (w.sigma_b_right - w.sigma_b_right.conj().T)

@@ -114,9 +114,9 @@ int qx_rgf_selected_inv(qx_handle* handle, void* stream,
  * off-diagonal blocks, and the layer current (n_stack, n_blocks-1) complex
  * when QX_SOLVE_CURRENT is set.  obc_*: OBCBlocks.retarded/lesser/greater.
  * corner (nullable, no counterpart in the reference): receives the retarded
- * corner block G^R_{N-1,0}, (n_stack, b_{N-1}, b_0) contiguous, from the
- * forward recursion g_{j,0} = -x^R_jj A_{j,j-1} g_{j-1,0}; it gives the
- * transmission Tr[Gamma_R G^R_{N-1,0} Gamma_L (G^R_{N-1,0})^H].
+ * corner block G^R_{0,N-1}, (n_stack, b_0, b_{N-1}) contiguous, from the
+ * forward recursion g_{0,j} = -g_{0,j-1} A_{j-1,j} x^R_jj; it gives the
+ * transmission Tr[Gamma_L G^R_{0,N-1} Gamma_R (G^R_{0,N-1})^H].
  * density (nullable): (n_stack, sum of block sizes) doubles, receives
  * Im diag(G^<) as each diagonal block is finished (the quantity of
  * quatrex/core/observables.py:57-86 without overlap), without a second pass

@@ -72,9 +72,10 @@ class RGF:
                        retarded_offdiag=False, corner=None):
         """rgf.py:115-414.
 
-        Extension: ``corner`` (array (*stack, b_{N-1}, b_0)) receives the
-        retarded corner block G^R_{N-1,0} from the forward recursion
-        g_{j,0} = -x^R_jj A_{j,j-1} g_{j-1,0}, g_{0,0} = x^R_00.
+        Extension: ``corner`` (array (*stack, b_0, b_{N-1})) receives the
+        retarded corner block G^R_{0,N-1} from the forward recursion
+        g_{0,j} = -g_{0,j-1} A_{j-1,j} x^R_jj, g_{0,0} = x^R_00 (the
+        left-connected first-row blocks; the last one is exact).
         """
         n = a.num_blocks
         if obc_blocks is None:
@@ -130,7 +131,7 @@ class RGF:
                 xrjjh = _h(xrjj)
                 xrd[j] = xrjj
                 if corner is not None:
-                    g_c = -xrjj @ (aji @ g_c)
+                    g_c = -(g_c @ a_.blocks[i, j]) @ xrjj
                 t = aji_xrii @ sl_.blocks[i, j]
                 xld[j] = xrjj @ (sljj + aji @ xld[i] @ ajih + _h(t) - t) @ xrjjh
                 t = aji_xrii @ sg_.blocks[i, j]
@@ -200,6 +201,6 @@ class RGF:
         return None
 
 
-def transmission(g_n1, gamma_l, gamma_r):
-    """T(E) = Tr[Gamma_R G^R_{N-1,0} Gamma_L (G^R_{N-1,0})^H] (Caroli/Landauer)."""
-    return np.trace(gamma_r @ g_n1 @ gamma_l @ _h(g_n1), axis1=-2, axis2=-1).real
+def transmission(g_1n, gamma_l, gamma_r):
+    """T(E) = Tr[Gamma_L G^R_{0,N-1} Gamma_R (G^R_{0,N-1})^H] (Caroli/Landauer)."""
+    return np.trace(gamma_l @ g_1n @ gamma_r @ _h(g_1n), axis1=-2, axis2=-1).real

@@ -246,7 +246,8 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   for (int i = 0; i < N; ++i) orb_off[i + 1] = orb_off[i] + A.bs(i);
   const int64_t n_orb = orb_off[N];
   double* dens = density ? density + e0 * n_orb : nullptr;
-  // corner block G^R_{N-1,0}: ping-pong slots + product temp
+  // corner block G^R_{0,N-1}: ping-pong slots + product temp, b0 x b_j
+  // matrices with leading dimension bmax
   const int b0 = A.bs(0), bl = A.bs(N - 1);
   int gc_cur = 3 * N + 17, gc_nxt = 3 * N + 18;
   const int gc_tmp = 3 * N + 19;
@@ -282,7 +283,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const Mat X0 = W.get(sr[0], b0);
     fold(st, S, b0, b0, A.diag(0), -1.0, OR.get(0, b0), X0);
     block_inv(st, S, b0, X0, info, prec, scr, W.get(NS1, b0), W.get(NS2, b0));
-    if (corner) ew_axpby(st, S, b0, b0, kOne, X0, kZero, Mat(), W.get(gc_cur, b0));
+    if (corner) ew_axpby(st, S, b0, b0, kOne, X0, kZero, Mat(), W.get(gc_cur, bmax));
     for (auto& q : qs) {
       const Mat T3 = W.get(T3s, b0), T6 = W.get(T6s, b0);
       fold(st, S, b0, b0, q.sig->diag(0), 1.0, q.obc->get(0, b0), T3);
@@ -307,11 +308,11 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Ajj, Xrj, gp);
     block_inv(st, S, bj, Xrj, info, prec, scr, W.get(NS1, bj), W.get(NS2, bj));
     if (corner) {
-      // g_{j,0} = -x^R_jj (a_ji g_{i,0})
-      zgemm(st, S, bj, b0, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get(gc_cur, b0), kZero,
-            W.get(gc_tmp, b0), gp);
-      zgemm(st, S, bj, b0, bj, QX_OP_N, QX_OP_N, kMinus, Xrj, W.get(gc_tmp, b0), kZero,
-            W.get(gc_nxt, b0), gp);
+      // g_{0,j} = -(g_{0,i} a_ij) x^R_jj
+      zgemm(st, S, b0, bj, bi, QX_OP_N, QX_OP_N, kOne, W.get(gc_cur, bmax), A.upper(i), kZero,
+            W.get(gc_tmp, bmax), gp);
+      zgemm(st, S, b0, bj, bj, QX_OP_N, QX_OP_N, kMinus, W.get(gc_tmp, bmax), Xrj, kZero,
+            W.get(gc_nxt, bmax), gp);
       std::swap(gc_cur, gc_nxt);
     }
     for (auto& q : qs) {
@@ -335,8 +336,8 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   }
 
   if (corner)
-    ew_axpby(st, S, bl, b0, kOne, W.get(gc_cur, b0), kZero, Mat(),
-             Mat(static_cast<z2*>(corner) + e0 * (int64_t)bl * b0, (int64_t)bl * b0, b0));
+    ew_axpby(st, S, b0, bl, kOne, W.get(gc_cur, bmax), kZero, Mat(),
+             Mat(static_cast<z2*>(corner) + e0 * (int64_t)b0 * bl, (int64_t)b0 * bl, bl));
 
   // ---- last diagonal blocks (rgf.py:297-305) ----
   {

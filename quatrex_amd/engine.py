@@ -172,7 +172,7 @@ class Lane:
         xg = p[o:o + c * d.nnz_d].view(c, d.nnz_d)
         o += c * d.nnz_d
         nl, n0 = d.block_sizes[-1], d.block_sizes[0]
-        corner = p[o:o + c * nl * n0].view(c, nl, n0)
+        corner = p[o:o + c * nl * n0].view(c, n0, nl)
         v = (d.container(a, False), d.container(xl, True), d.container(xg, True), corner)
         self._views[key] = v
         return v

@@ -261,9 +261,9 @@ def test_corner_and_transmission(device, workload):
 
     w, (oa, osl, osg), _ = workload
     b0, bl, ne = w.block_sizes[0], w.block_sizes[-1], oa.data.shape[0]
-    ref = np.zeros((ne, bl, b0), complex)
+    ref = np.zeros((ne, b0, bl), complex)
     orgf.RGF(16).selected_solve(oa, osl, osg, corner=ref)
-    corner = torch.zeros((ne, bl, b0), dtype=torch.complex128, device=device)
+    corner = torch.zeros((ne, b0, bl), dtype=torch.complex128, device=device)
     for mbs in (16, 3):
         RGF(mbs).selected_solve(to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device),
                                 corner=corner)

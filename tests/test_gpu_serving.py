@@ -41,7 +41,7 @@ def _direct(w, req, device):
     a, sl, sg = w2.assemble(device)
     ne, n_orb = len(req.energies), int(np.sum(w.block_sizes))
     dens = torch.zeros((ne, n_orb), dtype=torch.float64, device=device)
-    corner = torch.zeros((ne, w.block_sizes[-1], w.block_sizes[0]), dtype=torch.complex128,
+    corner = torch.zeros((ne, w.block_sizes[0], w.block_sizes[-1]), dtype=torch.complex128,
                          device=device)
     RGF(ne).selected_solve(a, sl, sg, density=dens, corner=corner)
     gl = 1j * (w.sigma_b_left - w.sigma_b_left.conj().T)

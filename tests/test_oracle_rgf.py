@@ -295,20 +295,20 @@ def test_obc_folding_equivalence(config1):
 
 
 def test_corner_block_and_transmission(config1):
-    """G^R_{N-1,0} from the forward recursion equals the dense inverse's corner;
+    """G^R_{0,N-1} from the forward recursion equals the dense inverse's corner;
     the Caroli transmission from it equals the dense formula."""
     from oracle.rgf import transmission
 
     w, A, SL, SG, GR, GL, GG = config1
     a, sl, sg = _containers(w, A, SL, SG)
     b0, bl = w.block_sizes[0], w.block_sizes[-1]
-    corner = np.zeros((A.shape[0], bl, b0), complex)
+    corner = np.zeros((A.shape[0], b0, bl), complex)
     RGF(max_batch_size=5).selected_solve(a, sl, sg, corner=corner)
     n = A.shape[-1]
-    assert rel_err(corner, GR[:, n - bl:, :b0]) < 1e-10
+    assert rel_err(corner, GR[:, :b0, n - bl:]) < 1e-10
     gl = 1j * (w.sigma_b_left - _h(w.sigma_b_left))
     gr = 1j * (w.sigma_b_right - _h(w.sigma_b_right))
     t = transmission(corner, gl, gr)
-    ref = np.trace(gr @ GR[:, n - bl:, :b0] @ gl @ _h(GR[:, n - bl:, :b0]), axis1=-2, axis2=-1).real
+    ref = np.trace(gl @ GR[:, :b0, n - bl:] @ gr @ _h(GR[:, :b0, n - bl:]), axis1=-2, axis2=-1).real
     assert np.allclose(t, ref, rtol=1e-10)
     assert np.all(t > 0)
