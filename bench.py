@@ -126,6 +126,9 @@ def cpu_baseline(w, budget_s: float, max_energies: int):
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "zgemm_256_hbm_traffic.json"
+# the dominant kernel: complex128 A B with both operands untransposed (70 % of
+# the RGF's GEMM launches), register-direct 64 x 64 workgroup tiles
+GEMM_KERNEL_KEY = "zgemm_reg_kernel<double,N,N,4M,64x64>"
 
 
 def measured_traffic(flops_per_launch: float):
@@ -137,7 +140,7 @@ def measured_traffic(flops_per_launch: float):
     except (OSError, ValueError):
         return None
     for name, k in rec.get("kernels", {}).items():
-        if name.startswith("zgemm_kernel<double,N,N,4M,64x64>") and \
+        if name.startswith(GEMM_KERNEL_KEY) and \
                 abs(k.get("flops_per_launch", -1.0) - flops_per_launch) < 1.0:
             return k["traffic_bytes"]
     return None
@@ -190,7 +193,7 @@ def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak):
         rec["roofline"] = {
             "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
-            "kernel": "zgemm_kernel (batched complex GEMM, 64x64 tiles)",
+            "kernel": "zgemm_reg_kernel (batched complex GEMM, register-direct, 64x64 tiles)",
             "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, calibrated "
                                "on a 1 GiB copy: profiles/zgemm_256_hbm_traffic.json")
             if traffic else None,
@@ -567,7 +570,7 @@ def kernel_probe(args):
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": copy_bytes, "copy_launches": 3,
                       "gemm": {"m": b, "n": b, "k": b, "batch": batch, "launches": 5,
-                               "compulsory_bytes": 16 * 4 * b * b * batch,
+                               "compulsory_bytes": 16 * 3 * b * b * batch,
                                "flops": 8.0 * b ** 3 * batch},
                       "inverse": {"n": b, "batch": batch, "launches": 3}}), flush=True)
 

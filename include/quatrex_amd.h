@@ -198,11 +198,22 @@ int qx_zgemm_ozaki(qx_handle* handle, void* stream, int64_t batch, int m, int n,
                    int ldb, int64_t stride_b, const double* beta, void* c,
                    int ldc, int64_t stride_c, int slices);
 
+/* GEMM kernel choices for qx_set_tuning's gemm_tile.  The register-direct
+ * kernels (complex128 arithmetic only; no LDS, no workgroup barriers) are
+ * named by their workgroup tile; 64 and 128 select the LDS-staged kernel with
+ * 64 x 64 or 128 x 128 tiles (all precisions). */
+#define QX_GEMM_AUTO 0
+#define QX_GEMM_REG_64x64 1
+#define QX_GEMM_REG_64x128 2
+#define QX_GEMM_REG_128x64 3
+#define QX_GEMM_LDS_64 64
+#define QX_GEMM_LDS_128 128
+
 /* Execution tuning of a handle (no counterpart in the reference): streams =
  * concurrent sub-batches of one RGF call (1..8, default 4, env
- * QX_RGF_STREAMS); gemm_tile = GEMM tile shape, 0 automatic, 64 or 128
- * (env QX_ZGEMM_TILE overrides).  Used by the autotuner and by callers that
- * run several handles side by side. */
+ * QX_RGF_STREAMS); gemm_tile = one of QX_GEMM_* (env QX_ZGEMM_TILE,
+ * QX_ZGEMM_KERNEL and QX_ZGEMM_REG override it).  Used by the autotuner and
+ * by callers that run several handles side by side. */
 int qx_set_tuning(qx_handle* handle, int streams, int gemm_tile);
 
 /* System matrix on the DSDBCOO nnz pattern.  Replaces

@@ -38,6 +38,9 @@ PRECISIONS = {"c128": PREC_C128, "complex128": PREC_C128, "c64": PREC_C64, "comp
 RGF_PRECISIONS = dict(PRECISIONS, mixed=PREC_MIXED)
 
 OP_N = 0
+# GEMM kernel choices for set_tuning (QX_GEMM_* of the C header)
+GEMM_AUTO, GEMM_REG_64x64, GEMM_REG_64x128, GEMM_REG_128x64 = 0, 1, 2, 3
+GEMM_LDS_64, GEMM_LDS_128 = 64, 128
 OP_C = 2
 
 SOLVE_RETARDED = 1
@@ -159,7 +162,7 @@ def new_handle(device: torch.device) -> ctypes.c_void_p:
 
 
 def set_tuning(h, streams: int, gemm_tile: int = 0) -> None:
-    """Sub-batch streams of one RGF call and GEMM tile shape (0 automatic)."""
+    """Sub-batch streams of one RGF call and GEMM kernel (GEMM_*; 0 automatic)."""
     check(load().qx_set_tuning(h, int(streams), int(gemm_tile)), h, "set_tuning")
 
 

@@ -536,8 +536,10 @@ int qx_set_tuning(qx_handle* h, int streams, int gemm_tile) {
   return guarded(h, [&]() {
     QX_REQUIRE(h, "null handle");
     QX_REQUIRE(streams >= 1 && streams <= 8, "set_tuning: streams must be in [1, 8]");
-    QX_REQUIRE(gemm_tile == 0 || gemm_tile == 64 || gemm_tile == 128,
-               "set_tuning: gemm_tile must be 0 (automatic), 64 or 128");
+    QX_REQUIRE(gemm_tile == QX_GEMM_AUTO || gemm_tile == QX_GEMM_REG_64x64 ||
+                   gemm_tile == QX_GEMM_REG_64x128 || gemm_tile == QX_GEMM_REG_128x64 ||
+                   gemm_tile == QX_GEMM_LDS_64 || gemm_tile == QX_GEMM_LDS_128,
+               "set_tuning: gemm_tile must be one of QX_GEMM_* (0, 1, 2, 3, 64, 128)");
     h->parts = streams;
     h->gemm_tile = gemm_tile;
     return QX_OK;

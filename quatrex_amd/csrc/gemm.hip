@@ -37,6 +37,7 @@
 // element e of lane l is D[l / 16 + 4 e][l % 16] for the f64 instruction and
 // D[4 (l / 16) + e][l % 16] for the f32 instruction.
 #include <cstdlib>
+#include <string>
 
 #include "qx_internal.h"
 
@@ -287,6 +288,210 @@ void zgemm_kernel(GemmArgs g) {
       }
 }
 
+// Register-direct complex128 kernel: no LDS and no barriers.
+//
+// Each wave loads its MFMA fragments straight from global memory (through
+// L1/L2) into a ring of D register slots and multiplies k-step kk while the
+// loads for k-steps kk+1 .. kk+D-1 are in flight (one k-step = 4 values of
+// k).  The four waves of a workgroup (2 x 2) never wait for each other, so a
+// SIMD's matrix core is not idled by workgroup barriers.  Wave tile is
+// (16 TM) x (16 TN); workgroup tile (32 TM) x (32 TN).
+//
+// Fragment layout of v_mfma_f64_16x16x4_f64: lane l supplies
+// op(A)[row l % 16][k l / 16] and op(B)[k l / 16][col l % 16].
+// op = C (conjugate transpose) reads the stored element transposed and
+// negates its imaginary part through the instruction's neg modifier (bits
+// 0, 1, 2 of the last builtin operand negate the A, B, C sources).
+//
+// CHECK = false requires m % BM == 0, n % BN == 0 and k % (4 D) == 0.
+// CHECK = true clamps every address into the matrix (rows of A past m and
+// columns of B past n only feed outputs that are not stored); the pipelined
+// loop then covers whole groups of D full k-steps and a short tail loop the
+// rest, with both operands zeroed past k.  No value is selected on the
+// pipelined path, so its loads are never waited for early.
+//
+// The arithmetic per output element (order of the k sum, 4M / 3M split) is
+// the same as zgemm_kernel's, so both kernels give bit-identical results.
+template <int OPA, int OPB, int SPLIT, int TM, int TN, int D, bool CHECK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void zgemm_reg_kernel(GemmArgs g) {
+  constexpr int BM = 32 * TM, BN = 32 * TN;
+  // neg-modifier masks: s_a = -1 when op(A) = C (imag part negated), s_b ditto
+  constexpr bool CA = OPA == QX_OP_C, CB = OPB == QX_OP_C;
+  const int id = xcd_remap(blockIdx.x, gridDim.x);
+  const int tiles = g.tilesM * g.tilesN;
+  const int64_t e = id / tiles;
+  const int t = id - (int)(e * tiles);
+  const int m0 = (t / g.tilesN) * BM, n0 = (t % g.tilesN) * BN;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const z2* __restrict__ A = g.A + e * g.sA;
+  const z2* __restrict__ B = g.B + e * g.sB;
+  const int r16 = lane & 15, k4 = lane >> 4;
+
+  // pa[i] / pb[j]: this lane's element at k = 0 (k index k4 added below)
+  const z2* pa[TM];
+  const z2* pb[TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    int row = m0 + wm * 16 * TM + i * 16 + r16;
+    if (CHECK) row = min(row, g.m - 1);
+    pa[i] = CA ? A + row : A + (int64_t)row * g.lda;
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    int col = n0 + wn * 16 * TN + j * 16 + r16;
+    if (CHECK) col = min(col, g.n - 1);
+    pb[j] = CB ? B + (int64_t)col * g.ldb : B + col;
+  }
+  const int64_t ka = CA ? g.lda : 1, kb_ = CB ? 1 : g.ldb;  // element step per k
+  const int nk = (g.k + 3) / 4;  // k-steps, the last one possibly partial
+  // k-steps of the pipelined loop: whole groups of D full k-steps.  The rest
+  // (CHECK only) runs in the tail loop below.
+  const int nkm = CHECK ? (g.k / (4 * D)) * D : nk;
+
+  // Loads of the pipelined loop.  Past its end (prefetch of k-steps it will
+  // not use) the address is clamped to stay inside the matrix.
+  auto fetch = [&](int kl, z2* fa, z2* fb) {
+    const int kidx = min(4 * kl + k4, 4 * nkm - 4 + k4);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) fa[i] = pa[i][kidx * ka];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) fb[j] = pb[j][kidx * kb_];
+  };
+
+  d4 acc1[TM][TN], acc2[TM][TN], acc3[SPLIT == 3 ? TM : 1][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc1[i][j] = d4{0, 0, 0, 0};
+      acc2[i][j] = d4{0, 0, 0, 0};
+      if constexpr (SPLIT == 3) acc3[i][j] = d4{0, 0, 0, 0};
+    }
+
+  // One k-step of MFMAs on fragments fa[TM], fb[TN].
+  auto step = [&](const z2* fa, const z2* fb) {
+    if constexpr (SPLIT == 3) {
+      double as[TM], bs[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) as[i] = CA ? fa[i].x - fa[i].y : fa[i].x + fa[i].y;
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bs[j] = CB ? fb[j].x - fb[j].y : fb[j].x + fb[j].y;
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].x, fb[j].x, acc1[i][j], 0, 0, 0);
+          // P2 += (s_a im_a)(s_b im_b)
+          acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].y, fb[j].y, acc2[i][j], 0, 0,
+                                                            (CA != CB) ? 1 : 0);
+          acc3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[i], bs[j], acc3[i][j], 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].x, fb[j].x, acc1[i][j], 0, 0, 0);
+          // Ci += ar (s_b im_b)
+          acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].x, fb[j].y, acc2[i][j], 0, 0,
+                                                            CB ? 2 : 0);
+        }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          // Cr += -(s_a im_a)(s_b im_b)
+          acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].y, fb[j].y, acc1[i][j], 0, 0,
+                                                            (CA == CB) ? 1 : 0);
+          // Ci += (s_a im_a) br
+          acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].y, fb[j].x, acc2[i][j], 0, 0,
+                                                            CA ? 1 : 0);
+        }
+    }
+  };
+
+  if (nkm > 0) {
+    z2 fa[D][TM], fb[D][TN];
+#pragma unroll
+    for (int s = 0; s < D - 1; ++s) fetch(s, fa[s], fb[s]);
+    // Opaque copies of the prologue values: without them the compiler merges
+    // a slot's prologue load and its in-loop load into a single load at the
+    // point of use, which removes the prefetch distance.
+#pragma unroll
+    for (int s = 0; s < D - 1; ++s) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        asm volatile("" : "+v"(fa[s][i].x));
+        asm volatile("" : "+v"(fa[s][i].y));
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        asm volatile("" : "+v"(fb[s][j].x));
+        asm volatile("" : "+v"(fb[s][j].y));
+      }
+    }
+    for (int kb = 0; kb < nkm; kb += D) {
+#pragma unroll
+      for (int s = 0; s < D; ++s) {
+        fetch(kb + s + D - 1, fa[(s + D - 1) % D], fb[(s + D - 1) % D]);
+        step(fa[s], fb[s]);
+      }
+    }
+  }
+  if constexpr (CHECK) {
+    // Tail: the remaining k-steps, operands zeroed past k.
+    for (int kl = nkm; kl < nk; ++kl) {
+      const int kraw = 4 * kl + k4;
+      const bool ok = kraw < g.k;
+      const int kidx = min(kraw, g.k - 1);
+      z2 fa[TM], fb[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const z2 v = pa[i][kidx * ka];
+        fa[i] = ok ? v : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const z2 v = pb[j][kidx * kb_];
+        fb[j] = ok ? v : make_double2(0.0, 0.0);
+      }
+      step(fa, fb);
+    }
+  }
+
+  z2* C = g.C + e * g.sC;
+  const z2* Ci = g.Cin + e * g.sCin;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + wm * 16 * TM + i * 16 + k4 + 4 * q;
+        const int col = n0 + wn * 16 * TN + j * 16 + r16;
+        if (!CHECK || (row < g.m && col < g.n)) {
+          double vr, vi;
+          if constexpr (SPLIT == 3) {
+            const double p1 = acc1[i][j][q], p2 = acc2[i][j][q];
+            vr = p1 - p2;
+            vi = acc3[i][j][q] - p1 - p2;
+          } else {
+            vr = acc1[i][j][q];
+            vi = acc2[i][j][q];
+          }
+          double outr = g.ar * vr - g.ai * vi, outi = g.ar * vi + g.ai * vr;
+          if (!g.beta_zero) {
+            const z2 c = Ci[(int64_t)row * g.ldcin + col];
+            outr += g.br * c.x - g.bi * c.y;
+            outi += g.br * c.y + g.bi * c.x;
+          }
+          C[(int64_t)row * g.ldc + col] = make_double2(outr, outi);
+        }
+      }
+}
+
 template <typename T, int SPLIT, int BM, int BN, int WM, int WN>
 void launch_shape(hipStream_t s, int64_t batch, int opA, int opB, GemmArgs g) {
   g.tilesM = (g.m + BM - 1) / BM;
@@ -326,8 +531,105 @@ int gemm_tile(int m, int n, int64_t batch) {
   return (m >= 128 && n >= 128 && big >= 256) ? 128 : 64;
 }
 
+template <int SPLIT, int TM, int TN, int D>
+void launch_reg(hipStream_t s, int64_t batch, int opA, int opB, GemmArgs g) {
+  constexpr int BM = 32 * TM, BN = 32 * TN;
+  g.tilesM = (g.m + BM - 1) / BM;
+  g.tilesN = (g.n + BN - 1) / BN;
+  const int64_t blocks = batch * g.tilesM * g.tilesN;
+  QX_REQUIRE(blocks < (int64_t(1) << 31), "zgemm: grid too large");
+  const bool full = g.m % BM == 0 && g.n % BN == 0 && g.k % (4 * D) == 0;
+  dim3 grid((unsigned)blocks), block(256);
+  KernelTimer* timer = active_timer;
+  hipEvent_t ev0 = nullptr;
+  if (timer) timer->begin(s, &ev0);
+#define QX_REG_LAUNCH(OA, OB)                                                                   \
+  if (full)                                                                                     \
+    hipLaunchKernelGGL((zgemm_reg_kernel<OA, OB, SPLIT, TM, TN, D, false>), grid, block, 0, s, g); \
+  else                                                                                          \
+    hipLaunchKernelGGL((zgemm_reg_kernel<OA, OB, SPLIT, TM, TN, D, true>), grid, block, 0, s, g);
+  if (opA == QX_OP_N && opB == QX_OP_N) {
+    QX_REG_LAUNCH(QX_OP_N, QX_OP_N)
+  } else if (opA == QX_OP_N && opB == QX_OP_C) {
+    QX_REG_LAUNCH(QX_OP_N, QX_OP_C)
+  } else if (opA == QX_OP_C && opB == QX_OP_N) {
+    QX_REG_LAUNCH(QX_OP_C, QX_OP_N)
+  } else {
+    QX_REG_LAUNCH(QX_OP_C, QX_OP_C)
+  }
+#undef QX_REG_LAUNCH
+  QX_HIP_CHECK(hipGetLastError());
+  if (timer) timer->end(s, ev0, KernelTimer::GEMM, 8.0 * g.m * g.n * (double)g.k * batch);
+}
+
+// Shapes of the register-direct kernel, as (TM, TN, D): workgroup tile
+// (32 TM) x (32 TN), D register slots.  3M keeps three accumulator sets and
+// only fits 2 x 2 without spilling.
+enum RegShape { REG_2x2x4 = 0, REG_2x4x2, REG_4x2x2, REG_NSHAPES };
+
+int env_int(const char* name, int dflt) {
+  const char* e = std::getenv(name);
+  return e ? std::atoi(e) : dflt;
+}
+
+// Register-direct shape for a complex128 launch, or -1 for the LDS-staged
+// kernel.  Precedence: QX_ZGEMM_TILE (64 | 128: LDS shape), QX_ZGEMM_KERNEL=
+// lds, QX_ZGEMM_REG (shape index), then the calling handle's gemm_tile
+// (qx_set_tuning: 64 | 128 LDS, QX_GEMM_REG_* register-direct), then the
+// automatic choice.
+int reg_shape(int split, int opA, int opB, int m, int n) {
+  static const int env_tile = env_int("QX_ZGEMM_TILE", 0);
+  static const bool env_lds = [] {
+    const char* e = std::getenv("QX_ZGEMM_KERNEL");
+    return e && std::string(e) == "lds";
+  }();
+  static const int env_reg = env_int("QX_ZGEMM_REG", -1);
+  if (env_tile == 64 || env_tile == 128 || env_lds) return -1;
+  int shape;
+  if (env_reg >= 0 && env_reg < REG_NSHAPES) {
+    shape = env_reg;
+  } else {
+    switch (active_gemm_tile) {
+      case QX_GEMM_LDS_64: case QX_GEMM_LDS_128: return -1;
+      case QX_GEMM_REG_64x64: shape = REG_2x2x4; break;
+      case QX_GEMM_REG_64x128: shape = REG_2x4x2; break;
+      case QX_GEMM_REG_128x64: shape = REG_4x2x2; break;
+      default: {
+        // Automatic (measured at 32-256 matrices of 64-512): A op(B) with
+        // op(B) = B^H reads both operands with a 64-byte stride pattern;
+        // the 128 x 64 tile halves the fragment loads per MFMA there.  The
+        // 3M split cannot use the larger tiles (three accumulator sets) and
+        // is faster through LDS for that case.
+        const bool nc = opA == QX_OP_N && opB == QX_OP_C;
+        if (nc && split == 3) return -1;
+        shape = (nc && m >= 256 && n >= 256) ? REG_4x2x2 : REG_2x2x4;
+      }
+    }
+  }
+  return split == 3 ? REG_2x2x4 : shape;
+}
+
+template <int SPLIT>
+void launch_reg_shape(hipStream_t s, int64_t batch, int opA, int opB, const GemmArgs& g, int shape) {
+  if constexpr (SPLIT == 4) {
+    switch (shape) {
+      case REG_2x4x2: launch_reg<SPLIT, 2, 4, 2>(s, batch, opA, opB, g); return;
+      case REG_4x2x2: launch_reg<SPLIT, 4, 2, 2>(s, batch, opA, opB, g); return;
+      default: break;
+    }
+  }
+  launch_reg<SPLIT, 2, 2, 4>(s, batch, opA, opB, g);
+}
+
 template <typename T, int SPLIT>
 void launch_typed(hipStream_t s, int64_t batch, int opA, int opB, const GemmArgs& g) {
+  if constexpr (sizeof(T) == 8) {
+    const int shape = reg_shape(SPLIT, opA, opB, g.m, g.n);
+    if (shape >= 0) {
+      launch_reg_shape<SPLIT>(s, batch, opA, opB, g, shape);
+      return;
+    }
+  }
   // complex128 3M at 128 x 128 needs more than 256 registers per lane (it
   // would spill): that combination stays on the 64 x 64 shape.
   constexpr bool big_ok = !(sizeof(T) == 8 && SPLIT == 3);

@@ -103,6 +103,41 @@ def test_zgemm_large_stack(device, mnkb, ops, prec):
         assert rel_err(got[e], alpha * opA @ opB + beta * c[e]) < tol
 
 
+REG_CASES = [(256, 256, 256, 8), (128, 128, 64, 4), (200, 136, 72, 3), (5, 7, 3, 2),
+             (416, 416, 416, 2)]
+
+
+@pytest.mark.parametrize("mnkb", REG_CASES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("ops", [(0, 0), (0, 2), (2, 0), (2, 2)], ids=["NN", "NC", "CN", "CC"])
+@pytest.mark.parametrize("prec", [0, 3], ids=["c128", "c128_3m"])
+def test_zgemm_register_kernels_match_lds(device, mnkb, ops, prec):
+    """Every register-direct GEMM shape gives the bits of the LDS-staged
+    kernel: both sum k in the same order with the same 4M / 3M split (full
+    tiles, ragged tiles and k not a multiple of 4)."""
+    L = _lib()
+    m, n, k, batch = mnkb
+    rng = np.random.default_rng(m + 2 * n + 3 * k)
+    opa, opb = ops
+    a_shape = (batch, m, k) if opa == 0 else (batch, k, m)
+    b_shape = (batch, k, n) if opb == 0 else (batch, n, k)
+    A = torch.as_tensor(_cplx(rng, a_shape), device=device)
+    B = torch.as_tensor(_cplx(rng, b_shape), device=device)
+    C0 = torch.as_tensor(_cplx(rng, (batch, m, n)), device=device)
+    alpha, beta = 0.9 + 0.4j, 0.3 - 0.7j
+    h = L.handle(device)
+    outs = {}
+    try:
+        for code in (L.GEMM_LDS_64, L.GEMM_REG_64x64, L.GEMM_REG_64x128, L.GEMM_REG_128x64):
+            L.set_tuning(h, 4, code)
+            C = C0.clone()
+            _zgemm(device, A, B, C, opa, opb, alpha, beta, prec)
+            outs[code] = C
+    finally:
+        L.set_tuning(h, 4, L.GEMM_AUTO)
+    for code in (L.GEMM_REG_64x64, L.GEMM_REG_64x128, L.GEMM_REG_128x64):
+        assert torch.equal(outs[code], outs[L.GEMM_LDS_64]), code
+
+
 def _inv(dev, A, X, prec, S=None):
     L = _lib()
     lib = L.load()
