@@ -170,15 +170,23 @@ def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak):
         timer.__enter__()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    marks = [t0]
     for i in range(steps):
         step(lat)
         log(f"{rgf.precision}: timed step {i} done")
+        marks.append(time.perf_counter())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     if timer:
         timer.__exit__(None, None, None)
     lat_ms = np.array(lat) * 1e3
+    # a selected_solve call returns after checking the device error flags of
+    # its work (a stream synchronization), so the marks bound each step
+    step_ms = np.diff(np.array(marks)) * 1e3
     rec = {"elapsed": elapsed,
+           "step_ms": {"median": round(float(np.median(step_ms)), 2),
+                       "min": round(float(step_ms.min()), 2),
+                       "max": round(float(step_ms.max()), 2)},
            "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
                           "p99": round(float(np.percentile(lat_ms, 99)), 2)}}
     if timer:
@@ -578,7 +586,7 @@ def kernel_probe(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="c128", choices=list(PEAK_TFLOPS))
     ap.add_argument("--compare", default="c128-3m,mixed",
@@ -693,6 +701,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * prim["elapsed"] / args.steps, 2),
+        "step_ms": prim["step_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
