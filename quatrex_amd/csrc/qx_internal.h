@@ -54,6 +54,15 @@ struct Error {
 
 // ---- kernels (launchers) --------------------------------------------------
 
+// Fused selected solve for stacks of small blocks (rgf_small.hip); returns
+// false, having done nothing, when the call is not eligible.
+bool rgf_small_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, const void* a,
+                     const qx_btd_layout* lsl, const void* sl, const qx_btd_layout* lsg,
+                     const void* sg, const void* const* obc_r, const void* const* obc_l,
+                     const void* const* obc_g, const qx_btd_layout* lxl, void* xl,
+                     const qx_btd_layout* lxg, void* xg, const qx_btd_layout* lxr, void* xr,
+                     void* current, void* corner, double* density, uint32_t flags, int prec);
+
 // C = alpha op(A) op(B) + beta C for `batch` matrices; m x n output, inner k.
 void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
            Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& C,

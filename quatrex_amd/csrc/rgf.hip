@@ -228,6 +228,10 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   QX_REQUIRE(N >= 1, "selected_solve: need at least one block");
   if (Stot == 0) return;
   const int bmax = block_limits(la, "selected_solve");
+  if (rgf_small_solve(h, st0, la, a, lsl, sl, lsg, sg, obc_r, obc_l, obc_g, lxl, xl, lxg, xg,
+                      want_r ? lxr : nullptr, want_r ? xr : nullptr, want_cur ? current : nullptr,
+                      corner, density, flags, prec))
+    return;
   z2* ws = static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, true)));
   int* info_all = h->ensure_info(Stot);
   const Btd A0(la, a), SL0(lsl, sl), SG0(lsg, sg), XL0(lxl, xl), XG0(lxg, xg),

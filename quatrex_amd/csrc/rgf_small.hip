@@ -1,0 +1,691 @@
+// Fused selected solve for small blocks (gfx950): one workgroup per energy.
+//
+// Same algorithm, outputs and operation order as rgf_selected_solve in
+// rgf.hip (reference src/qttools/greens_function_solver/rgf.py:164-414), for
+// stacks whose blocks are all at most kSB = 32 wide and at most kSN = 16 long.
+// At such sizes every block product is a few microseconds of matrix-core
+// work, so the general path (one launch per product over the whole stack)
+// is bound by launch latency; here the whole sweep of one energy runs in one
+// workgroup of four waves, with every intermediate block in a per-energy
+// workspace that stays in L2.
+//
+// - Block products (sgemm): the four waves each own a 16 x 16 quarter of the
+//   output and load their MFMA fragments straight from memory, all k-steps
+//   issued before the first v_mfma_f64_16x16x4_f64.  4M or 3M split, summed
+//   over k in the same order as the batched kernels.
+// - Block inverse (sinv): one wave, two lanes per row (16 columns each),
+//   Gauss-Jordan with partial pivoting: the pivot is a DPP wave maximum of
+//   32-bit keys (float magnitude with the row index in the low bits, as in
+//   inv.hip), rows p and k are exchanged through LDS, and the column
+//   interchanges are undone at the end.  No barrier inside the wave.
+// - Elementwise steps (folds, X - X^H parts, traces) as workgroup loops.
+// - A barrier closes every step, which makes its global writes visible to
+//   the other waves of the workgroup.
+#include <cstdlib>
+
+#include "qx_internal.h"
+
+namespace qx {
+namespace {
+
+constexpr int kSB = 32;   // largest block size
+constexpr int kSN = 16;   // largest block count
+constexpr int kThreads = 256;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct SLay {
+  int32_t diag[kSN], up[kSN], lo[kSN];  // element offsets, -1: not stored
+};
+
+struct SmallArgs {
+  int N, flags, NS, split;
+  int bs[kSN];
+  int orb[kSN];  // first orbital of each block (density row offsets)
+  int64_t n_orb;
+  int64_t sA, sSL, sSG, sXL, sXG, sXR;  // stack strides (elements)
+  const z2 *A, *SL, *SG;
+  z2 *XL, *XG, *XR;
+  SLay lA, lSL, lSG, lXL, lXG, lXR;
+  const z2* obc[3][kSN];  // retarded, lesser, greater boundary blocks or null
+  z2* ws;                 // per energy NS slots of kSB * kSB
+  int* info;
+  z2* current;   // (S, N - 1) or null
+  z2* corner;    // (S, b_0, b_{N-1}) or null
+  double* density;  // (S, n_orb) or null
+};
+
+struct M {
+  z2* p;
+  int ld;
+};
+
+__device__ inline M blk(const z2* base, int64_t stride, int64_t e, int32_t off, int ld) {
+  if (!base || off < 0) return M{nullptr, ld};
+  return M{const_cast<z2*>(base) + e * stride + off, ld};
+}
+
+// Pivot key and wave maximum as in inv.hip.
+__device__ inline unsigned s_pivot_key(double re, double im, int r) {
+  const float m = (float)(fabs(re) + fabs(im));
+  return (__float_as_uint(m) & 0xFFFFFE00u) | (unsigned)(511 - r);
+}
+
+__device__ inline unsigned s_wave_max(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xf, 0xf, false));
+  const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)v, 0);
+  const unsigned b = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
+  const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)v, 32);
+  const unsigned d = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+  return max(max(a, b), max(c, d));
+}
+
+// Value of lane QQ of each pair of lanes (DPP quad_perm [0,0,2,2] / [1,1,3,3]).
+template <int QQ>
+__device__ inline double s_pair_bcast(double x) {
+  constexpr int sel = QQ == 0 ? 0xA0 : 0xF5;
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), sel, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), sel, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline void s_crecip(double ar, double ai, double& qr, double& qi) {
+  // 1 / (ar + i ai), Smith's scaling (as inv.hip)
+  if (fabs(ar) >= fabs(ai)) {
+    const double r = ai / ar, d = ar + ai * r;
+    qr = 1.0 / d;
+    qi = -r / d;
+  } else {
+    const double r = ar / ai, d = ai + ar * r;
+    qr = r / d;
+    qi = -1.0 / d;
+  }
+}
+
+struct Shared {
+  z2 prow[kSB], krow[kSB];
+  int perm[kSB], idx[kSB];
+  z2 tile[kSB][kSB + 1];
+  double red[2][kThreads / 64];
+};
+
+// C = alpha op(A) op(B) + beta Cin, m, n, k <= 32; a null A or B is zero.
+__device__ void sgemm(int split, int m, int n, int k, int opa, int opb, Cplx al, M A, M B,
+                      Cplx be, M Cin, M C) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = (w >> 1) * 16, c0 = (w & 1) * 16;
+  const int r16 = lane & 15, k4 = lane >> 4;
+  if (r0 < m && c0 < n) {
+    d4 a1 = {0, 0, 0, 0}, a2 = {0, 0, 0, 0}, a3 = {0, 0, 0, 0};
+    if (A.p && B.p) {
+      z2 fa[kSB / 4], fb[kSB / 4];
+#pragma unroll
+      for (int s = 0; s < kSB / 4; ++s) {
+        const int kk = 4 * s + k4;
+        z2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
+        if (kk < k && r0 + r16 < m) {
+          if (opa == QX_OP_N) {
+            va = A.p[(r0 + r16) * A.ld + kk];
+          } else {
+            va = A.p[kk * A.ld + r0 + r16];
+            va.y = -va.y;
+          }
+        }
+        if (kk < k && c0 + r16 < n) {
+          if (opb == QX_OP_N) {
+            vb = B.p[kk * B.ld + c0 + r16];
+          } else {
+            vb = B.p[(c0 + r16) * B.ld + kk];
+            vb.y = -vb.y;
+          }
+        }
+        fa[s] = va;
+        fb[s] = vb;
+      }
+#pragma unroll
+      for (int s = 0; s < kSB / 4; ++s) {
+        if (4 * s < k) {
+          if (split == 3) {
+            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s].x, fb[s].x, a1, 0, 0, 0);
+            a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s].y, fb[s].y, a2, 0, 0, 0);
+            a3 = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s].x + fa[s].y, fb[s].x + fb[s].y, a3,
+                                                      0, 0, 0);
+          } else {
+            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s].x, fb[s].x, a1, 0, 0, 0);
+            a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s].x, fb[s].y, a2, 0, 0, 0);
+            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s].y, fb[s].y, a1, 0, 0, 1);
+            a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s].y, fb[s].x, a2, 0, 0, 0);
+          }
+        }
+      }
+    }
+    const bool use_b = Cin.p && (be.re != 0.0 || be.im != 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = r0 + k4 + 4 * q, col = c0 + r16;
+      if (row < m && col < n) {
+        double vr, vi;
+        if (split == 3) {
+          vr = a1[q] - a2[q];
+          vi = a3[q] - a1[q] - a2[q];
+        } else {
+          vr = a1[q];
+          vi = a2[q];
+        }
+        double outr = al.re * vr - al.im * vi, outi = al.re * vi + al.im * vr;
+        if (use_b) {
+          const z2 c = Cin.p[row * Cin.ld + col];
+          outr += be.re * c.x - be.im * c.y;
+          outi += be.re * c.y + be.im * c.x;
+        }
+        C.p[row * C.ld + col] = make_double2(outr, outi);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Y = a A + b B (null operands are zero).
+__device__ void saxpby(int m, int n, Cplx a, M A, Cplx b, M B, M Y) {
+  for (int t = threadIdx.x; t < m * n; t += kThreads) {
+    const int r = t / n, c = t - r * n;
+    double yr = 0.0, yi = 0.0;
+    if (A.p) {
+      const z2 v = A.p[r * A.ld + c];
+      yr += a.re * v.x - a.im * v.y;
+      yi += a.re * v.y + a.im * v.x;
+    }
+    if (B.p) {
+      const z2 v = B.p[r * B.ld + c];
+      yr += b.re * v.x - b.im * v.y;
+      yi += b.re * v.y + b.im * v.x;
+    }
+    Y.p[r * Y.ld + c] = make_double2(yr, yi);
+  }
+  __syncthreads();
+}
+
+// Y = cy Y + ca A + cb B^H + cc D (real coefficients; Y, m x n, aliases none
+// of A, B, D; B is n x m).
+__device__ void sherm(int m, int n, double cy, double ca, M A, double cb, M B, M Y,
+                      double cc = 0.0, M D = M{nullptr, 0}) {
+  for (int t = threadIdx.x; t < m * n; t += kThreads) {
+    const int r = t / n, c = t - r * n;
+    double yr = 0.0, yi = 0.0;
+    if (cy != 0.0) {
+      const z2 v = Y.p[r * Y.ld + c];
+      yr = cy * v.x;
+      yi = cy * v.y;
+    }
+    if (ca != 0.0 && A.p) {
+      const z2 v = A.p[r * A.ld + c];
+      yr += ca * v.x;
+      yi += ca * v.y;
+    }
+    if (cb != 0.0 && B.p) {
+      const z2 v = B.p[c * B.ld + r];
+      yr += cb * v.x;
+      yi -= cb * v.y;
+    }
+    if (cc != 0.0 && D.p) {
+      const z2 v = D.p[r * D.ld + c];
+      yr += cc * v.x;
+      yi += cc * v.y;
+    }
+    Y.p[r * Y.ld + c] = make_double2(yr, yi);
+  }
+  __syncthreads();
+}
+
+__device__ void sdiag_imag(int n, M X, double* out) {
+  for (int t = threadIdx.x; t < n; t += kThreads) out[t] = X.p[t * X.ld + t].y;
+  __syncthreads();
+}
+
+// out = Tr(X Y) - Tr(Z W), n x n.
+__device__ void strace_diff(int n, M X, M Y, M Z, M W, z2* out, Shared& sh) {
+  double sr = 0.0, si = 0.0;
+  for (int t = threadIdx.x; t < n * n; t += kThreads) {
+    const int a = t / n, b = t - a * n;
+    if (X.p && Y.p) {
+      const z2 x = X.p[a * X.ld + b], y = Y.p[b * Y.ld + a];
+      sr += x.x * y.x - x.y * y.y;
+      si += x.x * y.y + x.y * y.x;
+    }
+    if (Z.p && W.p) {
+      const z2 z = Z.p[a * Z.ld + b], w = W.p[b * W.ld + a];
+      sr -= z.x * w.x - z.y * w.y;
+      si -= z.x * w.y + z.y * w.x;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off, 64);
+    si += __shfl_xor(si, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh.red[0][threadIdx.x >> 6] = sr;
+    sh.red[1][threadIdx.x >> 6] = si;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tr = 0.0, ti = 0.0;
+    for (int u = 0; u < kThreads / 64; ++u) {
+      tr += sh.red[0][u];
+      ti += sh.red[1][u];
+    }
+    *out = make_double2(tr, ti);
+  }
+  __syncthreads();
+}
+
+// In-place inverse of the n x n block X (n <= 32) by wave 0.
+__device__ void sinv(int n, M X, int* info, Shared& sh) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x, r = lane >> 1, h = lane & 1;
+    double xr[16], xi[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int col = 16 * h + c;
+      z2 v = make_double2(0.0, 0.0);
+      if (r < n && col < n) v = X.p[r * X.ld + col];
+      xr[c] = v.x;
+      xi[c] = v.y;
+    }
+#pragma unroll
+    for (int k = 0; k < kSB; ++k) {
+      if (k < n) {
+        const int hk = k >> 4, ck = k & 15;
+        unsigned key = 0;
+        if (h == hk && r >= k && r < n) key = s_pivot_key(xr[ck], xi[ck], r);
+        const unsigned kmax = s_wave_max(key);
+        int p = 511 - (int)(kmax & 0x1FFu);
+        if (p < k || p >= n) p = k;
+        if (lane == 0) {
+          sh.perm[k] = p;
+          if ((kmax >> 9) == 0u) *info = 1;  // no nonzero pivot: singular
+        }
+        // publish the pivot row and row k (one wave: LDS keeps program order)
+        if (r == p) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c) sh.prow[16 * h + c] = make_double2(xr[c], xi[c]);
+        }
+        if (r == k) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c) sh.krow[16 * h + c] = make_double2(xr[c], xi[c]);
+        }
+        __builtin_amdgcn_wave_barrier();
+        const z2 x = sh.prow[k];
+        double vr, vi;
+        s_crecip(x.x, x.y, vr, vi);
+        double pr[16], pi[16];  // this lane's half of the scaled pivot row
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          const int col = 16 * h + c;
+          if (col == k) {
+            pr[c] = vr;
+            pi[c] = vi;
+          } else {
+            const z2 v = sh.prow[col];
+            pr[c] = v.x * vr - v.y * vi;
+            pi[c] = v.x * vi + v.y * vr;
+          }
+        }
+        if (r == k) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c) {
+            xr[c] = pr[c];
+            xi[c] = pi[c];
+          }
+        } else if (r == p) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c) {
+            const z2 v = sh.krow[16 * h + c];
+            xr[c] = v.x;
+            xi[c] = v.y;
+          }
+        }
+        // multiplier: the row's (swapped-in) value in column k
+        const double fr = hk == 0 ? s_pair_bcast<0>(xr[ck]) : s_pair_bcast<1>(xr[ck]);
+        const double fi = hk == 0 ? s_pair_bcast<0>(xi[ck]) : s_pair_bcast<1>(xi[ck]);
+        if (r != k && r < n) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c) {
+            const int col = 16 * h + c;
+            if (col == k) {
+              xr[c] = -(fr * pr[c] - fi * pi[c]);
+              xi[c] = -(fr * pi[c] + fi * pr[c]);
+            } else {
+              xr[c] -= fr * pr[c] - fi * pi[c];
+              xi[c] -= fr * pi[c] + fi * pr[c];
+            }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    // undo the column interchanges: reverse order of the pivots
+    if (lane == 0) {
+      for (int c = 0; c < n; ++c) sh.idx[c] = c;
+      for (int k = n - 1; k >= 0; --k) {
+        const int pq = sh.perm[k];
+        const int t = sh.idx[k];
+        sh.idx[k] = sh.idx[pq];
+        sh.idx[pq] = t;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int col = 16 * h + c;
+      if (r < n && col < n) sh.tile[r][col] = make_double2(xr[c], xi[c]);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int col = 16 * h + c;
+      if (r < n && col < n) X.p[r * X.ld + col] = sh.tile[r][sh.idx[col]];
+    }
+  }
+  __syncthreads();
+}
+
+const Cplx kOneS{1.0, 0.0}, kMinusS{-1.0, 0.0}, kZeroS{0.0, 0.0};
+
+__global__ __launch_bounds__(kThreads) void rgf_small_kernel(SmallArgs g) {
+  __shared__ Shared sh;
+  const int64_t e = blockIdx.x;
+  const int N = g.N, sp = g.split;
+  const bool want_r = (g.flags & QX_SOLVE_RETARDED) != 0;
+  const bool want_r_off = (g.flags & QX_SOLVE_RETARDED_OFFDIAG) != 0;
+  const bool want_cur = (g.flags & QX_SOLVE_CURRENT) != 0 && g.current;
+  const bool skip_lower[2] = {(g.flags & QX_SOLVE_SKIP_LOWER_L) != 0,
+                              (g.flags & QX_SOLVE_SKIP_LOWER_G) != 0};
+  int* info = g.info + e;
+  z2* W = g.ws + e * (int64_t)g.NS * kSB * kSB;
+  auto slot = [&](int s, int ld) { return M{W + (int64_t)s * kSB * kSB, ld}; };
+  auto bs = [&](int i) { return g.bs[i]; };
+  // slots: forward x^R, x^<, x^> (0..3N-1), full G^R, G^<, G^> from the
+  // backward sweep (3N..6N-1), then temporaries
+  auto F = [&](int q, int i) { return q * N + i; };                         // q: 0 r, 1 l, 2 g
+  auto full = [&](int q, int i) { return i == N - 1 ? q * N + i : (3 + q) * N + i; };
+  const int T1 = 6 * N, T3 = T1 + 1, T4 = T1 + 2, T5 = T1 + 3, T6 = T1 + 4, B1 = T1 + 5,
+            B2 = T1 + 6, B3 = T1 + 7, B4 = T1 + 8, C1 = T1 + 9, C2 = T1 + 10, C3 = T1 + 11,
+            C4 = T1 + 12, GC0 = T1 + 13, GC1 = T1 + 14, GCT = T1 + 15;
+  auto Ad = [&](int i) { return blk(g.A, g.sA, e, g.lA.diag[i], bs(i)); };
+  auto Au = [&](int i) { return blk(g.A, g.sA, e, g.lA.up[i], bs(i + 1)); };   // (i, i+1)
+  auto Al = [&](int i) { return blk(g.A, g.sA, e, g.lA.lo[i], bs(i)); };       // (i+1, i)
+  auto Sd = [&](int q, int i) {
+    return q == 1 ? blk(g.SL, g.sSL, e, g.lSL.diag[i], bs(i)) : blk(g.SG, g.sSG, e, g.lSG.diag[i], bs(i));
+  };
+  auto Su = [&](int q, int i) {
+    return q == 1 ? blk(g.SL, g.sSL, e, g.lSL.up[i], bs(i + 1))
+                  : blk(g.SG, g.sSG, e, g.lSG.up[i], bs(i + 1));
+  };
+  auto Od = [&](int q, int i) {
+    return q == 1 ? blk(g.XL, g.sXL, e, g.lXL.diag[i], bs(i)) : blk(g.XG, g.sXG, e, g.lXG.diag[i], bs(i));
+  };
+  auto Ou = [&](int q, int i) {
+    return q == 1 ? blk(g.XL, g.sXL, e, g.lXL.up[i], bs(i + 1)) : blk(g.XG, g.sXG, e, g.lXG.up[i], bs(i + 1));
+  };
+  auto Ol = [&](int q, int i) {
+    return q == 1 ? blk(g.XL, g.sXL, e, g.lXL.lo[i], bs(i)) : blk(g.XG, g.sXG, e, g.lXG.lo[i], bs(i));
+  };
+  auto obc = [&](int q, int i) {
+    const z2* p = g.obc[q][i];
+    return M{p ? const_cast<z2*>(p) + e * (int64_t)bs(i) * bs(i) : nullptr, bs(i)};
+  };
+  const int b0 = bs(0), bl = bs(N - 1);
+  int gc_cur = GC0, gc_nxt = GC1;
+
+  // ---- block 0 (rgf.py:207-231) ----
+  {
+    const M X0 = slot(F(0, 0), b0);
+    saxpby(b0, b0, kOneS, Ad(0), kMinusS, obc(0, 0), X0);
+    sinv(b0, X0, info, sh);
+    if (g.corner) saxpby(b0, b0, kOneS, X0, kZeroS, M{nullptr, 0}, slot(gc_cur, kSB));
+    for (int q = 1; q <= 2; ++q) {
+      const M t3 = slot(T3, b0), t6 = slot(T6, b0);
+      saxpby(b0, b0, kOneS, Sd(q, 0), kOneS, obc(q, 0), t3);
+      sgemm(sp, b0, b0, b0, QX_OP_N, QX_OP_N, kOneS, X0, t3, kZeroS, M{nullptr, 0}, t6);
+      sgemm(sp, b0, b0, b0, QX_OP_N, QX_OP_C, kOneS, t6, X0, kZeroS, M{nullptr, 0}, slot(F(q, 0), b0));
+    }
+  }
+
+  // ---- forward sweep (rgf.py:234-295) ----
+  for (int i = 0; i + 1 < N; ++i) {
+    const int j = i + 1, bi = bs(i), bj = bs(j);
+    const M Xri = slot(F(0, i), bi), Xrj = slot(F(0, j), bj), t1 = slot(T1, bi);
+    sgemm(sp, bj, bi, bi, QX_OP_N, QX_OP_N, kOneS, Al(i), Xri, kZeroS, M{nullptr, 0}, t1);
+    M Ajj = Ad(j);
+    if (obc(0, j).p) {
+      saxpby(bj, bj, kOneS, Ad(j), kMinusS, obc(0, j), Xrj);
+      Ajj = Xrj;
+    }
+    sgemm(sp, bj, bj, bi, QX_OP_N, QX_OP_N, kMinusS, t1, Au(i), kOneS, Ajj, Xrj);
+    sinv(bj, Xrj, info, sh);
+    if (g.corner) {
+      sgemm(sp, b0, bj, bi, QX_OP_N, QX_OP_N, kOneS, slot(gc_cur, kSB), Au(i), kZeroS,
+            M{nullptr, 0}, slot(GCT, kSB));
+      sgemm(sp, b0, bj, bj, QX_OP_N, QX_OP_N, kMinusS, slot(GCT, kSB), Xrj, kZeroS,
+            M{nullptr, 0}, slot(gc_nxt, kSB));
+      const int t = gc_cur;
+      gc_cur = gc_nxt;
+      gc_nxt = t;
+    }
+    for (int q = 1; q <= 2; ++q) {
+      const M t3 = slot(T3, bj), t4 = slot(T4, bi), t5 = slot(T5, bj), t6 = slot(T6, bj);
+      M Sjj = Sd(q, j);
+      if (obc(q, j).p) {
+        saxpby(bj, bj, kOneS, Sd(q, j), kOneS, obc(q, j), t3);
+        Sjj = t3;
+      }
+      sgemm(sp, bj, bi, bi, QX_OP_N, QX_OP_N, kOneS, Al(i), slot(F(q, i), bi), kZeroS,
+            M{nullptr, 0}, t4);
+      sgemm(sp, bj, bj, bi, QX_OP_N, QX_OP_C, kOneS, t4, Al(i), kOneS, Sjj, t3);
+      const M sij = Su(q, i);
+      if (sij.p) {
+        sgemm(sp, bj, bj, bi, QX_OP_N, QX_OP_N, kOneS, t1, sij, kZeroS, M{nullptr, 0}, t5);
+        sherm(bj, bj, 1.0, -1.0, t5, 1.0, t5, t3);  // t3 += t5^H - t5
+      }
+      sgemm(sp, bj, bj, bj, QX_OP_N, QX_OP_N, kOneS, Xrj, t3, kZeroS, M{nullptr, 0}, t6);
+      sgemm(sp, bj, bj, bj, QX_OP_N, QX_OP_C, kOneS, t6, Xrj, kZeroS, M{nullptr, 0},
+            slot(F(q, j), bj));
+    }
+  }
+
+  if (g.corner)
+    saxpby(b0, bl, kOneS, slot(gc_cur, kSB), kZeroS, M{nullptr, 0},
+           M{g.corner + e * (int64_t)b0 * bl, bl});
+
+  // ---- last diagonal blocks (rgf.py:297-305) ----
+  {
+    const int bn = bs(N - 1);
+    if (g.density) sdiag_imag(bn, slot(F(1, N - 1), bn), g.density + e * g.n_orb + g.orb[N - 1]);
+    for (int q = 1; q <= 2; ++q)
+      if (Od(q, N - 1).p) sherm(bn, bn, 0.0, 0.5, slot(F(q, N - 1), bn), -0.5, slot(F(q, N - 1), bn), Od(q, N - 1));
+    if (want_r) {
+      const M xr = blk(g.XR, g.sXR, e, g.lXR.diag[N - 1], bn);
+      if (xr.p) saxpby(bn, bn, kOneS, slot(F(0, N - 1), bn), kZeroS, M{nullptr, 0}, xr);
+    }
+  }
+
+  // ---- backward sweep (rgf.py:308-404) ----
+  for (int i = N - 2; i >= 0; --i) {
+    const int j = i + 1, bi = bs(i), bj = bs(j);
+    const M Xri = slot(F(0, i), bi), Xrj = slot(full(0, j), bj);
+    const M Aij = Au(i), Aji = Al(i);
+    const M b1 = slot(B1, bj), b2 = slot(B2, bi), b3 = slot(B3, bj), b4 = slot(B4, bi);
+    const M none{nullptr, 0};
+    sgemm(sp, bi, bj, bi, QX_OP_N, QX_OP_N, kOneS, Xri, Aij, kZeroS, none, b1);  // xr_ii a_ij
+    sgemm(sp, bj, bi, bj, QX_OP_N, QX_OP_N, kOneS, Xrj, Aji, kZeroS, none, b2);  // xr_jj a_ji
+    sgemm(sp, bi, bj, bj, QX_OP_N, QX_OP_N, kOneS, b1, Xrj, kZeroS, none, b3);   // xr_ii a_ij xr_jj
+    sgemm(sp, bi, bi, bj, QX_OP_N, QX_OP_N, kOneS, b1, b2, kZeroS, none, b4);    // ... a_ji
+
+    if (want_cur) {
+      // layer current from the forward blocks of layer i and the full blocks
+      // of layer j (rgf.py:381-400)
+      const M c1 = slot(C1, bi), c2 = slot(C2, bj), c3 = slot(C3, bj), c4 = slot(C4, bj);
+      sgemm(sp, bj, bi, bi, QX_OP_N, QX_OP_N, kOneS, Aji, Xri, kZeroS, none, c1);  // a_ji xr_ii
+      const M tl[2] = {c2, c3};
+      for (int q = 1; q <= 2; ++q) {
+        const M t4 = slot(T4, bi);
+        sgemm(sp, bj, bi, bi, QX_OP_N, QX_OP_N, kOneS, Aji, slot(F(q, i), bi), kZeroS, none, t4);
+        sgemm(sp, bj, bj, bi, QX_OP_N, QX_OP_C, kOneS, t4, Aji, kZeroS, none, tl[q - 1]);
+        const M sij = Su(q, i);
+        if (sij.p) {
+          sgemm(sp, bj, bj, bi, QX_OP_N, QX_OP_N, kOneS, c1, sij, kZeroS, none, c4);
+          sherm(bj, bj, 1.0, -1.0, c4, 1.0, c4, tl[q - 1]);
+        }
+      }
+      strace_diff(bj, c3, slot(full(1, j), bj), slot(full(2, j), bj), c2,
+                  g.current + e * (int64_t)(N - 1) + i, sh);
+    }
+
+    for (int q = 1; q <= 2; ++q) {
+      const M Xi = slot(F(q, i), bi), Xj = slot(full(q, j), bj);
+      const M t1 = slot(T1, bi), t5 = slot(T5, bj);
+      const M sij = Su(q, i);
+      // temp_1x = B4 x_ii - xr_ii s_ij B3^H
+      sgemm(sp, bi, bi, bi, QX_OP_N, QX_OP_N, kOneS, b4, Xi, kZeroS, none, t1);
+      if (sij.p) {
+        sgemm(sp, bi, bj, bi, QX_OP_N, QX_OP_N, kOneS, Xri, sij, kZeroS, none, t5);
+        sgemm(sp, bi, bi, bj, QX_OP_N, QX_OP_C, kMinusS, t5, b3, kOneS, t1, t1);
+      }
+      // x_ii' = x_ii + (temp_1x - temp_1x^H) + temp_2x B1^H
+      const M Z = slot(full(q, i), bi);
+      sherm(bi, bi, 0.0, 1.0, t1, -1.0, t1, Z, 1.0, Xi);
+      // x_ij = -temp_2x - x_ii B2^H + xr_ii s_ij xr_jj^H,  temp_2x = B1 x_jj
+      M Oij = Ou(q, i);
+      if (!Oij.p) Oij = slot(T4, bj);
+      sgemm(sp, bi, bj, bj, QX_OP_N, QX_OP_N, kMinusS, b1, Xj, kZeroS, none, Oij);  // -temp_2x
+      sgemm(sp, bi, bi, bj, QX_OP_N, QX_OP_C, kMinusS, Oij, b1, kOneS, Z, Z);       // + temp_2x B1^H
+      sgemm(sp, bi, bj, bi, QX_OP_N, QX_OP_C, kMinusS, Xi, b2, kOneS, Oij, Oij);
+      if (sij.p) sgemm(sp, bi, bj, bj, QX_OP_N, QX_OP_C, kOneS, t5, Xrj, kOneS, Oij, Oij);
+      const M low = Ol(q, i);
+      if (!skip_lower[q - 1] && low.p) sherm(bj, bi, 0.0, 0.0, none, -1.0, Oij, low);
+      if (Od(q, i).p) sherm(bi, bi, 0.0, 0.5, Z, -0.5, Z, Od(q, i));
+      if (g.density && q == 1) sdiag_imag(bi, Z, g.density + e * g.n_orb + g.orb[i]);
+    }
+
+    // retarded: G^R_ii = xr_ii + B4 xr_ii; off-diagonal blocks on request
+    if (want_r && want_r_off) {
+      const M up = blk(g.XR, g.sXR, e, g.lXR.up[i], bj), lo = blk(g.XR, g.sXR, e, g.lXR.lo[i], bi);
+      if (up.p) saxpby(bi, bj, kMinusS, b3, kZeroS, none, up);
+      if (lo.p) sgemm(sp, bj, bi, bi, QX_OP_N, QX_OP_N, kMinusS, b2, Xri, kZeroS, none, lo);
+    }
+    const M Zr = slot(full(0, i), bi);
+    sgemm(sp, bi, bi, bi, QX_OP_N, QX_OP_N, kOneS, b4, Xri, kOneS, Xri, Zr);
+    if (want_r) {
+      const M xr = blk(g.XR, g.sXR, e, g.lXR.diag[i], bi);
+      if (xr.p) saxpby(bi, bi, kOneS, Zr, kZeroS, none, xr);
+    }
+  }
+}
+
+SLay make_lay(const qx_btd_layout* l, int N) {
+  SLay s;
+  for (int i = 0; i < kSN; ++i) s.diag[i] = s.up[i] = s.lo[i] = -1;
+  if (!l) return s;
+  for (int i = 0; i < N; ++i) s.diag[i] = (int32_t)l->diag[i];
+  for (int i = 0; i + 1 < N; ++i) {
+    s.up[i] = (int32_t)l->upper[i];
+    s.lo[i] = (int32_t)l->lower[i];
+  }
+  return s;
+}
+
+bool fits32(const qx_btd_layout* l, int N) {
+  if (!l) return true;
+  auto ok = [](int64_t v) { return v >= -1 && v < (int64_t(1) << 31); };
+  for (int i = 0; i < N; ++i)
+    if (!ok(l->diag[i])) return false;
+  for (int i = 0; i + 1 < N; ++i)
+    if (!ok(l->upper[i]) || !ok(l->lower[i])) return false;
+  return true;
+}
+
+}  // namespace
+
+// Fused path of rgf_selected_solve for small blocks; returns false (and does
+// nothing) when the call is not eligible: complex128 arithmetic (4M or 3M),
+// at most 16 blocks of at most 32, offsets below 2^31, and QX_RGF_FUSED not 0.
+bool rgf_small_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, const void* a,
+                     const qx_btd_layout* lsl, const void* sl, const qx_btd_layout* lsg,
+                     const void* sg, const void* const* obc_r, const void* const* obc_l,
+                     const void* const* obc_g, const qx_btd_layout* lxl, void* xl,
+                     const qx_btd_layout* lxg, void* xg, const qx_btd_layout* lxr, void* xr,
+                     void* current, void* corner, double* density, uint32_t flags, int prec) {
+  const char* env = std::getenv("QX_RGF_FUSED");
+  if (env && std::atoi(env) == 0) return false;
+  if (prec != QX_PREC_C128 && prec != QX_PREC_C128_3M) return false;
+  const int N = la->n_blocks;
+  if (N < 1 || N > kSN) return false;
+  for (int i = 0; i < N; ++i)
+    if (la->block_sizes[i] < 1 || la->block_sizes[i] > kSB) return false;
+  const qx_btd_layout* ls[6] = {la, lsl, lsg, lxl, lxg, lxr};
+  for (auto* l : ls)
+    if (!fits32(l, N)) return false;
+  const int64_t S = la->n_stack;
+  if (S <= 0) return true;
+  QX_REQUIRE(S < (int64_t(1) << 31), "selected_solve: stack too large");
+  SmallArgs g;
+  g.N = N;
+  g.flags = (int)flags;
+  g.NS = 6 * N + 16;
+  g.split = prec == QX_PREC_C128_3M ? 3 : 4;
+  int off = 0;
+  for (int i = 0; i < kSN; ++i) {
+    g.bs[i] = i < N ? la->block_sizes[i] : 0;
+    g.orb[i] = off;
+    if (i < N) off += la->block_sizes[i];
+  }
+  g.n_orb = off;
+  g.sA = la->stack_stride;
+  g.sSL = lsl ? lsl->stack_stride : 0;
+  g.sSG = lsg ? lsg->stack_stride : 0;
+  g.sXL = lxl->stack_stride;
+  g.sXG = lxg->stack_stride;
+  g.sXR = lxr ? lxr->stack_stride : 0;
+  g.A = static_cast<const z2*>(a);
+  g.SL = static_cast<const z2*>(sl);
+  g.SG = static_cast<const z2*>(sg);
+  g.XL = static_cast<z2*>(xl);
+  g.XG = static_cast<z2*>(xg);
+  g.XR = static_cast<z2*>(xr);
+  g.lA = make_lay(la, N);
+  g.lSL = make_lay(sl ? lsl : nullptr, N);
+  g.lSG = make_lay(sg ? lsg : nullptr, N);
+  g.lXL = make_lay(lxl, N);
+  g.lXG = make_lay(lxg, N);
+  g.lXR = make_lay(xr ? lxr : nullptr, N);
+  const void* const* ob[3] = {obc_r, obc_l, obc_g};
+  for (int q = 0; q < 3; ++q)
+    for (int i = 0; i < kSN; ++i)
+      g.obc[q][i] = (ob[q] && i < N) ? static_cast<const z2*>(ob[q][i]) : nullptr;
+  g.ws = static_cast<z2*>(h->ensure_ws((size_t)S * g.NS * kSB * kSB * sizeof(z2)));
+  g.info = h->ensure_info(S);
+  g.current = static_cast<z2*>(current);
+  g.corner = static_cast<z2*>(corner);
+  g.density = density;
+  KernelTimer* timer = active_timer;
+  hipEvent_t ev0 = nullptr;
+  if (timer) timer->begin(st, &ev0);
+  hipLaunchKernelGGL(rgf_small_kernel, dim3((unsigned)S), dim3(kThreads), 0, st, g);
+  QX_HIP_CHECK(hipGetLastError());
+  if (timer) {
+    // algorithmic GEMM flops of the sweep (8 b^3 per product, products with
+    // stored operands only): counted as GEMM work
+    double fl = 0.0;
+    for (int i = 0; i + 1 < N; ++i) {
+      const double b = la->block_sizes[i + 1];
+      fl += 23.0 * 8.0 * b * b * b;
+    }
+    timer->end(st, ev0, KernelTimer::GEMM, fl * (double)S);
+  }
+  return true;
+}
+
+}  // namespace qx

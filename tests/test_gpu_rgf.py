@@ -272,3 +272,41 @@ def test_corner_and_transmission(device, workload):
     gr = 1j * (w.sigma_b_right - w.sigma_b_right.conj().T)
     t = transmission(corner, np.broadcast_to(gl, (ne, b0, b0)), np.broadcast_to(gr, (ne, bl, bl)))
     assert np.allclose(t.cpu().numpy(), o_transmission(ref, gl, gr), rtol=1e-10)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c128-3m"])
+def test_fused_small_block_path_matches_general(device, monkeypatch, precision):
+    """Stacks of blocks <= 32 run the whole sweep of one energy in one
+    workgroup; its outputs (G^<, G^>, G^R with off-diagonal blocks, layer
+    current, corner block, density) equal the per-product path's, with Sigma
+    off-diagonal blocks, OBC blocks and mixed block sizes."""
+    bs = np.array([4, 8, 8, 6, 8, 4, 8, 2, 8])
+    dense = bt_dense(bs, seed=11)
+    coo = sps.coo_matrix(dense)
+    b = coo - coo.conj().T
+    ne, n = 6, len(bs)
+    oa = ODSDBCOO.from_sparray(coo, bs, (ne,))
+    ob = ODSDBCOO.from_sparray(sps.coo_matrix(b), bs, (ne,))
+    rng = np.random.default_rng(5)
+    gobc = OBCBlocks(n)
+    gobc.retarded[0] = torch.as_tensor(
+        0.05 * (rng.standard_normal((ne, 4, 4)) + 1j * rng.standard_normal((ne, 4, 4))),
+        device=device)
+    gobc.lesser[0] = torch.as_tensor(0.1j * np.broadcast_to(np.eye(4), (ne, 4, 4)).copy(),
+                                     device=device)
+    gobc.greater[n - 1] = torch.as_tensor(-0.1j * np.broadcast_to(np.eye(8), (ne, 8, 8)).copy(),
+                                          device=device)
+    outs = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("QX_RGF_FUSED", fused)
+        a, bl, bg = to_gpu(oa, device), to_gpu(ob, device), to_gpu(ob, device)
+        corner = torch.zeros((ne, bs[0], bs[-1]), dtype=torch.complex128, device=device)
+        dens = torch.zeros((ne, int(bs.sum())), dtype=torch.float64, device=device)
+        xl, xg, xr, cur = RGF(100, precision=precision, retarded_offdiag=True).selected_solve(
+            a, bl, bg, obc_blocks=gobc, return_retarded=True, return_current=True,
+            corner=corner, density=dens)
+        outs[fused] = [t.cpu().numpy() for t in (xl.data, xg.data, xr.data, cur, corner, dens)]
+    for got, ref in zip(outs["1"], outs["0"]):
+        scale = np.max(np.abs(ref))
+        assert scale > 0
+        assert np.max(np.abs(got - ref)) <= 1e-12 * scale
