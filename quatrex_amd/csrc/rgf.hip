@@ -127,7 +127,12 @@ size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve) {
   int bmax = 1;
   for (int i = 0; i < la->n_blocks; ++i) bmax = std::max(bmax, (int)la->block_sizes[i]);
   const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 22 : (int64_t)la->n_blocks + 10;  // see slot maps
-  return (size_t)(slots * la->n_stack * (int64_t)bmax * bmax) * sizeof(z2);
+  size_t bytes = (size_t)(slots * la->n_stack * (int64_t)bmax * bmax) * sizeof(z2);
+  // the fused small-block solve keeps 6 N + 16 slots of 32 x 32 per energy
+  if (solve && la->n_blocks <= 16 && bmax <= 32)
+    bytes = std::max(bytes, (size_t)((6 * (int64_t)la->n_blocks + 16) * la->n_stack * 1024) *
+                                sizeof(z2));
+  return bytes;
 }
 
 void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,

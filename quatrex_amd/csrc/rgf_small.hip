@@ -28,6 +28,7 @@
 namespace qx {
 namespace {
 
+// (the workspace bound of rgf_workspace_bytes in rgf.hip follows these)
 constexpr int kSB = 32;   // largest block size
 constexpr int kSN = 16;   // largest block count
 constexpr int kThreads = 256;

@@ -123,10 +123,13 @@ class Device:
 
     def lane_bytes_per_energy(self) -> int:
         """HBM a lane needs per energy of a chunk: A, two block-diagonal
-        outputs, the corner block and the RGF workspace slots."""
+        outputs, the corner block and the RGF workspace slots (the fused
+        small-block solve keeps 6 N + 16 slots of 32 x 32)."""
         b2 = self.b * self.b
-        slots = 3 * self.n_blocks + 22
-        return 16 * (self.nnz + 2 * self.nnz_d + b2 + slots * b2)
+        ws = (3 * self.n_blocks + 22) * b2
+        if self.n_blocks <= 16 and self.b <= 32:
+            ws = max(ws, (6 * self.n_blocks + 16) * 1024)
+        return 16 * (self.nnz + 2 * self.nnz_d + b2 + ws)
 
     def resident_bytes(self) -> int:
         return 16 * (2 * self.nnz + 2 * self.nnz_d) + 24 * self.n_energies * self.n_orb
@@ -271,6 +274,34 @@ class Item:
     flops: float = 0.0
 
 
+def plan_items(devices, n_lanes: int):
+    """Chunks of every device's energies, assigned to ``n_lanes`` lanes.
+
+    ``devices`` holds (n_energies, flops per energy, largest chunk) per
+    device.  A chunk is at most that size and at most a quarter of one
+    lane's fair share of the flops, so a device that dominates the mix is
+    spread over several lanes; chunks go, largest first, to the least loaded
+    lane (longest-processing-time rule).  Returns the item lists per lane and
+    the flops per lane.
+    """
+    items = []
+    total = sum(n * f for n, f, _ in devices)
+    share = total / (4 * n_lanes)
+    for k, (n, f, chunk) in enumerate(devices):
+        c = max(1, min(int(chunk), max(1, int(share // f)) if f > 0 else int(chunk)))
+        for e0 in range(0, n, c):
+            e1 = min(e0 + c, n)
+            items.append(Item(k, e0, e1, (e1 - e0) * f))
+    items.sort(key=lambda it: -it.flops)
+    load = [0.0] * n_lanes
+    assign = [[] for _ in range(n_lanes)]
+    for it in items:
+        k = int(np.argmin(load))
+        assign[k].append(it)
+        load[k] += it.flops
+    return assign, load
+
+
 @dataclass
 class UnevenBatch:
     """Devices of mixed shapes solved together on ``n_lanes`` lanes."""
@@ -293,28 +324,13 @@ class UnevenBatch:
         return max(1, min(c, cap, d.n_energies))
 
     def plan(self) -> None:
-        """Split every device's energies into chunks and assign them to lanes.
-
-        A chunk is at most the tuned size and at most a quarter of one
-        lane's fair share of the flops, so a device that dominates the mix
-        is spread over several lanes.
-        """
-        items = []
-        total = sum(d.n_energies * d.flops_per_energy() for d in self.devices)
-        share = total / (4 * len(self.lanes))
-        for k, d in enumerate(self.devices):
-            c = min(self.chunk_for(d), max(1, int(share // d.flops_per_energy())))
-            for e0 in range(0, d.n_energies, c):
-                e1 = min(e0 + c, d.n_energies)
-                items.append(Item(k, e0, e1, (e1 - e0) * d.flops_per_energy()))
-        items.sort(key=lambda it: -it.flops)
-        load = [0.0] * len(self.lanes)
-        for lane in self.lanes:
-            lane.items = []
-        for it in items:
-            k = int(np.argmin(load))
-            self.lanes[k].items.append(it)
-            load[k] += it.flops
+        """Split every device's energies into chunks and assign them to lanes
+        (`plan_items`)."""
+        assign, load = plan_items(
+            [(d.n_energies, d.flops_per_energy(), self.chunk_for(d)) for d in self.devices],
+            len(self.lanes))
+        for lane, items in zip(self.lanes, assign):
+            lane.items = items
         self.lane_flops = load
         self.graph = None
 

@@ -60,6 +60,15 @@ class Request:
         return self.job.transmission
 
 
+def next_admission(needs, used: int, budget: int):
+    """Index of the first queued request (arrival order) whose bytes
+    ``needs[i]`` fit beside ``used`` in ``budget``, or None."""
+    for i, need in enumerate(needs):
+        if used + need <= budget:
+            return i
+    return None
+
+
 @dataclass
 class Server:
     device: torch.device
@@ -152,22 +161,20 @@ class Server:
                 while i < len(pending) and pending[i].arrival <= t:
                     queue.append(pending[i])
                     i += 1
-                # admit: first request in arrival order that fits
-                admitted = True
-                while admitted and idle and queue:
-                    admitted = False
-                    for q, req in enumerate(queue):
-                        need = self._need(req)
-                        if state["used"] + need <= self.hbm_budget_bytes:
-                            queue.pop(q)
-                            k = idle.pop(0)
-                            state["used"] += need
-                            req.t_admit, req.lane = now(), k
-                            th = threading.Thread(target=lane_main, args=(k, req))
-                            threads.append(th)
-                            th.start()
-                            admitted = True
-                            break
+                # admit: first request in arrival order that fits, while a
+                # lane is idle
+                while idle and queue:
+                    q = next_admission([self._need(r) for r in queue], state["used"],
+                                       self.hbm_budget_bytes)
+                    if q is None:
+                        break
+                    req = queue.pop(q)
+                    k = idle.pop(0)
+                    state["used"] += self._need(req)
+                    req.t_admit, req.lane = now(), k
+                    th = threading.Thread(target=lane_main, args=(k, req))
+                    threads.append(th)
+                    th.start()
                 wait = None
                 if i < len(pending):
                     wait = max(0.0, pending[i].arrival - now())
