@@ -360,9 +360,13 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   }
 
   // ---- backward sweep (rgf.py:308-404) ----
+  // gr[i]: full G^R_ii once computed (the caller's output block when it is
+  // written, else a workspace slot)
+  std::vector<Mat> gr(N);
+  gr[N - 1] = W.get(sr[N - 1], bs(N - 1));
   for (int i = N - 2; i >= 0; --i) {
     const int j = i + 1, bi = bs(i), bj = bs(j);
-    const Mat Xri = W.get(sr[i], bi), Xrj = W.get(sr[j], bj);
+    const Mat Xri = W.get(sr[i], bi), Xrj = gr[j];
     const Mat Aij = A.upper(i), Aji = A.lower(i);
     const Mat B1 = W.get(B1s, bj), B2 = W.get(B2s, bi), B3 = W.get(B3s, bj), B4 = W.get(B4s, bi);
     zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, Aij, kZero, B1, gp);  // xr_ii a_ij
@@ -425,10 +429,17 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       if (!XR.lower(i).zero())
         zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, B2, Xri, kZero, XR.lower(i), gp);
     }
-    const Mat Z = W.get(spare, bi);
-    zgemm_in(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xri, kOne, Xri, Z, gp);
-    std::swap(spare, sr[i]);
-    if (want_r && !XR.diag(i).zero()) ew_axpby(st, S, bi, bi, kOne, Z, kZero, Mat(), XR.diag(i));
+    const Mat out_r = want_r ? XR.diag(i) : Mat();
+    if (!out_r.zero()) {
+      // straight into the output block, which the next step reads
+      zgemm_in(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xri, kOne, Xri, out_r, gp);
+      gr[i] = out_r;
+    } else {
+      const Mat Z = W.get(spare, bi);
+      zgemm_in(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xri, kOne, Xri, Z, gp);
+      std::swap(spare, sr[i]);
+      gr[i] = Z;
+    }
   }
   });
 }

@@ -23,6 +23,14 @@ from tests.conftest import bt_dense, rel_err
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True, params=["fused", "per-product"])
+def rgf_path(request, monkeypatch):
+    """Every case runs on both solve paths: small-block stacks take the
+    fused one-workgroup-per-energy kernel unless QX_RGF_FUSED=0."""
+    monkeypatch.setenv("QX_RGF_FUSED", "1" if request.param == "fused" else "0")
+    return request.param
+
 BLOCK_SIZES = [np.array([2] * 10), np.array([2] * 3 + [4] * 2 + [2] * 3)]
 STACKS = [(10,), (7, 2), (9, 2, 4)]
 
