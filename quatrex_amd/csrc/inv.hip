@@ -666,11 +666,20 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const
                  scr.int_stride >= 3 * (int64_t)n,
              "inv: scratch too small");
   QX_REQUIRE(X.p != scr.buf, "inv: output aliases the scratch buffer");
+  QX_REQUIRE(precision == QX_PREC_C128 || precision == QX_PREC_C64 ||
+                 precision == QX_PREC_C128_3M,
+             "inv: precision must be QX_PREC_C128, QX_PREC_C64 or QX_PREC_C128_3M");
   if (batch <= 0) return;
   QX_REQUIRE(batch < (int64_t(1) << 31), "inv: batch too large");
   KernelTimer* timer = active_timer;
   hipEvent_t ev0 = nullptr;
   if (timer) timer->begin(s, &ev0);
+
+  // n <= 32 in complex128: one wave per matrix, one launch
+  if (precision != QX_PREC_C64 && inv_small(s, batch, n, A, S, X, info)) {
+    if (timer) timer->end(s, ev0, KernelTimer::INV, 8.0 * (double)n * n * n * batch);
+    return;
+  }
 
   const bool big = n > 256;
   // Panel width: 32 columns up to n = 256 (16 above, where two rows per
@@ -690,9 +699,6 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const
       (A.p == start.p) && S.zero() && A.ld == start.ld && A.stride == start.stride;
   if (!in_place) ew_axpby(s, batch, n, n, Cplx{1.0, 0.0}, A, Cplx{-1.0, 0.0}, S, start);
 
-  QX_REQUIRE(precision == QX_PREC_C128 || precision == QX_PREC_C64 ||
-                 precision == QX_PREC_C128_3M,
-             "inv: precision must be QX_PREC_C128, QX_PREC_C64 or QX_PREC_C128_3M");
   if (precision == QX_PREC_C64) {
     if (big) run_inverse<float, 16, 2>(s, batch, n, X, scr, s0, info);
     else if (NB == 16) run_inverse<float, 16, 1>(s, batch, n, X, scr, s0, info);

@@ -54,6 +54,11 @@ struct Error {
 
 // ---- kernels (launchers) --------------------------------------------------
 
+// Batched X = inv(A - S) for blocks n <= 32, one wave per matrix
+// (rgf_small.hip); returns false, having done nothing, for larger n.
+bool inv_small(hipStream_t st, int64_t batch, int n, const Mat& A, const Mat& S, const Mat& X,
+               int* info);
+
 // Fused selected solve for stacks of small blocks (rgf_small.hip); returns
 // false, having done nothing, when the call is not eligible.
 bool rgf_small_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, const void* a,

@@ -106,15 +106,20 @@ __device__ inline void s_crecip(double ar, double ai, double& qr, double& qi) {
   }
 }
 
-struct Shared {
+// LDS of one wave's inverse.
+struct WaveShared {
   z2 prow[kSB], krow[kSB];
   int perm[kSB], idx[kSB];
   z2 tile[kSB][kSB + 1];
+};
+
+struct Shared {
+  WaveShared inv;
   double red[2][kThreads / 64];
 };
 
 // C = alpha op(A) op(B) + beta Cin, m, n, k <= 32; a null A or B is zero.
-__device__ void sgemm(int split, int m, int n, int k, int opa, int opb, Cplx al, M A, M B,
+__device__ __forceinline__ void sgemm(int split, int m, int n, int k, int opa, int opb, Cplx al, M A, M B,
                       Cplx be, M Cin, M C) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r0 = (w >> 1) * 16, c0 = (w & 1) * 16;
@@ -190,7 +195,7 @@ __device__ void sgemm(int split, int m, int n, int k, int opa, int opb, Cplx al,
 }
 
 // Y = a A + b B (null operands are zero).
-__device__ void saxpby(int m, int n, Cplx a, M A, Cplx b, M B, M Y) {
+__device__ __forceinline__ void saxpby(int m, int n, Cplx a, M A, Cplx b, M B, M Y) {
   for (int t = threadIdx.x; t < m * n; t += kThreads) {
     const int r = t / n, c = t - r * n;
     double yr = 0.0, yi = 0.0;
@@ -211,7 +216,7 @@ __device__ void saxpby(int m, int n, Cplx a, M A, Cplx b, M B, M Y) {
 
 // Y = cy Y + ca A + cb B^H + cc D (real coefficients; Y, m x n, aliases none
 // of A, B, D; B is n x m).
-__device__ void sherm(int m, int n, double cy, double ca, M A, double cb, M B, M Y,
+__device__ __forceinline__ void sherm(int m, int n, double cy, double ca, M A, double cb, M B, M Y,
                       double cc = 0.0, M D = M{nullptr, 0}) {
   for (int t = threadIdx.x; t < m * n; t += kThreads) {
     const int r = t / n, c = t - r * n;
@@ -241,13 +246,13 @@ __device__ void sherm(int m, int n, double cy, double ca, M A, double cb, M B, M
   __syncthreads();
 }
 
-__device__ void sdiag_imag(int n, M X, double* out) {
+__device__ __forceinline__ void sdiag_imag(int n, M X, double* out) {
   for (int t = threadIdx.x; t < n; t += kThreads) out[t] = X.p[t * X.ld + t].y;
   __syncthreads();
 }
 
 // out = Tr(X Y) - Tr(Z W), n x n.
-__device__ void strace_diff(int n, M X, M Y, M Z, M W, z2* out, Shared& sh) {
+__device__ __forceinline__ void strace_diff(int n, M X, M Y, M Z, M W, z2* out, Shared& sh) {
   double sr = 0.0, si = 0.0;
   for (int t = threadIdx.x; t < n * n; t += kThreads) {
     const int a = t / n, b = t - a * n;
@@ -283,16 +288,25 @@ __device__ void strace_diff(int n, M X, M Y, M Z, M W, z2* out, Shared& sh) {
   __syncthreads();
 }
 
-// In-place inverse of the n x n block X (n <= 32) by wave 0.
-__device__ void sinv(int n, M X, int* info, Shared& sh) {
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x, r = lane >> 1, h = lane & 1;
+// X = inv(A - S) for one n x n matrix (n <= 32; S may be null; X may
+// alias A), computed by one whole wave with no workgroup barrier.
+__device__ __forceinline__ void sinv_wave(int n, const z2* A, int lda, const z2* S, int lds, z2* X, int ldx,
+                          int* info, WaveShared& sh) {
+  {
+    const int lane = threadIdx.x & 63, r = lane >> 1, h = lane & 1;
     double xr[16], xi[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
       const int col = 16 * h + c;
       z2 v = make_double2(0.0, 0.0);
-      if (r < n && col < n) v = X.p[r * X.ld + col];
+      if (r < n && col < n) {
+        v = A[r * lda + col];
+        if (S) {
+          const z2 u = S[r * lds + col];
+          v.x -= u.x;
+          v.y -= u.y;
+        }
+      }
       xr[c] = v.x;
       xi[c] = v.y;
     }
@@ -387,10 +401,29 @@ __device__ void sinv(int n, M X, int* info, Shared& sh) {
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
       const int col = 16 * h + c;
-      if (r < n && col < n) X.p[r * X.ld + col] = sh.tile[r][sh.idx[col]];
+      if (r < n && col < n) X[r * ldx + col] = sh.tile[r][sh.idx[col]];
     }
   }
+}
+
+// In-place inverse of the block X by wave 0 of the workgroup.
+__device__ __forceinline__ void sinv(int n, M X, int* info, Shared& sh) {
+  if (threadIdx.x < 64) sinv_wave(n, X.p, X.ld, nullptr, 0, X.p, X.ld, info, sh.inv);
   __syncthreads();
+}
+
+// Batched X_e = inv(A_e - S_e), n <= 32: one matrix per wave, four per
+// workgroup (the whole-wave condition e < batch is uniform per wave).
+__global__ __launch_bounds__(kThreads) void inv_small_kernel(int64_t batch, int n, const z2* A,
+                                                             int64_t sA, int lda, const z2* S,
+                                                             int64_t sS, int lds, z2* X,
+                                                             int64_t sX, int ldx, int* info) {
+  __shared__ WaveShared sh[kThreads / 64];
+  const int w = threadIdx.x >> 6;
+  const int64_t e = (int64_t)blockIdx.x * (kThreads / 64) + w;
+  if (e < batch)
+    sinv_wave(n, A + e * sA, lda, S ? S + e * sS : nullptr, lds, X + e * sX, ldx, info + e,
+              sh[w]);
 }
 
 const Cplx kOneS{1.0, 0.0}, kMinusS{-1.0, 0.0}, kZeroS{0.0, 0.0};
@@ -609,6 +642,27 @@ bool fits32(const qx_btd_layout* l, int N) {
 }
 
 }  // namespace
+
+// Batched inverse X = inv(A - S) of small blocks in one launch (complex128
+// arithmetic): used for n <= 16, and for n <= 32 from 1024 matrices on (one
+// wave per matrix wins once the stack fills the chip; at n = 32 and 256
+// matrices the split panel kernels are faster, 48 vs 68 us).  Returns false
+// otherwise, or when QX_INV_SMALL=0 (QX_INV_SMALL=1 forces it for n <= 32).
+bool inv_small(hipStream_t st, int64_t batch, int n, const Mat& A, const Mat& S, const Mat& X,
+               int* info) {
+  const char* env = std::getenv("QX_INV_SMALL");
+  const int mode = env ? std::atoi(env) : -1;
+  if (mode == 0) return false;
+  if (n < 1 || n > kSB) return false;
+  if (mode != 1 && n > 16 && batch < 1024) return false;
+  if (batch <= 0) return true;
+  const int64_t blocks = (batch + kThreads / 64 - 1) / (kThreads / 64);
+  QX_REQUIRE(blocks < (int64_t(1) << 31), "inv: batch too large");
+  hipLaunchKernelGGL(inv_small_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, st, batch, n,
+                     A.p, A.stride, A.ld, S.p, S.stride, S.ld, X.p, X.stride, X.ld, info);
+  QX_HIP_CHECK(hipGetLastError());
+  return true;
+}
 
 // Fused path of rgf_selected_solve for small blocks; returns false (and does
 // nothing) when the call is not eligible: complex128 arithmetic (4M or 3M),

@@ -150,7 +150,7 @@ def _inv(dev, A, X, prec, S=None):
     L.sync_check(dev, "inv")
 
 
-@pytest.mark.parametrize("n", [1, 2, 15, 16, 17, 33, 64, 100, 256, 416, 512])
+@pytest.mark.parametrize("n", [1, 2, 15, 16, 17, 31, 32, 33, 64, 100, 256, 416, 512])
 @pytest.mark.parametrize("prec", [0, 1], ids=["c128", "c64"])
 def test_inverse(device, n, prec):
     rng = np.random.default_rng(n)
@@ -171,6 +171,28 @@ def test_inverse(device, n, prec):
         assert rel_err(got[e], ref[e]) < tol, (e, rel_err(got[e], ref[e]), tol)
 
 
+@pytest.mark.parametrize("n", [1, 3, 20, 32])
+@pytest.mark.parametrize("batch", [1, 5, 9])
+def test_small_inverse_matches_split_kernels(device, monkeypatch, n, batch):
+    """n <= 32 (complex128): one wave per matrix in one launch; same pivot
+    rule as the panel kernels, so the same permutation and results to
+    rounding (batch not a multiple of the four matrices per workgroup)."""
+    rng = np.random.default_rng(10 * n + batch)
+    a = _cplx(rng, (batch, n, n))
+    s = 0.1 * _cplx(rng, (batch, n, n))
+    A, S = torch.as_tensor(a, device=device), torch.as_tensor(s, device=device)
+    outs = []
+    for small in ("1", "0"):
+        monkeypatch.setenv("QX_INV_SMALL", small)
+        X = torch.empty_like(A)
+        _inv(device, A, X, 0, S)
+        outs.append(X.cpu().numpy())
+    cond = max(np.linalg.cond(a[e] - s[e]) for e in range(batch))
+    for e in range(batch):
+        assert rel_err(outs[0][e], outs[1][e]) < 1e-15 * cond * max(n, 4)
+        assert rel_err(outs[0][e], np.linalg.inv(a[e] - s[e])) < 1e-15 * cond * max(n, 4)
+
+
 def test_inverse_in_place(device):
     rng = np.random.default_rng(5)
     a = _cplx(rng, (3, 40, 40))
@@ -179,6 +201,12 @@ def test_inverse_in_place(device):
     got = A.cpu().numpy()
     for e in range(3):
         assert rel_err(got[e], np.linalg.inv(a[e])) < 1e-12
+    # the one-wave path (n <= 16) in place
+    b = _cplx(rng, (5, 12, 12))
+    B = torch.as_tensor(b, device=device)
+    _inv(device, B, B, 0)
+    for e in range(5):
+        assert rel_err(B[e].cpu().numpy(), np.linalg.inv(b[e])) < 1e-12
 
 
 def test_singular_block_raises(device):
