@@ -68,6 +68,9 @@ struct GemmArgs {
   int ldcin;
   int tilesM, tilesN;
   int beta_zero;
+  z2* Ct;  // optional second output Ct = -C^H (n x m), or null
+  int64_t sCt;
+  int ldct;
 };
 
 // Hardware dispatches workgroup ids round-robin over the 8 XCDs.  Return a
@@ -284,6 +287,7 @@ void zgemm_kernel(GemmArgs g) {
             outi += g.br * c.y + g.bi * c.x;
           }
           *dst = make_double2(outr, outi);
+          if (g.Ct) g.Ct[e * g.sCt + (int64_t)col * g.ldct + row] = make_double2(-outr, outi);
         }
       }
 }
@@ -488,6 +492,7 @@ void zgemm_reg_kernel(GemmArgs g) {
             outi += g.br * c.y + g.bi * c.x;
           }
           C[(int64_t)row * g.ldc + col] = make_double2(outr, outi);
+          if (g.Ct) g.Ct[e * g.sCt + (int64_t)col * g.ldct + row] = make_double2(-outr, outi);
         }
       }
 }
@@ -660,13 +665,15 @@ int gemm_split(int precision) {
 void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
            Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& C,
            int precision) {
-  zgemm_in(s, batch, m, n, k, opA, opB, alpha, A, B, beta, C, C, precision);
+  zgemm_in(s, batch, m, n, k, opA, opB, alpha, A, B, beta, C, C, precision, Mat());
 }
 
 void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
               Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
-              const Mat& C, int precision) {
+              const Mat& C, int precision, const Mat& Ct) {
   QX_REQUIRE(!C.zero(), "zgemm: output is a zero matrix");
+  QX_REQUIRE(Ct.zero() || (Ct.p != C.p && Ct.p != A.p && Ct.p != B.p && Ct.p != Cin.p),
+             "zgemm: the transposed output aliases an operand");
   QX_REQUIRE(opA == QX_OP_N || opA == QX_OP_C, "zgemm: bad op_a");
   QX_REQUIRE(opB == QX_OP_N || opB == QX_OP_C, "zgemm: bad op_b");
   QX_REQUIRE(precision == QX_PREC_C128 || precision == QX_PREC_C64 ||
@@ -677,9 +684,13 @@ void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int op
   if (A.zero() || B.zero() || k <= 0) {
     // op(A) op(B) == 0: C = beta Cin.
     ew_axpby(s, batch, m, n, beta_zero ? Cplx{0, 0} : beta, Cin, Cplx{0, 0}, Mat(), C);
+    if (!Ct.zero()) ew_neg_h(s, batch, m, n, C, Ct);
     return;
   }
   GemmArgs g;
+  g.Ct = Ct.p;
+  g.sCt = Ct.stride;
+  g.ldct = Ct.ld;
   g.m = m;
   g.n = n;
   g.k = k;

@@ -73,9 +73,10 @@ void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
            Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& C,
            int precision);
 // C = alpha op(A) op(B) + beta Cin (Cin may equal C; a zero Cin counts as 0).
+// A non-zero Ct (n x m, aliasing no operand) also receives -C^H.
 void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
               Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
-              const Mat& C, int precision);
+              const Mat& C, int precision, const Mat& Ct = Mat());
 
 // Ozaki-split GEMM on bf16 matrix cores: C = alpha op(A) op(B) + beta Cin
 // from `slices` 7-bit digit slices per operand (2..7); ws holds the digits.

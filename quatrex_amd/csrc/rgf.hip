@@ -414,10 +414,12 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       if (Oij.zero()) Oij = W.get(T4s, bj);
       zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, B1, Xj, kZero, Oij, gp);  // -temp_2x
       zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, Oij, B1, kOne, Z, gp);    // + temp_2x B1^H
-      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, Xi, B2, kOne, Oij, gp);
+      // the GEMM that completes x_ij also writes the lower block -x_ij^H
+      const Mat low = (!q.skip_lower) ? q.out->lower(i) : Mat();
+      zgemm_in(st, S, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, Xi, B2, kOne, Oij, Oij, gp,
+               sij.zero() ? low : Mat());
       if (!sij.zero())
-        zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_C, kOne, T5, Xrj, kOne, Oij, gp);
-      if (!q.skip_lower && !q.out->lower(i).zero()) ew_neg_h(st, S, bi, bj, Oij, q.out->lower(i));
+        zgemm_in(st, S, bi, bj, bj, QX_OP_N, QX_OP_C, kOne, T5, Xrj, kOne, Oij, Oij, gp, low);
       std::swap(spare, (*q.slot)[i]);
       if (!q.out->diag(i).zero()) ew_antiherm(st, S, bi, 0.5, Z, q.out->diag(i));
       if (dens && &q == &qs[0]) ew_diag_imag(st, S, bi, Z, dens + orb_off[i], n_orb);
