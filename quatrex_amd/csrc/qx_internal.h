@@ -59,6 +59,11 @@ struct Error {
 bool inv_small(hipStream_t st, int64_t batch, int n, const Mat& A, const Mat& S, const Mat& X,
                int* info);
 
+// Fused selected inversion for stacks of small blocks (rgf_small.hip);
+// returns false, having done nothing, when the call is not eligible.
+bool rgf_small_inv(qx_handle* h, hipStream_t st, const qx_btd_layout* la, const void* a,
+                   const void* const* obc_r, const qx_btd_layout* lx, void* x, int prec);
+
 // Fused selected solve for stacks of small blocks (rgf_small.hip); returns
 // false, having done nothing, when the call is not eligible.
 bool rgf_small_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, const void* a,

@@ -128,10 +128,12 @@ size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve) {
   for (int i = 0; i < la->n_blocks; ++i) bmax = std::max(bmax, (int)la->block_sizes[i]);
   const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 22 : (int64_t)la->n_blocks + 10;  // see slot maps
   size_t bytes = (size_t)(slots * la->n_stack * (int64_t)bmax * bmax) * sizeof(z2);
-  // the fused small-block solve keeps 6 N + 16 slots of 32 x 32 per energy
-  if (solve && la->n_blocks <= 16 && bmax <= 32)
-    bytes = std::max(bytes, (size_t)((6 * (int64_t)la->n_blocks + 16) * la->n_stack * 1024) *
-                                sizeof(z2));
+  // the fused small-block kernels keep 6 N + 16 (solve) or 2 N + 3 (inverse)
+  // slots of 32 x 32 per energy
+  if (la->n_blocks <= 16 && bmax <= 32) {
+    const int64_t fs = solve ? 6 * (int64_t)la->n_blocks + 16 : 2 * (int64_t)la->n_blocks + 3;
+    bytes = std::max(bytes, (size_t)(fs * la->n_stack * 1024) * sizeof(z2));
+  }
   return bytes;
 }
 
@@ -145,6 +147,8 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   const int64_t Stot = la->n_stack;
   if (Stot == 0) return;
   const int bmax = block_limits(la, "selected_inv");
+  gemm_precision(prec);  // validates the selector
+  if (rgf_small_inv(h, st0, la, a, obc_r, lx, x, prec)) return;
   z2* ws = static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, false)));
   int* info_all = h->ensure_info(Stot);
   const Btd A0(la, a), X0(lx, x);

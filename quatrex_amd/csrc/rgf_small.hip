@@ -619,6 +619,83 @@ __global__ __launch_bounds__(kThreads) void rgf_small_kernel(SmallArgs g) {
   }
 }
 
+// Fused selected inversion (rgf.py:25-113 restated in rgf_selected_inv of
+// rgf.hip), one workgroup per energy.
+struct SmallInvArgs {
+  int N, NS, split;
+  int bs[kSN];
+  int64_t sA, sX;
+  const z2* A;
+  z2* X;
+  SLay lA, lX;
+  const z2* obc[kSN];
+  z2* ws;
+  int* info;
+};
+
+__global__ __launch_bounds__(kThreads) void rgf_small_inv_kernel(SmallInvArgs g) {
+  __shared__ Shared sh;
+  const int64_t e = blockIdx.x;
+  const int N = g.N, sp = g.split;
+  int* info = g.info + e;
+  z2* W = g.ws + e * (int64_t)g.NS * kSB * kSB;
+  auto slot = [&](int s, int ld) { return M{W + (int64_t)s * kSB * kSB, ld}; };
+  auto bs = [&](int i) { return g.bs[i]; };
+  // slots: forward x_ii (0..N-1), full X_ii from the backward sweep
+  // (N..2N-1; block N-1 keeps its forward slot), temporaries
+  auto F = [&](int i) { return i; };
+  auto full = [&](int i) { return i == N - 1 ? i : N + i; };
+  const int T1 = 2 * N, T2 = T1 + 1, T3 = T1 + 2;
+  const M none{nullptr, 0};
+  auto Ad = [&](int i) { return blk(g.A, g.sA, e, g.lA.diag[i], bs(i)); };
+  auto Au = [&](int i) { return blk(g.A, g.sA, e, g.lA.up[i], bs(i + 1)); };
+  auto Al = [&](int i) { return blk(g.A, g.sA, e, g.lA.lo[i], bs(i)); };
+  auto obc = [&](int i) {
+    const z2* p = g.obc[i];
+    return M{p ? const_cast<z2*>(p) + e * (int64_t)bs(i) * bs(i) : nullptr, bs(i)};
+  };
+
+  // forward sweep (rgf.py:70-92)
+  saxpby(bs(0), bs(0), kOneS, Ad(0), kMinusS, obc(0), slot(F(0), bs(0)));
+  sinv(bs(0), slot(F(0), bs(0)), info, sh);
+  for (int i = 0; i + 1 < N; ++i) {
+    const int j = i + 1, bi = bs(i), bj = bs(j);
+    const M t1 = slot(T1, bi), Xj = slot(F(j), bj);
+    sgemm(sp, bj, bi, bi, QX_OP_N, QX_OP_N, kOneS, Al(i), slot(F(i), bi), kZeroS, none, t1);
+    M Ajj = Ad(j);
+    if (obc(j).p) {
+      saxpby(bj, bj, kOneS, Ad(j), kMinusS, obc(j), Xj);
+      Ajj = Xj;
+    }
+    sgemm(sp, bj, bj, bi, QX_OP_N, QX_OP_N, kMinusS, t1, Au(i), kOneS, Ajj, Xj);
+    sinv(bj, Xj, info, sh);
+  }
+  {
+    const M xd = blk(g.X, g.sX, e, g.lX.diag[N - 1], bs(N - 1));
+    if (xd.p) saxpby(bs(N - 1), bs(N - 1), kOneS, slot(F(N - 1), bs(N - 1)), kZeroS, none, xd);
+  }
+  // backward sweep (rgf.py:97-110)
+  for (int i = N - 2; i >= 0; --i) {
+    const int j = i + 1, bi = bs(i), bj = bs(j);
+    const M Xii = slot(F(i), bi), Xjj = slot(full(j), bj);
+    const M t1 = slot(T1, bi), t2 = slot(T2, bj);
+    // x_ji = -x_jj a_ji x_ii
+    sgemm(sp, bj, bi, bj, QX_OP_N, QX_OP_N, kOneS, Xjj, Al(i), kZeroS, none, t1);
+    M Xji = blk(g.X, g.sX, e, g.lX.lo[i], bi);
+    if (!Xji.p) Xji = slot(T3, bi);
+    sgemm(sp, bj, bi, bi, QX_OP_N, QX_OP_N, kMinusS, t1, Xii, kZeroS, none, Xji);
+    // x_ij = -x_ii a_ij x_jj
+    sgemm(sp, bi, bj, bi, QX_OP_N, QX_OP_N, kOneS, Xii, Au(i), kZeroS, none, t2);
+    const M up = blk(g.X, g.sX, e, g.lX.up[i], bj);
+    if (up.p) sgemm(sp, bi, bj, bj, QX_OP_N, QX_OP_N, kMinusS, t2, Xjj, kZeroS, none, up);
+    // x_ii <- x_ii - x_ii a_ij x_ji
+    const M Z = slot(full(i), bi);
+    sgemm(sp, bi, bi, bj, QX_OP_N, QX_OP_N, kMinusS, t2, Xji, kOneS, Xii, Z);
+    const M xd = blk(g.X, g.sX, e, g.lX.diag[i], bi);
+    if (xd.p) saxpby(bi, bi, kOneS, Z, kZeroS, none, xd);
+  }
+}
+
 SLay make_lay(const qx_btd_layout* l, int N) {
   SLay s;
   for (int i = 0; i < kSN; ++i) s.diag[i] = s.up[i] = s.lo[i] = -1;
@@ -737,6 +814,53 @@ bool rgf_small_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, cons
     for (int i = 0; i + 1 < N; ++i) {
       const double b = la->block_sizes[i + 1];
       fl += 23.0 * 8.0 * b * b * b;
+    }
+    timer->end(st, ev0, KernelTimer::GEMM, fl * (double)S);
+  }
+  return true;
+}
+
+// Fused path of rgf_selected_inv for small blocks (same eligibility as the
+// fused solve); returns false, having done nothing, when not eligible.
+bool rgf_small_inv(qx_handle* h, hipStream_t st, const qx_btd_layout* la, const void* a,
+                   const void* const* obc_r, const qx_btd_layout* lx, void* x, int prec) {
+  const char* env = std::getenv("QX_RGF_FUSED");
+  if (env && std::atoi(env) == 0) return false;
+  if (prec != QX_PREC_C128 && prec != QX_PREC_C128_3M) return false;
+  const int N = la->n_blocks;
+  if (N < 1 || N > kSN) return false;
+  for (int i = 0; i < N; ++i)
+    if (la->block_sizes[i] < 1 || la->block_sizes[i] > kSB) return false;
+  if (!fits32(la, N) || !fits32(lx, N)) return false;
+  const int64_t S = la->n_stack;
+  if (S <= 0) return true;
+  QX_REQUIRE(S < (int64_t(1) << 31), "selected_inv: stack too large");
+  SmallInvArgs g;
+  g.N = N;
+  g.NS = 2 * N + 3;
+  g.split = prec == QX_PREC_C128_3M ? 3 : 4;
+  for (int i = 0; i < kSN; ++i) {
+    g.bs[i] = i < N ? la->block_sizes[i] : 0;
+    g.obc[i] = (obc_r && i < N) ? static_cast<const z2*>(obc_r[i]) : nullptr;
+  }
+  g.sA = la->stack_stride;
+  g.sX = lx->stack_stride;
+  g.A = static_cast<const z2*>(a);
+  g.X = static_cast<z2*>(x);
+  g.lA = make_lay(la, N);
+  g.lX = make_lay(lx, N);
+  g.ws = static_cast<z2*>(h->ensure_ws((size_t)S * g.NS * kSB * kSB * sizeof(z2)));
+  g.info = h->ensure_info(S);
+  KernelTimer* timer = active_timer;
+  hipEvent_t ev0 = nullptr;
+  if (timer) timer->begin(st, &ev0);
+  hipLaunchKernelGGL(rgf_small_inv_kernel, dim3((unsigned)S), dim3(kThreads), 0, st, g);
+  QX_HIP_CHECK(hipGetLastError());
+  if (timer) {
+    double fl = 0.0;
+    for (int i = 0; i + 1 < N; ++i) {
+      const double b = la->block_sizes[i + 1];
+      fl += 7.0 * 8.0 * b * b * b;
     }
     timer->end(st, ev0, KernelTimer::GEMM, fl * (double)S);
   }
