@@ -13,11 +13,13 @@
 //   output and load their MFMA fragments straight from memory, all k-steps
 //   issued before the first v_mfma_f64_16x16x4_f64.  4M or 3M split, summed
 //   over k in the same order as the batched kernels.
-// - Block inverse (sinv): one wave, two lanes per row (16 columns each),
-//   Gauss-Jordan with partial pivoting: the pivot is a DPP wave maximum of
-//   32-bit keys (float magnitude with the row index in the low bits, as in
-//   inv.hip), rows p and k are exchanged through LDS, and the column
-//   interchanges are undone at the end.  No barrier inside the wave.
+// - Block inverse (sinv_wg): the four waves, eight lanes per row (four
+//   columns each), Gauss-Jordan with partial pivoting and one barrier per
+//   column: the pivot is a DPP wave maximum of 32-bit keys (float magnitude
+//   with the row index in the low bits, as in inv.hip) over each wave's rows,
+//   then the largest of the four published keys; the candidate rows and row k
+//   go through double-buffered LDS; the column interchanges are undone at the
+//   end.  (sinv_wave is the one-wave variant, kept for QX_INV_SMALL=1.)
 // - Elementwise steps (folds, X - X^H parts, traces) as workgroup loops.
 // - A barrier closes every step, which makes its global writes visible to
 //   the other waves of the workgroup.
@@ -113,8 +115,16 @@ struct WaveShared {
   z2 tile[kSB][kSB + 1];
 };
 
+struct WgInvShared {
+  z2 cand[2][kThreads / 64][kSB];
+  z2 krow[2][kSB];
+  unsigned keys[2][kThreads / 64];
+  int perm[kSB], idx[kSB];
+  z2 tile[kSB][kSB + 1];
+};
+
 struct Shared {
-  WaveShared inv;
+  WgInvShared winv;
   double red[2][kThreads / 64];
 };
 
@@ -406,10 +416,158 @@ __device__ __forceinline__ void sinv_wave(int n, const z2* A, int lda, const z2*
   }
 }
 
-// In-place inverse of the block X by wave 0 of the workgroup.
-__device__ __forceinline__ void sinv(int n, M X, int* info, Shared& sh) {
-  if (threadIdx.x < 64) sinv_wave(n, X.p, X.ld, nullptr, 0, X.p, X.ld, info, sh.inv);
+// X = inv(A - S) for one n x n matrix (n <= 32) by the whole workgroup
+// (four waves, eight lanes per row, four columns per lane): Gauss-Jordan with
+// partial pivoting and one barrier per column.  Before the barrier each wave
+// publishes its best candidate row (DPP maximum of the pivot keys over its
+// eight rows) and the lanes of row k publish row k; after it every thread
+// takes the largest of the four keys, reads the pivot row and computes the
+// reciprocal itself (one wave per SIMD, so nothing is serialised).  The
+// buffers alternate between columns: no wave can write a buffer before every
+// wave has passed the barrier that follows its last read.
+
+
+__device__ __forceinline__ void sinv_wg(int n, const z2* A, int lda, const z2* S, int lds,
+                                        z2* X, int ldx, int* info, WgInvShared& sh) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r = tid >> 3, q = tid & 7;  // row, quarter-of-eight within the row
+  double xr[4], xi[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int col = 4 * q + c;
+    z2 v = make_double2(0.0, 0.0);
+    if (r < n && col < n) {
+      v = A[r * lda + col];
+      if (S) {
+        const z2 u = S[r * lds + col];
+        v.x -= u.x;
+        v.y -= u.y;
+      }
+    }
+    xr[c] = v.x;
+    xi[c] = v.y;
+  }
+#pragma unroll
+  for (int k = 0; k < kSB; ++k) {
+    if (k < n) {
+      const int kq = k >> 2, kc = k & 3, b = k & 1;
+      unsigned key = 0;
+      if (q == kq && r >= k && r < n) key = s_pivot_key(xr[kc], xi[kc], r);
+      const unsigned wkey = s_wave_max(key);
+      if (wkey != 0u && r == 511 - (int)(wkey & 0x1FFu)) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sh.cand[b][w][4 * q + c] = make_double2(xr[c], xi[c]);
+      }
+      if (lane == 0) sh.keys[b][w] = wkey;
+      if (r == k) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sh.krow[b][4 * q + c] = make_double2(xr[c], xi[c]);
+      }
+      __syncthreads();
+      unsigned kmax = sh.keys[b][0];
+      int ws = 0;
+#pragma unroll
+      for (int u = 1; u < kThreads / 64; ++u) {
+        const unsigned v = sh.keys[b][u];
+        if (v > kmax) {
+          kmax = v;
+          ws = u;
+        }
+      }
+      int p = 511 - (int)(kmax & 0x1FFu);
+      if (p < k || p >= n) p = k;
+      if (tid == 0) {
+        sh.perm[k] = p;
+        if ((kmax >> 9) == 0u) *info = 1;  // no nonzero pivot: singular
+      }
+      const z2 x = sh.cand[b][ws][k];
+      double vr, vi;
+      s_crecip(x.x, x.y, vr, vi);
+      double pr[4], pi[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int col = 4 * q + c;
+        if (col == k) {
+          pr[c] = vr;
+          pi[c] = vi;
+        } else {
+          const z2 v = sh.cand[b][ws][col];
+          pr[c] = v.x * vr - v.y * vi;
+          pi[c] = v.x * vi + v.y * vr;
+        }
+      }
+      if (r == k) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          xr[c] = pr[c];
+          xi[c] = pi[c];
+        }
+      } else if (r == p) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const z2 v = sh.krow[b][4 * q + c];
+          xr[c] = v.x;
+          xi[c] = v.y;
+        }
+      }
+      // multiplier: the row's (swapped-in) value in column k, from the lane
+      // of this row that holds column k
+      const int src = (lane & ~7) + kq;
+      const double fr = __shfl(xr[kc], src, 64), fi = __shfl(xi[kc], src, 64);
+      if (r < n && r != k) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int col = 4 * q + c;
+          if (col == k) {
+            xr[c] = -(fr * pr[c] - fi * pi[c]);
+            xi[c] = -(fr * pi[c] + fi * pr[c]);
+          } else {
+            xr[c] -= fr * pr[c] - fi * pi[c];
+            xi[c] -= fr * pi[c] + fi * pr[c];
+          }
+        }
+      }
+    }
+  }
+  // undo the column interchanges: reverse order of the pivots
+  if (tid == 0) {
+    for (int c = 0; c < n; ++c) sh.idx[c] = c;
+    for (int k = n - 1; k >= 0; --k) {
+      const int pq = sh.perm[k];
+      const int t = sh.idx[k];
+      sh.idx[k] = sh.idx[pq];
+      sh.idx[pq] = t;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int col = 4 * q + c;
+    if (r < n && col < n) sh.tile[r][col] = make_double2(xr[c], xi[c]);
+  }
   __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int col = 4 * q + c;
+    if (r < n && col < n) X[r * ldx + col] = sh.tile[r][sh.idx[col]];
+  }
+  __syncthreads();
+}
+
+
+
+// In-place inverse of the block X by the workgroup.
+__device__ __forceinline__ void sinv(int n, M X, int* info, Shared& sh) {
+  sinv_wg(n, X.p, X.ld, nullptr, 0, X.p, X.ld, info, sh.winv);
+}
+
+// Batched X_e = inv(A_e - S_e), n <= 32: one matrix per workgroup.
+__global__ __launch_bounds__(kThreads) void inv_small_wg_kernel(int n, const z2* A, int64_t sA,
+                                                                int lda, const z2* S, int64_t sS,
+                                                                int lds, z2* X, int64_t sX,
+                                                                int ldx, int* info) {
+  __shared__ WgInvShared sh;
+  const int64_t e = blockIdx.x;
+  sinv_wg(n, A + e * sA, lda, S ? S + e * sS : nullptr, lds, X + e * sX, ldx, info + e, sh);
 }
 
 // Batched X_e = inv(A_e - S_e), n <= 32: one matrix per wave, four per
@@ -720,23 +878,28 @@ bool fits32(const qx_btd_layout* l, int N) {
 
 }  // namespace
 
-// Batched inverse X = inv(A - S) of small blocks in one launch (complex128
-// arithmetic): used for n <= 16, and for n <= 32 from 1024 matrices on (one
-// wave per matrix wins once the stack fills the chip; at n = 32 and 256
-// matrices the split panel kernels are faster, 48 vs 68 us).  Returns false
-// otherwise, or when QX_INV_SMALL=0 (QX_INV_SMALL=1 forces it for n <= 32).
+// Batched inverse X = inv(A - S) of blocks n <= 32 in one launch (complex128
+// arithmetic), one matrix per workgroup: measured at n = 32 it beats both the
+// split panel kernels and one matrix per wave from 16 to 4096 matrices (37 vs
+// 49 / 69 us at 256, 207 vs 623 / 296 us at 4096).  QX_INV_SMALL=0 sends every
+// size to the split panel kernels, 1 to the per-wave kernel.
 bool inv_small(hipStream_t st, int64_t batch, int n, const Mat& A, const Mat& S, const Mat& X,
                int* info) {
   const char* env = std::getenv("QX_INV_SMALL");
   const int mode = env ? std::atoi(env) : -1;
   if (mode == 0) return false;
   if (n < 1 || n > kSB) return false;
-  if (mode != 1 && n > 16 && batch < 1024) return false;
   if (batch <= 0) return true;
-  const int64_t blocks = (batch + kThreads / 64 - 1) / (kThreads / 64);
-  QX_REQUIRE(blocks < (int64_t(1) << 31), "inv: batch too large");
-  hipLaunchKernelGGL(inv_small_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, st, batch, n,
-                     A.p, A.stride, A.ld, S.p, S.stride, S.ld, X.p, X.stride, X.ld, info);
+  QX_REQUIRE(batch < (int64_t(1) << 31), "inv: batch too large");
+  const bool per_wave = mode == 1;
+  if (per_wave) {
+    const int64_t blocks = (batch + kThreads / 64 - 1) / (kThreads / 64);
+    hipLaunchKernelGGL(inv_small_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, st, batch,
+                       n, A.p, A.stride, A.ld, S.p, S.stride, S.ld, X.p, X.stride, X.ld, info);
+  } else {
+    hipLaunchKernelGGL(inv_small_wg_kernel, dim3((unsigned)batch), dim3(kThreads), 0, st, n,
+                       A.p, A.stride, A.ld, S.p, S.stride, S.ld, X.p, X.stride, X.ld, info);
+  }
   QX_HIP_CHECK(hipGetLastError());
   return true;
 }

@@ -174,23 +174,25 @@ def test_inverse(device, n, prec):
 @pytest.mark.parametrize("n", [1, 3, 20, 32])
 @pytest.mark.parametrize("batch", [1, 5, 9])
 def test_small_inverse_matches_split_kernels(device, monkeypatch, n, batch):
-    """n <= 32 (complex128): one wave per matrix in one launch; same pivot
-    rule as the panel kernels, so the same permutation and results to
-    rounding (batch not a multiple of the four matrices per workgroup)."""
+    """n <= 32 (complex128): one launch, one matrix per wave or per
+    workgroup; same pivot rule as the panel kernels, so the same permutation
+    and results to rounding (batch not a multiple of the four matrices per
+    workgroup of the per-wave kernel)."""
     rng = np.random.default_rng(10 * n + batch)
     a = _cplx(rng, (batch, n, n))
     s = 0.1 * _cplx(rng, (batch, n, n))
     A, S = torch.as_tensor(a, device=device), torch.as_tensor(s, device=device)
     outs = []
-    for small in ("1", "0"):
+    for small in ("1", "2", "0"):  # per wave, per workgroup, split panel kernels
         monkeypatch.setenv("QX_INV_SMALL", small)
         X = torch.empty_like(A)
         _inv(device, A, X, 0, S)
         outs.append(X.cpu().numpy())
     cond = max(np.linalg.cond(a[e] - s[e]) for e in range(batch))
     for e in range(batch):
-        assert rel_err(outs[0][e], outs[1][e]) < 1e-15 * cond * max(n, 4)
-        assert rel_err(outs[0][e], np.linalg.inv(a[e] - s[e])) < 1e-15 * cond * max(n, 4)
+        for o in outs[:2]:
+            assert rel_err(o[e], outs[2][e]) < 1e-15 * cond * max(n, 4)
+            assert rel_err(o[e], np.linalg.inv(a[e] - s[e])) < 1e-15 * cond * max(n, 4)
 
 
 def test_inverse_in_place(device):
