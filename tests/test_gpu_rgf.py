@@ -318,3 +318,25 @@ def test_fused_small_block_path_matches_general(device, monkeypatch, precision):
         scale = np.max(np.abs(ref))
         assert scale > 0
         assert np.max(np.abs(got - ref)) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("block_sizes", [[7], [1, 1], [32, 5], [1, 32, 1], [16] * 16,
+                                         [32] * 3 + [1]],
+                         ids=["N1", "b1x2", "32-5", "1-32-1", "16x16", "32x3-1"])
+def test_edge_block_structures(device, block_sizes):
+    """Single blocks, 1 x 1 blocks, the largest fused block size and block
+    count, against the oracle (selected_inv and selected_solve with the
+    retarded output), on both solve paths."""
+    bs = np.array(block_sizes)
+    dense = bt_dense(bs, seed=len(bs) + int(bs.sum()))
+    coo = sps.coo_matrix(dense)
+    b = coo - coo.conj().T
+    oa = ODSDBCOO.from_sparray(coo, bs, (3,))
+    ob = ODSDBCOO.from_sparray(sps.coo_matrix(b), bs, (3,))
+    ref_inv = orgf.RGF().selected_inv(oa)
+    refs = orgf.RGF().selected_solve(oa, ob, ob, return_retarded=True)
+    a, bl = to_gpu(oa, device), to_gpu(ob, device)
+    assert close(RGF().selected_inv(a), ref_inv, 1e-10)
+    got = RGF().selected_solve(a, bl, bl, return_retarded=True)
+    for g, o in zip(got, refs):
+        assert close(g, o, 1e-10)
