@@ -62,25 +62,6 @@ struct InvMma<float> {
   static __device__ inline int row(int lane, int e) { return 4 * (lane >> 4) + e; }
 };
 
-__device__ inline void crecip(double ar, double ai, double& qr, double& qi) {
-  // 1 / (ar + i ai), Smith's scaling against overflow.
-  if (fabs(ar) >= fabs(ai)) {
-    const double r = ai / ar, d = ar + ai * r;
-    qr = 1.0 / d;
-    qi = -r / d;
-  } else {
-    const double r = ar / ai, d = ai + ar * r;
-    qr = r / d;
-    qi = -1.0 / d;
-  }
-}
-
-// Pivot key: larger key = larger magnitude, ties -> smaller row (n <= 512).
-__device__ inline unsigned pivot_key(double re, double im, int r) {
-  const float m = (float)(fabs(re) + fabs(im));
-  return (__float_as_uint(m) & 0xFFFFFE00u) | (unsigned)(511 - r);
-}
-
 __device__ inline unsigned wave_max(unsigned v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -123,7 +104,7 @@ __device__ inline double quad_bcast(double x, int qq) {
 }
 
 struct PanelArgs {
-  int n, k0, nbw, last;
+  int n, k0, nbw;
   const z2* Xc;
   int64_t sXc;
   int ldc;
@@ -135,6 +116,37 @@ struct PanelArgs {
   int* info;
 };
 
+// Row and column gathers of a finished panel.  Each position is traced
+// through the panel's transpositions S_kk = (k0 + kk <-> piv[kk]) in
+// registers, so every thread resolves its own positions (the swaps applied
+// one after another to an array would be a serial chain of dependent LDS
+// round trips on one thread).
+//   sig[u]: row gather of this panel, the identity after the swaps
+//           S_0, ..., S_{nbw-1} in that order: S_0(S_1(... S_{nbw-1}(u)));
+//   idx[u]: column gather of the whole inverse, the identity after the
+//           swaps of all panels in reverse order, i.e. u pushed forward
+//           through every swap of the factorisation: each panel extends the
+//           previous panel's idx by its own swaps, so the last one holds the
+//           complete gather and no panel traces more than NB steps.
+template <int NTH>
+__device__ __forceinline__ void panel_gathers(const int* piv, int n, int k0, int nbw, int tid,
+                                              int* perm, int* sig, int* idx) {
+  for (int u = tid; u < n; u += NTH) {
+    int pc = (k0 == 0) ? u : idx[u];
+    int pr = u;
+#pragma unroll 8
+    for (int kk = 0; kk < nbw; ++kk) {
+      const int k = k0 + kk, p = piv[kk];
+      pc = (pc == k) ? p : ((pc == p) ? k : pc);
+      const int k2 = k0 + nbw - 1 - kk, p2 = piv[nbw - 1 - kk];
+      pr = (pr == k2) ? p2 : ((pr == p2) ? k2 : pr);
+    }
+    sig[u] = pr;
+    idx[u] = pc;
+  }
+  for (int u = tid; u < nbw; u += NTH) perm[k0 + u] = piv[u];
+}
+
 // Each thread keeps its row(s) of the panel in registers, rotated so that
 // the column being eliminated is always element 0: at step kk the row holds
 // columns (kk, kk+1, ..., NB-1, 0, ..., kk-1).  All register indices are then
@@ -143,8 +155,7 @@ template <typename T, int NB, int RPT>
 __global__ __launch_bounds__(NT) void inv_panel(PanelArgs g) {
   __shared__ T prow_r[NB], prow_i[NB], krow_r[NB], krow_i[NB];
   __shared__ unsigned red[NT / 64];
-  __shared__ int perm_l[kInvMaxN];
-  __shared__ int sig_l[kInvMaxN];
+  __shared__ int perm_l[NB];
   const int64_t e = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int n = g.n, k0 = g.k0, nbw = g.nbw;
@@ -192,7 +203,7 @@ __global__ __launch_bounds__(NT) void inv_panel(PanelArgs g) {
       const int r = tid + NT * j;
       if (r == p) {
         double vr, vi;
-        crecip((double)rr[j][0], (double)ri[j][0], vr, vi);
+        complex_recip((double)rr[j][0], (double)ri[j][0], vr, vi);
         prow_r[0] = (T)vr;
         prow_i[0] = (T)vi;
 #pragma unroll
@@ -284,38 +295,9 @@ __global__ __launch_bounds__(NT) void inv_panel(PanelArgs g) {
       }
     }
   }
-  // composed row gather of this panel, pivot record
-  for (int r = tid; r < n; r += NT) sig_l[r] = r;
-  if (g.last)
-    for (int r = tid; r < k0; r += NT) perm_l[NB + r] = perm[r];  // earlier panels
-  __syncthreads();
-  if (tid == 0) {
-    for (int kk = 0; kk < nbw; ++kk) {
-      const int q = perm_l[kk];
-      const int t = sig_l[k0 + kk];
-      sig_l[k0 + kk] = sig_l[q];
-      sig_l[q] = t;
-    }
-  }
-  __syncthreads();
-  for (int r = tid; r < n; r += NT) sig[r] = sig_l[r];
-  for (int kk = tid; kk < nbw; kk += NT) perm[k0 + kk] = perm_l[kk];
-  if (g.last) {
-    // column gather of the whole inverse: reverse order of all pivots
-    __syncthreads();
-    for (int r = tid; r < n; r += NT) sig_l[r] = r;
-    __syncthreads();
-    if (tid == 0) {
-      for (int k = n - 1; k >= 0; --k) {
-        const int q = (k >= k0) ? perm_l[k - k0] : perm_l[NB + k];
-        const int t = sig_l[k];
-        sig_l[k] = sig_l[q];
-        sig_l[q] = t;
-      }
-    }
-    __syncthreads();
-    for (int r = tid; r < n; r += NT) idx[r] = sig_l[r];
-  }
+  // pivot record and gathers (this panel's pivots are in perm_l since the
+  // column loop's last barrier)
+  panel_gathers<NT>(perm_l, n, k0, nbw, tid, perm, sig, idx);
 }
 
 // Panel kernel for n <= 256 with four threads per row (1024 threads): thread
@@ -333,8 +315,7 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
   constexpr int NB = 32, CQ = 8;
   __shared__ T prow_r[NB], prow_i[NB], krow_r[NB], krow_i[NB];
   __shared__ unsigned red[NT4 / 64];
-  __shared__ int perm_l[kInvMaxN];
-  __shared__ int sig_l[kInvMaxN];
+  __shared__ int perm_l[NB];
   const int64_t e = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int r = tid >> 2, q = tid & 3;
@@ -361,7 +342,11 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
   if (lane == 0) red[w] = key;
   __syncthreads();
 
-#pragma unroll
+  // The quarters run as a loop and only the eight columns of a quarter are
+  // unrolled: the fully unrolled 32 columns are ~78 KB of code, more than
+  // the instruction cache, and every column then waits on instruction
+  // fetches from L2.
+#pragma unroll 1
   for (int qq = 0; qq < 4; ++qq) {
 #pragma unroll
     for (int c = 0; c < CQ; ++c) {
@@ -378,7 +363,7 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
         const double pi = quad_bcast((double)ri[c], qq);
         if (r == p) {
           double vr, vi;
-          crecip(pr, pi, vr, vi);
+          complex_recip(pr, pi, vr, vi);
 #pragma unroll
           for (int cc = 0; cc < CQ; ++cc) {
             if (q == qq && cc == c) {
@@ -438,10 +423,9 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
         // pivot candidates of the next column
         key = 0;
         const int kn = kk + 1;
-        if (kn < nbw && r > k && r < n && q == kn / CQ) {
-          const int cn = kn % CQ;  // compile-time after unrolling
+        const int cn = (c + 1) % CQ;  // register of column kn (constant once unrolled)
+        if (kn < nbw && r > k && r < n && q == qq + (c + 1) / CQ)
           key = pivot_key((double)rr[cn], (double)ri[cn], r);
-        }
         key = wave_max_dpp(key);
         if (lane == 0) red[w] = key;
         __syncthreads();
@@ -457,38 +441,9 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
         Xn[(int64_t)r * g.ldn + k0 + col] = make_double2((double)rr[c], (double)ri[c]);
     }
   }
-  // composed row gather of this panel, pivot record
-  for (int u = tid; u < n; u += NT4) sig_l[u] = u;
-  if (g.last)
-    for (int u = tid; u < k0; u += NT4) perm_l[NB + u] = perm[u];  // earlier panels
-  __syncthreads();
-  if (tid == 0) {
-    for (int kk = 0; kk < nbw; ++kk) {
-      const int pq = perm_l[kk];
-      const int t = sig_l[k0 + kk];
-      sig_l[k0 + kk] = sig_l[pq];
-      sig_l[pq] = t;
-    }
-  }
-  __syncthreads();
-  for (int u = tid; u < n; u += NT4) sig[u] = sig_l[u];
-  for (int u = tid; u < nbw; u += NT4) perm[k0 + u] = perm_l[u];
-  if (g.last) {
-    // column gather of the whole inverse: reverse order of all pivots
-    __syncthreads();
-    for (int u = tid; u < n; u += NT4) sig_l[u] = u;
-    __syncthreads();
-    if (tid == 0) {
-      for (int k = n - 1; k >= 0; --k) {
-        const int pq = (k >= k0) ? perm_l[k - k0] : perm_l[NB + k];
-        const int t = sig_l[k];
-        sig_l[k] = sig_l[pq];
-        sig_l[pq] = t;
-      }
-    }
-    __syncthreads();
-    for (int u = tid; u < n; u += NT4) idx[u] = sig_l[u];
-  }
+  // pivot record and gathers (this panel's pivots are in perm_l since the
+  // column loop's last barrier)
+  panel_gathers<NT4>(perm_l, n, k0, nbw, tid, perm, sig, idx);
 }
 
 struct UpdArgs {
@@ -633,7 +588,7 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
   const int tn = (n + 63) / 64;
   for (int p = 0; p < npan; ++p) {
     const int k0 = p * NB, nbw = min(NB, n - k0), nxt = 1 - cur;
-    PanelArgs pa{n, k0, nbw, p == npan - 1 ? 1 : 0, B[cur].p, B[cur].stride, B[cur].ld,
+    PanelArgs pa{n, k0, nbw, B[cur].p, B[cur].stride, B[cur].ld,
                  B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, info};
     if (NB == 32 && RPT == 1 && use_panel4())
       hipLaunchKernelGGL((inv_panel4<T>), dim3((unsigned)batch), dim3(NT4), 0, s, pa);

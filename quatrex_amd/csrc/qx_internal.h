@@ -33,6 +33,38 @@ struct Cplx {
   double re, im;
 };
 
+// Pivot key of the batched inverses: a larger key means a larger |re| + |im|,
+// ties go to the smaller row r (r < 512 in the low 9 bits).  The magnitude
+// enters through the upper word of its double (sign 0, 11 exponent bits, the
+// top mantissa bits), which orders non-negative doubles over the whole range;
+// after the row bits, 11 mantissa bits remain, so candidates within a
+// relative 2^-11 of each other count as equal.  A key below 512 (magnitude
+// zero or below ~2^-1033) marks a zero pivot.
+__device__ __forceinline__ unsigned pivot_key(double re, double im, int r) {
+  const double m = fabs(re) + fabs(im);
+  return ((unsigned)__double2hiint(m) & 0xFFFFFE00u) | (unsigned)(511 - r);
+}
+
+// 1 / (ar + i ai) = (ar - i ai) / (ar^2 + ai^2) for the pivots of the
+// batched inverses.  Both parts are first scaled by the same power of two
+// (exactly) so that the larger magnitude lies in [0.5, 1): the sum of squares
+// d is then in [0.25, 2) and nothing overflows or underflows before the final
+// rescale.  1 / d is the hardware reciprocal estimate refined by two Newton
+// steps y <- y + y (1 - d y), each of which squares the relative error, so y
+// is within a few ulp of 1 / d.  The dependent chain is ~10 operations
+// against ~25 for three IEEE divisions (Smith's algorithm).  A zero pivot
+// gives inf / nan, as a division by zero would; callers flag it separately.
+__device__ __forceinline__ void complex_recip(double ar, double ai, double& qr, double& qi) {
+  const int e = __builtin_amdgcn_frexp_exp(fmax(fabs(ar), fabs(ai)));
+  const double sr = __builtin_amdgcn_ldexp(ar, -e), si = __builtin_amdgcn_ldexp(ai, -e);
+  const double d = fma(sr, sr, si * si);
+  double y = __builtin_amdgcn_rcp(d);
+  y = fma(y, fma(-d, y, 1.0), y);
+  y = fma(y, fma(-d, y, 1.0), y);
+  qr = __builtin_amdgcn_ldexp(sr * y, -e);
+  qi = __builtin_amdgcn_ldexp(-si * y, -e);
+}
+
 // Status carried up to the C ABI.
 struct Error {
   int code;

@@ -68,12 +68,7 @@ __device__ inline M blk(const z2* base, int64_t stride, int64_t e, int32_t off, 
   return M{const_cast<z2*>(base) + e * stride + off, ld};
 }
 
-// Pivot key and wave maximum as in inv.hip.
-__device__ inline unsigned s_pivot_key(double re, double im, int r) {
-  const float m = (float)(fabs(re) + fabs(im));
-  return (__float_as_uint(m) & 0xFFFFFE00u) | (unsigned)(511 - r);
-}
-
+// Wave maximum of pivot keys (DPP), as in inv.hip.
 __device__ inline unsigned s_wave_max(unsigned v) {
   v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, false));
   v = max(v, (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xf, 0xf, false));
@@ -93,19 +88,6 @@ __device__ inline double s_pair_bcast(double x) {
   const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), sel, 0xf, 0xf, false);
   const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), sel, 0xf, 0xf, false);
   return __hiloint2double(hi, lo);
-}
-
-__device__ inline void s_crecip(double ar, double ai, double& qr, double& qi) {
-  // 1 / (ar + i ai), Smith's scaling (as inv.hip)
-  if (fabs(ar) >= fabs(ai)) {
-    const double r = ai / ar, d = ar + ai * r;
-    qr = 1.0 / d;
-    qi = -r / d;
-  } else {
-    const double r = ar / ai, d = ai + ar * r;
-    qr = r / d;
-    qi = -1.0 / d;
-  }
 }
 
 // LDS of one wave's inverse.
@@ -325,7 +307,7 @@ __device__ __forceinline__ void sinv_wave(int n, const z2* A, int lda, const z2*
       if (k < n) {
         const int hk = k >> 4, ck = k & 15;
         unsigned key = 0;
-        if (h == hk && r >= k && r < n) key = s_pivot_key(xr[ck], xi[ck], r);
+        if (h == hk && r >= k && r < n) key = pivot_key(xr[ck], xi[ck], r);
         const unsigned kmax = s_wave_max(key);
         int p = 511 - (int)(kmax & 0x1FFu);
         if (p < k || p >= n) p = k;
@@ -345,7 +327,7 @@ __device__ __forceinline__ void sinv_wave(int n, const z2* A, int lda, const z2*
         __builtin_amdgcn_wave_barrier();
         const z2 x = sh.prow[k];
         double vr, vi;
-        s_crecip(x.x, x.y, vr, vi);
+        complex_recip(x.x, x.y, vr, vi);
         double pr[16], pi[16];  // this lane's half of the scaled pivot row
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
@@ -452,7 +434,7 @@ __device__ __forceinline__ void sinv_wg(int n, const z2* A, int lda, const z2* S
     if (k < n) {
       const int kq = k >> 2, kc = k & 3, b = k & 1;
       unsigned key = 0;
-      if (q == kq && r >= k && r < n) key = s_pivot_key(xr[kc], xi[kc], r);
+      if (q == kq && r >= k && r < n) key = pivot_key(xr[kc], xi[kc], r);
       const unsigned wkey = s_wave_max(key);
       if (wkey != 0u && r == 511 - (int)(wkey & 0x1FFu)) {
 #pragma unroll
@@ -482,7 +464,7 @@ __device__ __forceinline__ void sinv_wg(int n, const z2* A, int lda, const z2* S
       }
       const z2 x = sh.cand[b][ws][k];
       double vr, vi;
-      s_crecip(x.x, x.y, vr, vi);
+      complex_recip(x.x, x.y, vr, vi);
       double pr[4], pi[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {

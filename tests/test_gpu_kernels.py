@@ -195,6 +195,26 @@ def test_small_inverse_matches_split_kernels(device, monkeypatch, n, batch):
             assert rel_err(o[e], np.linalg.inv(a[e] - s[e])) < 1e-15 * cond * max(n, 4)
 
 
+@pytest.mark.parametrize("n", [12, 32, 100, 300])
+@pytest.mark.parametrize("scale", [2.0 ** -1000, 1e-150, 1e150, 2.0 ** 1000])
+def test_inverse_extreme_scales(device, n, scale):
+    """Pivot reciprocals are formed from parts scaled by a power of two:
+    matrices whose entries are near the ends of the double range invert as
+    accurately as unit-scale ones (inv(s A) = inv(A) / s), on the per-workgroup
+    small path (n <= 32) and both panel kernels (n <= 256, n > 256)."""
+    rng = np.random.default_rng(n)
+    a = _cplx(rng, (2, n, n))
+    a[:, 0, 0] = 0.0
+    A = torch.as_tensor(a * scale, device=device)
+    X = torch.empty_like(A)
+    _inv(device, A, X, 0)
+    got = X.cpu().numpy() * scale
+    for e in range(2):
+        ref = np.linalg.inv(a[e])
+        assert np.isfinite(got[e]).all()
+        assert rel_err(got[e], ref) < 1e-15 * np.linalg.cond(a[e]) * n, (e, rel_err(got[e], ref))
+
+
 def test_inverse_in_place(device):
     rng = np.random.default_rng(5)
     a = _cplx(rng, (3, 40, 40))
