@@ -279,12 +279,13 @@ void zgemm_kernel(GemmArgs g) {
             vr = (double)acc1[i][j][q];
             vi = (double)acc2[i][j][q];
           }
-          double outr = g.ar * vr - g.ai * vi, outi = g.ar * vi + g.ai * vr;
+          // explicit fma: the same rounding whatever the compiler contracts
+          double outr = fma(g.ar, vr, -(g.ai * vi)), outi = fma(g.ar, vi, g.ai * vr);
           z2* dst = C + (int64_t)row * g.ldc + col;
           if (!g.beta_zero) {
             const z2 c = Ci[(int64_t)row * g.ldcin + col];
-            outr += g.br * c.x - g.bi * c.y;
-            outi += g.br * c.y + g.bi * c.x;
+            outr += fma(g.br, c.x, -(g.bi * c.y));
+            outi += fma(g.br, c.y, g.bi * c.x);
           }
           *dst = make_double2(outr, outi);
           if (g.Ct) g.Ct[e * g.sCt + (int64_t)col * g.ldct + row] = make_double2(-outr, outi);
@@ -314,8 +315,9 @@ void zgemm_kernel(GemmArgs g) {
 // rest, with both operands zeroed past k.  No value is selected on the
 // pipelined path, so its loads are never waited for early.
 //
-// The arithmetic per output element (order of the k sum, 4M / 3M split) is
-// the same as zgemm_kernel's, so both kernels give bit-identical results.
+// The arithmetic per output element (order of the k sum, 4M / 3M split, the
+// epilogue's explicit fma) is the same as zgemm_kernel's, so both kernels
+// give bit-identical results.
 template <int OPA, int OPB, int SPLIT, int TM, int TN, int D, bool CHECK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void zgemm_reg_kernel(GemmArgs g) {
@@ -485,11 +487,12 @@ void zgemm_reg_kernel(GemmArgs g) {
             vr = acc1[i][j][q];
             vi = acc2[i][j][q];
           }
-          double outr = g.ar * vr - g.ai * vi, outi = g.ar * vi + g.ai * vr;
+          // explicit fma, as in zgemm_kernel
+          double outr = fma(g.ar, vr, -(g.ai * vi)), outi = fma(g.ar, vi, g.ai * vr);
           if (!g.beta_zero) {
             const z2 c = Ci[(int64_t)row * g.ldcin + col];
-            outr += g.br * c.x - g.bi * c.y;
-            outi += g.br * c.y + g.bi * c.x;
+            outr += fma(g.br, c.x, -(g.bi * c.y));
+            outi += fma(g.br, c.y, g.bi * c.x);
           }
           C[(int64_t)row * g.ldc + col] = make_double2(outr, outi);
           if (g.Ct) g.Ct[e * g.sCt + (int64_t)col * g.ldct + row] = make_double2(-outr, outi);
