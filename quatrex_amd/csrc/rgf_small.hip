@@ -429,102 +429,111 @@ __device__ __forceinline__ void sinv_wg(int n, const z2* A, int lda, const z2* S
     xr[c] = v.x;
     xi[c] = v.y;
   }
+  // Groups of four columns run as a loop, the four columns of a group are
+  // unrolled (register indices stay constants): the fully unrolled 32
+  // columns are ~64 KB of code per call site, beyond the instruction cache.
+#pragma unroll 1
+  for (int kq = 0; kq < kSB / 4; ++kq) {
 #pragma unroll
-  for (int k = 0; k < kSB; ++k) {
-    if (k < n) {
-      const int kq = k >> 2, kc = k & 3, b = k & 1;
-      unsigned key = 0;
-      if (q == kq && r >= k && r < n) key = pivot_key(xr[kc], xi[kc], r);
-      const unsigned wkey = s_wave_max(key);
-      if (wkey != 0u && r == 511 - (int)(wkey & 0x1FFu)) {
+    for (int kc = 0; kc < 4; ++kc) {
+      const int k = 4 * kq + kc, b = kc & 1;
+      if (k < n) {
+        unsigned key = 0;
+        if (q == kq && r >= k && r < n) key = pivot_key(xr[kc], xi[kc], r);
+        const unsigned wkey = s_wave_max(key);
+        if (wkey != 0u && r == 511 - (int)(wkey & 0x1FFu)) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sh.cand[b][w][4 * q + c] = make_double2(xr[c], xi[c]);
-      }
-      if (lane == 0) sh.keys[b][w] = wkey;
-      if (r == k) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) sh.krow[b][4 * q + c] = make_double2(xr[c], xi[c]);
-      }
-      __syncthreads();
-      unsigned kmax = sh.keys[b][0];
-      int ws = 0;
-#pragma unroll
-      for (int u = 1; u < kThreads / 64; ++u) {
-        const unsigned v = sh.keys[b][u];
-        if (v > kmax) {
-          kmax = v;
-          ws = u;
+          for (int c = 0; c < 4; ++c) sh.cand[b][w][4 * q + c] = make_double2(xr[c], xi[c]);
         }
-      }
-      int p = 511 - (int)(kmax & 0x1FFu);
-      if (p < k || p >= n) p = k;
-      if (tid == 0) {
-        sh.perm[k] = p;
-        if ((kmax >> 9) == 0u) *info = 1;  // no nonzero pivot: singular
-      }
-      const z2 x = sh.cand[b][ws][k];
-      double vr, vi;
-      complex_recip(x.x, x.y, vr, vi);
-      double pr[4], pi[4];
+        if (lane == 0) sh.keys[b][w] = wkey;
+        if (r == k) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int col = 4 * q + c;
-        if (col == k) {
-          pr[c] = vr;
-          pi[c] = vi;
-        } else {
-          const z2 v = sh.cand[b][ws][col];
-          pr[c] = v.x * vr - v.y * vi;
-          pi[c] = v.x * vi + v.y * vr;
+          for (int c = 0; c < 4; ++c) sh.krow[b][4 * q + c] = make_double2(xr[c], xi[c]);
         }
-      }
-      if (r == k) {
+        __syncthreads();
+        unsigned kmax = sh.keys[b][0];
+        int ws = 0;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          xr[c] = pr[c];
-          xi[c] = pi[c];
+        for (int u = 1; u < kThreads / 64; ++u) {
+          const unsigned v = sh.keys[b][u];
+          if (v > kmax) {
+            kmax = v;
+            ws = u;
+          }
         }
-      } else if (r == p) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const z2 v = sh.krow[b][4 * q + c];
-          xr[c] = v.x;
-          xi[c] = v.y;
+        int p = 511 - (int)(kmax & 0x1FFu);
+        if (p < k || p >= n) p = k;
+        if (tid == 0) {
+          sh.perm[k] = p;
+          if ((kmax >> 9) == 0u) *info = 1;  // no nonzero pivot: singular
         }
-      }
-      // multiplier: the row's (swapped-in) value in column k, from the lane
-      // of this row that holds column k
-      const int src = (lane & ~7) + kq;
-      const double fr = __shfl(xr[kc], src, 64), fi = __shfl(xi[kc], src, 64);
-      if (r < n && r != k) {
+        const z2 x = sh.cand[b][ws][k];
+        double vr, vi;
+        complex_recip(x.x, x.y, vr, vi);
+        double pr[4], pi[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int col = 4 * q + c;
           if (col == k) {
-            xr[c] = -(fr * pr[c] - fi * pi[c]);
-            xi[c] = -(fr * pi[c] + fi * pr[c]);
+            pr[c] = vr;
+            pi[c] = vi;
           } else {
-            xr[c] -= fr * pr[c] - fi * pi[c];
-            xi[c] -= fr * pi[c] + fi * pr[c];
+            const z2 v = sh.cand[b][ws][col];
+            pr[c] = v.x * vr - v.y * vi;
+            pi[c] = v.x * vi + v.y * vr;
+          }
+        }
+        if (r == k) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            xr[c] = pr[c];
+            xi[c] = pi[c];
+          }
+        } else if (r == p) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const z2 v = sh.krow[b][4 * q + c];
+            xr[c] = v.x;
+            xi[c] = v.y;
+          }
+        }
+        // multiplier: the row's (swapped-in) value in column k, from the lane
+        // of this row that holds column k
+        const int src = (lane & ~7) + kq;
+        const double fr = __shfl(xr[kc], src, 64), fi = __shfl(xi[kc], src, 64);
+        if (r < n && r != k) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int col = 4 * q + c;
+            if (col == k) {
+              xr[c] = -(fr * pr[c] - fi * pi[c]);
+              xi[c] = -(fr * pi[c] + fi * pr[c]);
+            } else {
+              xr[c] -= fr * pr[c] - fi * pi[c];
+              xi[c] -= fr * pi[c] + fi * pr[c];
+            }
           }
         }
       }
-    }
-  }
-  // undo the column interchanges: reverse order of the pivots
-  if (tid == 0) {
-    for (int c = 0; c < n; ++c) sh.idx[c] = c;
-    for (int k = n - 1; k >= 0; --k) {
-      const int pq = sh.perm[k];
-      const int t = sh.idx[k];
-      sh.idx[k] = sh.idx[pq];
-      sh.idx[pq] = t;
     }
   }
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int col = 4 * q + c;
     if (r < n && col < n) sh.tile[r][col] = make_double2(xr[c], xi[c]);
+  }
+  __syncthreads();  // tile, and the pivot record written by thread 0
+  // Undo the column interchanges (the identity after the swaps (k, perm k)
+  // for k = n - 1 down to 0): position c is traced forward through the
+  // swaps for k = 0 .. n - 1, one position per thread.
+  if (tid < n) {
+    int pos = tid;
+#pragma unroll 8
+    for (int k = 0; k < n; ++k) {
+      const int p = sh.perm[k];
+      pos = (pos == k) ? p : ((pos == p) ? k : pos);
+    }
+    sh.idx[tid] = pos;
   }
   __syncthreads();
 #pragma unroll
