@@ -146,8 +146,10 @@ def measured_traffic(flops_per_launch: float):
     return None
 
 
-def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak):
-    """Warm-up + timed steps of one arithmetic mode; returns its record."""
+def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak, dominant_flops):
+    """Warm-up + timed steps of one arithmetic mode; returns its record.
+    dominant_flops: flops of one launch of the dominant GEMM shape (one
+    sub-batch of b x b x b products), the shape the traffic was measured at."""
     import numpy as np
     import torch
 
@@ -197,7 +199,7 @@ def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak):
         achieved = g["flops"] / (g["busy_ms"] / 1e3) / 1e12
         per_launch = g["flops"] / max(g["launches"], 1)
         avg_ms = g["ms"] / max(g["launches"], 1)
-        traffic = measured_traffic(per_launch) if rgf.precision == "c128" else None
+        traffic = measured_traffic(dominant_flops) if rgf.precision == "c128" else None
         rec["roofline"] = {
             "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
@@ -205,6 +207,7 @@ def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak):
             "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, calibrated "
                                "on a 1 GiB copy: profiles/zgemm_256_hbm_traffic.json")
             if traffic else None,
+            "traffic_launch_flops": dominant_flops if traffic else None,
             "flops_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
             "per_launch_tflops": round(per_launch / (avg_ms / 1e3) / 1e12, 3),
             "busy_ms": round(g["busy_ms"], 2)}
@@ -669,8 +672,10 @@ def main():
     for m in modes:
         rgf = RGF(max_batch_size=chunk, precision=m)
         peak = PEAK_TFLOPS[m]
+        # the RGF splits a call into 4 concurrent sub-batches (QX_RGF_STREAMS)
+        dominant = 8.0 * b ** 3 * ((chunk + 3) // 4)
         legs[m] = timed_leg(rgf, views, (xl, xg, xr), args.steps, args.warmup, dev,
-                            not args.no_timer, peak)
+                            not args.no_timer, peak, dominant)
         legs[m]["value"] = E * args.steps / legs[m]["elapsed"]
         log(f"{m}: {legs[m]['value']:.1f} energies/s")
     del xl, xg, xr
