@@ -71,10 +71,6 @@ struct GemmArgs {
   z2* Ct;  // optional second output Ct = -C^H (n x m), or null
   int64_t sCt;
   int ldct;
-  // 1: the result is skew-Hermitian (m == n, whole tiles, BM == BN): only
-  // tiles on or above the diagonal are computed, each off-diagonal one also
-  // written mirrored, C[tj][ti] = -C[ti][tj]^H (register-direct kernel only)
-  int tri;
 };
 
 // Hardware dispatches workgroup ids round-robin over the 8 XCDs.  Return a
@@ -329,33 +325,10 @@ void zgemm_reg_kernel(GemmArgs g) {
   // neg-modifier masks: s_a = -1 when op(A) = C (imag part negated), s_b ditto
   constexpr bool CA = OPA == QX_OP_C, CB = OPB == QX_OP_C;
   const int id = xcd_remap(blockIdx.x, gridDim.x);
-  const int tiles = g.tri ? g.tilesM * (g.tilesM + 1) / 2 : g.tilesM * g.tilesN;
+  const int tiles = g.tilesM * g.tilesN;
   const int64_t e = id / tiles;
   const int t = id - (int)(e * tiles);
-  int ti, tj;
-  if (g.tri) {
-    // tile t of the upper triangle, row by row
-    ti = 0;
-    int rem = t;
-    while (rem >= g.tilesM - ti) {
-      rem -= g.tilesM - ti;
-      ++ti;
-    }
-    tj = ti + rem;
-  } else {
-    ti = t / g.tilesN;
-    tj = t % g.tilesN;
-  }
-  const int m0 = ti * BM, n0 = tj * BN;
-  // second output: Ct = -C^H, or the mirrored tile of a skew-Hermitian result
-  z2* Ct = g.Ct;
-  int64_t sCt = g.sCt;
-  int ldct = g.ldct;
-  if (g.tri) {
-    Ct = (ti != tj) ? g.C : nullptr;
-    sCt = g.sC;
-    ldct = g.ldc;
-  }
+  const int m0 = (t / g.tilesN) * BM, n0 = (t % g.tilesN) * BN;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wm = w >> 1, wn = w & 1;
   const z2* __restrict__ A = g.A + e * g.sA;
@@ -522,7 +495,7 @@ void zgemm_reg_kernel(GemmArgs g) {
             outi += fma(g.br, c.y, g.bi * c.x);
           }
           C[(int64_t)row * g.ldc + col] = make_double2(outr, outi);
-          if (Ct) Ct[e * sCt + (int64_t)col * ldct + row] = make_double2(-outr, outi);
+          if (g.Ct) g.Ct[e * g.sCt + (int64_t)col * g.ldct + row] = make_double2(-outr, outi);
         }
       }
 }
@@ -571,11 +544,9 @@ void launch_reg(hipStream_t s, int64_t batch, int opA, int opB, GemmArgs g) {
   constexpr int BM = 32 * TM, BN = 32 * TN;
   g.tilesM = (g.m + BM - 1) / BM;
   g.tilesN = (g.n + BN - 1) / BN;
-  const int64_t per = g.tri ? (int64_t)g.tilesM * (g.tilesM + 1) / 2 : (int64_t)g.tilesM * g.tilesN;
-  const int64_t blocks = batch * per;
+  const int64_t blocks = batch * g.tilesM * g.tilesN;
   QX_REQUIRE(blocks < (int64_t(1) << 31), "zgemm: grid too large");
   const bool full = g.m % BM == 0 && g.n % BN == 0 && g.k % (4 * D) == 0;
-  QX_REQUIRE(!g.tri || (full && BM == BN && g.m == g.n), "zgemm: bad skew-Hermitian launch");
   dim3 grid((unsigned)blocks), block(256);
   KernelTimer* timer = active_timer;
   hipEvent_t ev0 = nullptr;
@@ -596,11 +567,7 @@ void launch_reg(hipStream_t s, int64_t batch, int opA, int opB, GemmArgs g) {
   }
 #undef QX_REG_LAUNCH
   QX_HIP_CHECK(hipGetLastError());
-  // a skew-Hermitian launch counts the flops of the tiles it computes
-  if (timer)
-    timer->end(s, ev0, KernelTimer::GEMM,
-               g.tri ? 8.0 * BM * BN * (double)g.k * (double)blocks
-                     : 8.0 * g.m * g.n * (double)g.k * batch);
+  if (timer) timer->end(s, ev0, KernelTimer::GEMM, 8.0 * g.m * g.n * (double)g.k * batch);
 }
 
 // Shapes of the register-direct kernel, as (TM, TN, D): workgroup tile
@@ -704,11 +671,9 @@ void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
   zgemm_in(s, batch, m, n, k, opA, opB, alpha, A, B, beta, C, C, precision, Mat());
 }
 
-namespace {
-
-void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
-                Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
-                const Mat& C, int precision, const Mat& Ct, bool skew) {
+void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+              Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
+              const Mat& C, int precision, const Mat& Ct) {
   QX_REQUIRE(!C.zero(), "zgemm: output is a zero matrix");
   QX_REQUIRE(Ct.zero() || (Ct.p != C.p && Ct.p != A.p && Ct.p != B.p && Ct.p != Cin.p),
              "zgemm: the transposed output aliases an operand");
@@ -749,17 +714,7 @@ void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int 
   g.sCin = Cin.zero() ? C.stride : Cin.stride;
   g.ldcin = Cin.zero() ? C.ld : Cin.ld;
   g.beta_zero = beta_zero ? 1 : 0;
-  g.tri = 0;
   const int split = gemm_split(precision);
-  if (skew && precision != QX_PREC_C64 && m == n && m % 64 == 0 && k % 16 == 0 &&
-      C.p != A.p && C.p != B.p && reg_shape(split, opA, opB, m, n) >= 0) {
-    g.tri = 1;
-    if (split == 3)
-      launch_reg<3, 2, 2, 4>(s, batch, opA, opB, g);
-    else
-      launch_reg<4, 2, 2, 4>(s, batch, opA, opB, g);
-    return;
-  }
   if (precision == QX_PREC_C64) {
     if (split == 3)
       launch_typed<float, 3>(s, batch, opA, opB, g);
@@ -771,20 +726,6 @@ void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int 
     else
       launch_typed<double, 4>(s, batch, opA, opB, g);
   }
-}
-
-}  // namespace
-
-void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
-              Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
-              const Mat& C, int precision, const Mat& Ct) {
-  zgemm_impl(s, batch, m, n, k, opA, opB, alpha, A, B, beta, Cin, C, precision, Ct, false);
-}
-
-void zgemm_skew(hipStream_t s, int64_t batch, int n, int k, int opA, int opB, Cplx alpha,
-                const Mat& A, const Mat& B, Cplx beta, const Mat& Cin, const Mat& C,
-                int precision) {
-  zgemm_impl(s, batch, n, n, k, opA, opB, alpha, A, B, beta, Cin, C, precision, Mat(), true);
 }
 
 }  // namespace qx

@@ -114,15 +114,6 @@ void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
 void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
               Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
               const Mat& C, int precision, const Mat& Ct = Mat());
-// C = alpha op(A) op(B) + beta Cin (n x n) for a product whose result is
-// skew-Hermitian (C = -C^H, so beta Cin must be skew-Hermitian too): only the
-// 64 x 64 tiles on or above the diagonal are computed, each off-diagonal one
-// is also stored mirrored, C[tj][ti] = -C[ti][tj]^H.  Falls back to the full
-// product where that path does not apply (complex64, n % 64 != 0,
-// k % 16 != 0, C aliasing A or B, LDS-staged kernel requested).
-void zgemm_skew(hipStream_t s, int64_t batch, int n, int k, int opA, int opB, Cplx alpha,
-                const Mat& A, const Mat& B, Cplx beta, const Mat& Cin, const Mat& C,
-                int precision);
 
 // Ozaki-split GEMM on bf16 matrix cores: C = alpha op(A) op(B) + beta Cin
 // from `slices` 7-bit digit slices per operand (2..7); ws holds the digits.

@@ -212,16 +212,6 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   });
 }
 
-// The lesser / greater self-energies are skew-Hermitian (the contract of
-// selected_solve, rgf.py:138-143), and so is every product of the form
-// x^R s x^R^H or a s a^H with s skew-Hermitian.  Those products compute only
-// the tiles on and above the diagonal and mirror the rest (zgemm_skew);
-// QX_RGF_SKEW=0 (read per call) computes them in full.
-bool skew_products() {
-  const char* env = std::getenv("QX_RGF_SKEW");
-  return !(env && env[0] == '0');
-}
-
 void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
                         const void* a, const qx_btd_layout* lsl, const void* sl,
                         const qx_btd_layout* lsg, const void* sg,
@@ -257,18 +247,9 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       XR0(want_r ? lxr : nullptr, want_r ? xr : nullptr);
 
   const int gp = gemm_precision(prec);
-  const bool skew = skew_products();
   run_split(h, st0, Stot, [&](hipStream_t st, int64_t e0, int64_t S) {
   Btd A = A0.sub(e0), SL = SL0.sub(e0), SG = SG0.sub(e0), XL = XL0.sub(e0), XG = XG0.sub(e0),
       XR = XR0.sub(e0);
-  // C = alpha op(A) op(B) + beta Cin (n x n) for a skew-Hermitian result
-  auto sk_gemm = [&](int n, int k, int opA, int opB, Cplx al, const Mat& Ma, const Mat& Mb,
-                     Cplx be, const Mat& Cin, const Mat& C) {
-    if (skew)
-      zgemm_skew(st, S, n, k, opA, opB, al, Ma, Mb, be, Cin, C, gp);
-    else
-      zgemm_in(st, S, n, n, k, opA, opB, al, Ma, Mb, be, Cin, C, gp);
-  };
   const Slots W{ws, Stot, (int64_t)bmax * bmax, e0};
   const Obc OR{obc_r, e0}, OL{obc_l, e0}, OG{obc_g, e0};
   int* info = info_all + e0;
@@ -320,7 +301,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       const Mat T3 = W.get(T3s, b0), T6 = W.get(T6s, b0);
       fold(st, S, b0, b0, q.sig->diag(0), 1.0, q.obc->get(0, b0), T3);
       zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_N, kOne, X0, T3, kZero, T6, gp);
-      sk_gemm(b0, b0, QX_OP_N, QX_OP_C, kOne, T6, X0, kZero, Mat(), W.get((*q.slot)[0], b0));
+      zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_C, kOne, T6, X0, kZero, W.get((*q.slot)[0], b0), gp);
     }
   }
 
@@ -356,14 +337,14 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
         Sjj = T3;
       }
       zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get((*q.slot)[i], bi), kZero, T4, gp);
-      sk_gemm(bj, bi, QX_OP_N, QX_OP_C, kOne, T4, A.lower(i), kOne, Sjj, T3);
+      zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, A.lower(i), kOne, Sjj, T3, gp);
       const Mat sij = q.sig->upper(i);
       if (!sij.zero()) {
         zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kOne, T1, sij, kZero, T5, gp);
         ew_add_hdiff(st, S, bj, T5, T3);
       }
       zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_N, kOne, Xrj, T3, kZero, T6, gp);
-      sk_gemm(bj, bj, QX_OP_N, QX_OP_C, kOne, T6, Xrj, kZero, Mat(), W.get((*q.slot)[j], bj));
+      zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_C, kOne, T6, Xrj, kZero, W.get((*q.slot)[j], bj), gp);
     }
   }
 
@@ -407,7 +388,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
         const Q& q = qs[qi];
         const Mat T4 = W.get(T4s, bi);
         zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, Aji, W.get((*q.slot)[i], bi), kZero, T4, gp);
-        sk_gemm(bj, bi, QX_OP_N, QX_OP_C, kOne, T4, Aji, kZero, Mat(), tl[qi]);
+        zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, Aji, kZero, tl[qi], gp);
         const Mat sij = q.sig->upper(i);
         if (!sij.zero()) {
           zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kOne, C1, sij, kZero, C4, gp);
@@ -436,7 +417,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       Mat Oij = q.out->upper(i);
       if (Oij.zero()) Oij = W.get(T4s, bj);
       zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, B1, Xj, kZero, Oij, gp);  // -temp_2x
-      sk_gemm(bi, bj, QX_OP_N, QX_OP_C, kMinus, Oij, B1, kOne, Z, Z);  // + temp_2x B1^H
+      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, Oij, B1, kOne, Z, gp);    // + temp_2x B1^H
       // the GEMM that completes x_ij also writes the lower block -x_ij^H
       const Mat low = (!q.skip_lower) ? q.out->lower(i) : Mat();
       zgemm_in(st, S, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, Xi, B2, kOne, Oij, Oij, gp,

@@ -146,6 +146,24 @@ def test_workload_parity_c128_3m(device, workload):
     assert rel_err(cur.cpu().numpy(), rcur) < 1e-10
 
 
+@pytest.mark.parametrize("precision", ["c128", "c128-3m"])
+def test_deep_recursion_parity(device, precision):
+    """16 blocks of 128 at eta = 1e-3 (the benchmark's distributions): the
+    left-connected G^R grows like 1 / eta, so the Hermitian rounding part of
+    each x^< product grows by ~|x^R|^2 per block.  It is harmless while it
+    stays Hermitian (the final (X - X^H) / 2 drops it) but not if any step
+    mixes it into the skew-Hermitian solution; this depth and block size
+    expose such a step (a product with mirrored upper tiles reached relative
+    error ~10 here), which the shallow cases above do not."""
+    w = make_workload(seed=2, n_energies=2, n_blocks=16, block_size=128)
+    oa, osl, osg = _oracle_containers(w)
+    rl, rg, rr = orgf.RGF(2).selected_solve(oa, osl, osg, return_retarded=True)
+    a, sl, sg = to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device)
+    xl, xg, xr = RGF(2, precision=precision).selected_solve(a, sl, sg, return_retarded=True)
+    for g, o in ((xl, rl), (xg, rg), (xr, rr)):
+        assert _per_block_worst(w, g, o) < 1e-10
+
+
 def test_workload_device_assembly_matches(device, workload):
     """Workload.assemble (device, dense-block layout) equals the oracle containers."""
     w, (oa, osl, osg), _ = workload
@@ -314,51 +332,6 @@ def test_fused_small_block_path_matches_general(device, monkeypatch, precision):
             a, bl, bg, obc_blocks=gobc, return_retarded=True, return_current=True,
             corner=corner, density=dens)
         outs[fused] = [t.cpu().numpy() for t in (xl.data, xg.data, xr.data, cur, corner, dens)]
-    for got, ref in zip(outs["1"], outs["0"]):
-        scale = np.max(np.abs(ref))
-        assert scale > 0
-        assert np.max(np.abs(got - ref)) <= 1e-12 * scale
-
-
-@pytest.mark.parametrize("precision", ["c128", "c128-3m"])
-def test_skew_hermitian_products(device, monkeypatch, precision):
-    """The products x^R s x^R^H and a s a^H (s skew-Hermitian, as the
-    self-energies must be) compute the tiles on and above the diagonal and
-    mirror the rest when the block is a multiple of 64.  Blocks of 64 and 128
-    take that path, 96 falls back to the full product; with Sigma
-    off-diagonal blocks, OBC blocks and every optional output the results
-    equal the full-product path's (QX_RGF_SKEW=0) to rounding and the
-    oracle's within the 1e-10 bar."""
-    bs = np.array([64, 128, 96, 64])
-    dense = bt_dense(bs, seed=3)
-    coo = sps.coo_matrix(dense)
-    b = coo - coo.conj().T
-    ne, n = 3, len(bs)
-    oa = ODSDBCOO.from_sparray(coo, bs, (ne,))
-    ob = ODSDBCOO.from_sparray(sps.coo_matrix(b), bs, (ne,))
-    refs = orgf.RGF(100).selected_solve(oa, ob, ob, return_retarded=True)
-    rng = np.random.default_rng(8)
-    gobc = OBCBlocks(n)
-    gobc.retarded[0] = torch.as_tensor(
-        0.05 * (rng.standard_normal((ne, 64, 64)) + 1j * rng.standard_normal((ne, 64, 64))),
-        device=device)
-    gobc.lesser[0] = torch.as_tensor(0.1j * np.broadcast_to(np.eye(64), (ne, 64, 64)).copy(),
-                                     device=device)
-    gobc.greater[n - 1] = torch.as_tensor(
-        -0.1j * np.broadcast_to(np.eye(64), (ne, 64, 64)).copy(), device=device)
-    outs = {}
-    for sk in ("1", "0"):
-        monkeypatch.setenv("QX_RGF_SKEW", sk)
-        a, bl, bg = to_gpu(oa, device), to_gpu(ob, device), to_gpu(ob, device)
-        xs = RGF(100, precision=precision).selected_solve(a, bl, bg, return_retarded=True)
-        for x, ref in zip(xs, refs):
-            assert close(x, ref, 1e-10)
-        corner = torch.zeros((ne, bs[0], bs[-1]), dtype=torch.complex128, device=device)
-        dens = torch.zeros((ne, int(bs.sum())), dtype=torch.float64, device=device)
-        xl, xg, xr, cur = RGF(100, precision=precision, retarded_offdiag=True).selected_solve(
-            to_gpu(oa, device), bl, bg, obc_blocks=gobc, return_retarded=True,
-            return_current=True, corner=corner, density=dens)
-        outs[sk] = [t.cpu().numpy() for t in (xl.data, xg.data, xr.data, cur, corner, dens)]
     for got, ref in zip(outs["1"], outs["0"]):
         scale = np.max(np.abs(ref))
         assert scale > 0
