@@ -162,6 +162,9 @@ def test_deep_recursion_parity(device, precision):
     xl, xg, xr = RGF(2, precision=precision).selected_solve(a, sl, sg, return_retarded=True)
     for g, o in ((xl, rl), (xg, rg), (xr, rr)):
         assert _per_block_worst(w, g, o) < 1e-10
+    # selected_inv on the same system (G^R only)
+    xi = RGF(2, precision=precision).selected_inv(a)
+    assert _per_block_worst(w, xi, orgf.RGF(2).selected_inv(oa)) < 1e-10
 
 
 def test_workload_device_assembly_matches(device, workload):
