@@ -484,20 +484,8 @@ __global__ __launch_bounds__(NT) void inv_update(UpdArgs g) {
   if (tid < 64) sig_r[tid] = (r0 + tid < n) ? sig[r0 + tid] : 0;
   if (tid < NB) sig_k[tid] = (tid < nbw) ? sig[k0 + tid] : 0;
   __syncthreads();
-  // A = reduced panel rows of this tile, B = gathered pivot rows
-  for (int q = tid; q < 64 * NB; q += NT) {
-    const int r = q / NB, k = q - r * NB;
-    z2 v = make_double2(0.0, 0.0);
-    if (r0 + r < n && k < nbw) v = Xn[(int64_t)(r0 + r) * g.ldn + k0 + k];
-    As[0][k][r] = (T)v.x;
-    As[1][k][r] = (T)v.y;
-    const int k2 = q >> 6, c = q & 63;
-    z2 u = make_double2(0.0, 0.0);
-    if (k2 < nbw && c0 + c < n) u = Xc[(int64_t)sig_k[k2] * g.ldc + c0 + c];
-    Bs[0][k2][c] = (T)u.x;
-    Bs[1][k2][c] = (T)u.y;
-  }
-  // C init: gathered rows, pivot rows zeroed
+  // C init: gathered rows, pivot rows zeroed.  Issued before the operand
+  // staging below so that both sets of loads are in flight together.
   acc_t aR[2][2], aI[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -513,6 +501,20 @@ __global__ __launch_bounds__(NT) void inv_update(UpdArgs g) {
         aR[i][j][q] = (T)v.x;
         aI[i][j][q] = (T)v.y;
       }
+  // A = reduced panel rows of this tile, B = gathered pivot rows
+#pragma unroll
+  for (int q = tid; q < 64 * NB; q += NT) {
+    const int r = q / NB, k = q - r * NB;
+    z2 v = make_double2(0.0, 0.0);
+    if (r0 + r < n && k < nbw) v = Xn[(int64_t)(r0 + r) * g.ldn + k0 + k];
+    As[0][k][r] = (T)v.x;
+    As[1][k][r] = (T)v.y;
+    const int k2 = q >> 6, c = q & 63;
+    z2 u = make_double2(0.0, 0.0);
+    if (k2 < nbw && c0 + c < n) u = Xc[(int64_t)sig_k[k2] * g.ldc + c0 + c];
+    Bs[0][k2][c] = (T)u.x;
+    Bs[1][k2][c] = (T)u.y;
+  }
   __syncthreads();
 #pragma unroll
   for (int kq = 0; kq < NB / 4; ++kq) {
