@@ -71,7 +71,23 @@ struct GemmArgs {
   z2* Ct;  // optional second output Ct = -C^H (n x m), or null
   int64_t sCt;
   int ldct;
+  // optional: the beta term is Cin + E - E^H (m == n), E n x n
+  const z2* E;
+  int64_t sE;
+  int lde;
 };
+
+// beta-term source of element (row, col) of matrix e: Cin, or Cin + E - E^H
+// with the additions in the order of the Hermitian-part kernel
+// ((E - E^H) + Cin), so that the fused form rounds like the separate pass.
+__device__ __forceinline__ z2 beta_source(const GemmArgs& g, int64_t e, const z2* Ci, int row,
+                                          int col) {
+  const z2 c = Ci[(int64_t)row * g.ldcin + col];
+  if (!g.E) return c;
+  const z2* E = g.E + e * g.sE;
+  const z2 a = E[(int64_t)row * g.lde + col], b = E[(int64_t)col * g.lde + row];
+  return make_double2((a.x - b.x) + c.x, (a.y + b.y) + c.y);
+}
 
 // Hardware dispatches workgroup ids round-robin over the 8 XCDs.  Return a
 // logical id such that each XCD receives one contiguous logical range.
@@ -283,7 +299,7 @@ void zgemm_kernel(GemmArgs g) {
           double outr = fma(g.ar, vr, -(g.ai * vi)), outi = fma(g.ar, vi, g.ai * vr);
           z2* dst = C + (int64_t)row * g.ldc + col;
           if (!g.beta_zero) {
-            const z2 c = Ci[(int64_t)row * g.ldcin + col];
+            const z2 c = beta_source(g, e, Ci, row, col);
             outr += fma(g.br, c.x, -(g.bi * c.y));
             outi += fma(g.br, c.y, g.bi * c.x);
           }
@@ -490,7 +506,7 @@ void zgemm_reg_kernel(GemmArgs g) {
           // explicit fma, as in zgemm_kernel
           double outr = fma(g.ar, vr, -(g.ai * vi)), outi = fma(g.ar, vi, g.ai * vr);
           if (!g.beta_zero) {
-            const z2 c = Ci[(int64_t)row * g.ldcin + col];
+            const z2 c = beta_source(g, e, Ci, row, col);
             outr += fma(g.br, c.x, -(g.bi * c.y));
             outi += fma(g.br, c.y, g.bi * c.x);
           }
@@ -671,10 +687,15 @@ void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
   zgemm_in(s, batch, m, n, k, opA, opB, alpha, A, B, beta, C, C, precision, Mat());
 }
 
-void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
-              Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
-              const Mat& C, int precision, const Mat& Ct) {
+namespace {
+
+void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+                Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
+                const Mat& C, int precision, const Mat& Ct, const Mat& E) {
   QX_REQUIRE(!C.zero(), "zgemm: output is a zero matrix");
+  QX_REQUIRE(E.zero() || (m == n && !Cin.zero() && E.p != C.p && Cin.p != C.p),
+             "zgemm: the D + E - E^H beta term needs m == n, a non-zero D and an output "
+             "aliasing neither");
   QX_REQUIRE(Ct.zero() || (Ct.p != C.p && Ct.p != A.p && Ct.p != B.p && Ct.p != Cin.p),
              "zgemm: the transposed output aliases an operand");
   QX_REQUIRE(opA == QX_OP_N || opA == QX_OP_C, "zgemm: bad op_a");
@@ -685,12 +706,21 @@ void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int op
   if (batch <= 0 || m <= 0 || n <= 0) return;
   const bool beta_zero = (beta.re == 0.0 && beta.im == 0.0) || Cin.zero();
   if (A.zero() || B.zero() || k <= 0) {
-    // op(A) op(B) == 0: C = beta Cin.
-    ew_axpby(s, batch, m, n, beta_zero ? Cplx{0, 0} : beta, Cin, Cplx{0, 0}, Mat(), C);
+    // op(A) op(B) == 0: C = beta Cin (Cin + E - E^H).
+    if (!E.zero() && !beta_zero) {
+      ew_herm_add(s, batch, n, Cin, 1.0, E, C);
+      if (beta.re != 1.0 || beta.im != 0.0)
+        ew_axpby(s, batch, m, n, beta, C, Cplx{0, 0}, Mat(), C);
+    } else {
+      ew_axpby(s, batch, m, n, beta_zero ? Cplx{0, 0} : beta, Cin, Cplx{0, 0}, Mat(), C);
+    }
     if (!Ct.zero()) ew_neg_h(s, batch, m, n, C, Ct);
     return;
   }
   GemmArgs g;
+  g.E = E.p;
+  g.sE = E.stride;
+  g.lde = E.ld;
   g.Ct = Ct.p;
   g.sCt = Ct.stride;
   g.ldct = Ct.ld;
@@ -726,6 +756,20 @@ void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int op
     else
       launch_typed<double, 4>(s, batch, opA, opB, g);
   }
+}
+
+}  // namespace
+
+void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+              Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
+              const Mat& C, int precision, const Mat& Ct) {
+  zgemm_impl(s, batch, m, n, k, opA, opB, alpha, A, B, beta, Cin, C, precision, Ct, Mat());
+}
+
+void zgemm_hin(hipStream_t s, int64_t batch, int n, int k, int opA, int opB, Cplx alpha,
+               const Mat& A, const Mat& B, Cplx beta, const Mat& D, const Mat& E, const Mat& C,
+               int precision) {
+  zgemm_impl(s, batch, n, n, k, opA, opB, alpha, A, B, beta, D, C, precision, Mat(), E);
 }
 
 }  // namespace qx

@@ -114,6 +114,12 @@ void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
 void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
               Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
               const Mat& C, int precision, const Mat& Ct = Mat());
+// C = alpha op(A) op(B) + beta (D + E - E^H), n x n: the Hermitian-part sum
+// (ew_herm_add) formed in the product's epilogue, rounded as that kernel
+// rounds it.  C aliases neither D nor E.
+void zgemm_hin(hipStream_t s, int64_t batch, int n, int k, int opA, int opB, Cplx alpha,
+               const Mat& A, const Mat& B, Cplx beta, const Mat& D, const Mat& E, const Mat& C,
+               int precision);
 
 // Ozaki-split GEMM on bf16 matrix cores: C = alpha op(A) op(B) + beta Cin
 // from `slices` 7-bit digit slices per operand (2..7); ws holds the digits.

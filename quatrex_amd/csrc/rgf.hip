@@ -410,14 +410,14 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
         zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, sij, kZero, T5, gp);
         zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, T5, B3, kOne, T1, gp);
       }
-      // x_ii' = x_ii + (temp_1x - temp_1x^H) [+ temp_2x B1^H below]
+      // x_ii' = x_ii + (temp_1x - temp_1x^H) + temp_2x B1^H, the first two
+      // terms formed in the epilogue of the product below
       const Mat Z = W.get(spare, bi);
-      ew_herm_add(st, S, bi, Xi, 1.0, T1, Z);
       // x_ij = -temp_2x - x_ii B2^H + xr_ii s_ij xr_jj^H,  temp_2x = B1 x_jj
       Mat Oij = q.out->upper(i);
       if (Oij.zero()) Oij = W.get(T4s, bj);
       zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, B1, Xj, kZero, Oij, gp);  // -temp_2x
-      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, Oij, B1, kOne, Z, gp);    // + temp_2x B1^H
+      zgemm_hin(st, S, bi, bj, QX_OP_N, QX_OP_C, kMinus, Oij, B1, kOne, Xi, T1, Z, gp);
       // the GEMM that completes x_ij also writes the lower block -x_ij^H
       const Mat low = (!q.skip_lower) ? q.out->lower(i) : Mat();
       zgemm_in(st, S, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, Xi, B2, kOne, Oij, Oij, gp,
