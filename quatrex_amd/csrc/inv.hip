@@ -615,6 +615,25 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
 
 }  // namespace
 
+// Panel width: 32 columns up to n = 256 (16 above, where two rows per
+// thread double the per-column work); QX_INV_NB=16|32 overrides for n <= 256.
+int inv_panel_width(int n) {
+  static const int forced_nb = []() {
+    const char* e = std::getenv("QX_INV_NB");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 16 || v == 32) ? v : 0;
+  }();
+  return n > 256 ? 16 : (forced_nb ? forced_nb : 32);
+}
+
+Mat zinv_input(int n, const Mat& X, const InvScratch& scr) {
+  const int NB = inv_panel_width(n);
+  const int npan = (n + NB - 1) / NB;
+  // the panels ping-pong between X and the scratch buffer, the last one
+  // writing the scratch buffer: an odd count starts in X, an even one in it
+  return (npan % 2) ? X : Mat(scr.buf, scr.buf_stride, n);
+}
+
 void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const Mat& X,
           int* info, int precision, const InvScratch& scr) {
   QX_REQUIRE(n >= 1 && n <= kInvMaxN, "inv: block size must be in [1, 512]");
@@ -639,14 +658,7 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const
   }
 
   const bool big = n > 256;
-  // Panel width: 32 columns up to n = 256 (16 above, where two rows per
-  // thread double the per-column work); QX_INV_NB=16|32 overrides for n <= 256.
-  static const int forced_nb = []() {
-    const char* e = std::getenv("QX_INV_NB");
-    const int v = e ? std::atoi(e) : 0;
-    return (v == 16 || v == 32) ? v : 0;
-  }();
-  const int NB = big ? 16 : (forced_nb ? forced_nb : 32);
+  const int NB = inv_panel_width(n);
   const int npan = (n + NB - 1) / NB;
   // start in the buffer that makes the last panel write the scratch buffer
   const int s0 = 1 ^ (npan % 2);

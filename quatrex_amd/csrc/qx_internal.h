@@ -131,7 +131,8 @@ void zgemm_ozaki(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int
 // X = inv(A - S) with partial pivoting (S may be zero).  info[e] is set to a
 // nonzero value for an exactly singular matrix.  n <= kInvMaxN.  The scratch
 // holds, per matrix, an n x n complex buffer (buf + e * buf_stride) and 3 n
-// ints (ints + e * int_stride); A may alias X but neither may alias buf.
+// ints (ints + e * int_stride); A may alias X, or be zinv_input(n, X, scr);
+// X may not alias buf.
 constexpr int kInvMaxN = 512;
 struct InvScratch {
   z2* buf;
@@ -141,6 +142,10 @@ struct InvScratch {
 };
 void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S,
           const Mat& X, int* info, int precision, const InvScratch& scr);
+// The buffer an inverse into X starts from (X itself or the scratch buffer):
+// a matrix formed there and passed as A (S zero) is inverted into X without
+// the initial copy.
+Mat zinv_input(int n, const Mat& X, const InvScratch& scr);
 
 // Elementwise helpers on batched m x n blocks.
 // Y = a*A + b*B            (A, B may be zero matrices)

@@ -97,14 +97,16 @@ int gemm_precision(int prec) {
 // entries are of the order of the FP32 rounding error and would be lost in
 // FP32), the small correction X R by the FP32 GEMM.  Mc and R are n x n
 // temporaries.
-void block_inv(hipStream_t st, int64_t S, int n, const Mat& X, int* info, int prec,
-               const InvScratch& scr, const Mat& Mc, const Mat& R) {
+// X = inv(M) for M in Ain, which is X or zinv_input(n, X, scr): the products
+// that form M write it there, so the inverse starts without a copy.
+void block_inv(hipStream_t st, int64_t S, int n, const Mat& Ain, const Mat& X, int* info,
+               int prec, const InvScratch& scr, const Mat& Mc, const Mat& R) {
   if (prec != QX_PREC_MIXED) {
-    zinv(st, S, n, X, Mat(), X, info, prec == QX_PREC_C64 ? QX_PREC_C64 : QX_PREC_C128, scr);
+    zinv(st, S, n, Ain, Mat(), X, info, prec == QX_PREC_C64 ? QX_PREC_C64 : QX_PREC_C128, scr);
     return;
   }
-  ew_axpby(st, S, n, n, kOne, X, kZero, Mat(), Mc);  // M
-  zinv(st, S, n, X, Mat(), X, info, QX_PREC_C64, scr);
+  ew_axpby(st, S, n, n, kOne, Ain, kZero, Mat(), Mc);  // M
+  zinv(st, S, n, Ain, Mat(), X, info, QX_PREC_C64, scr);
   zgemm(st, S, n, n, n, QX_OP_N, QX_OP_N, kOne, Mc, X, kZero, R, QX_PREC_C128);
   ew_ident_sub(st, S, n, R);                          // R = I - M X
   ew_axpby(st, S, n, n, kOne, X, kZero, Mat(), Mc);  // X0
@@ -167,21 +169,23 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     auto bs = [&](int i) { return A.bs(i); };
 
     // Forward sweep (rgf.py:70-92).
-    fold(st, S, bs(0), bs(0), A.diag(0), -1.0, OR.get(0, bs(0)), W.get(xd[0], bs(0)));
-    block_inv(st, S, bs(0), W.get(xd[0], bs(0)), info, prec, scr, W.get(N + 6, bs(0)),
-              W.get(N + 7, bs(0)));
+    {
+      const Mat X0 = W.get(xd[0], bs(0)), M0 = zinv_input(bs(0), X0, scr);
+      fold(st, S, bs(0), bs(0), A.diag(0), -1.0, OR.get(0, bs(0)), M0);
+      block_inv(st, S, bs(0), M0, X0, info, prec, scr, W.get(N + 6, bs(0)), W.get(N + 7, bs(0)));
+    }
     for (int i = 0; i + 1 < N; ++i) {
       const int j = i + 1, bi = bs(i), bj = bs(j);
       const Mat T1 = W.get(t1, bi);
       zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get(xd[i], bi), kZero, T1, gp);
-      const Mat Xj = W.get(xd[j], bj);
+      const Mat Xj = W.get(xd[j], bj), Mj = zinv_input(bj, Xj, scr);
       Mat Ajj = A.diag(j);
       if (!OR.get(j, bj).zero()) {
-        fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xj);
-        Ajj = Xj;
+        fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Mj);
+        Ajj = Mj;
       }
-      zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Ajj, Xj, gp);
-      block_inv(st, S, bj, Xj, info, prec, scr, W.get(N + 6, bj), W.get(N + 7, bj));
+      zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Ajj, Mj, gp);
+      block_inv(st, S, bj, Mj, Xj, info, prec, scr, W.get(N + 6, bj), W.get(N + 7, bj));
     }
     // rgf.py:94-95
     if (!X.diag(N - 1).zero())
@@ -293,9 +297,9 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   // ---- block 0 (rgf.py:207-231) ----
   {
     const int b0 = bs(0);
-    const Mat X0 = W.get(sr[0], b0);
-    fold(st, S, b0, b0, A.diag(0), -1.0, OR.get(0, b0), X0);
-    block_inv(st, S, b0, X0, info, prec, scr, W.get(NS1, b0), W.get(NS2, b0));
+    const Mat X0 = W.get(sr[0], b0), M0 = zinv_input(b0, X0, scr);
+    fold(st, S, b0, b0, A.diag(0), -1.0, OR.get(0, b0), M0);
+    block_inv(st, S, b0, M0, X0, info, prec, scr, W.get(NS1, b0), W.get(NS2, b0));
     if (corner) ew_axpby(st, S, b0, b0, kOne, X0, kZero, Mat(), W.get(gc_cur, bmax));
     for (auto& q : qs) {
       const Mat T3 = W.get(T3s, b0), T6 = W.get(T6s, b0);
@@ -312,14 +316,16 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const Mat T1 = W.get(T1s, bi);
     // a_ji xr_ii
     zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), Xri, kZero, T1, gp);
-    // xr_jj = inv(a_jj - obc - a_ji xr_ii a_ij)
+    // xr_jj = inv(a_jj - obc - a_ji xr_ii a_ij), the argument formed where
+    // the inverse starts
+    const Mat Mj = zinv_input(bj, Xrj, scr);
     Mat Ajj = A.diag(j);
     if (!OR.get(j, bj).zero()) {
-      fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Xrj);
-      Ajj = Xrj;
+      fold(st, S, bj, bj, A.diag(j), -1.0, OR.get(j, bj), Mj);
+      Ajj = Mj;
     }
-    zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Ajj, Xrj, gp);
-    block_inv(st, S, bj, Xrj, info, prec, scr, W.get(NS1, bj), W.get(NS2, bj));
+    zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Ajj, Mj, gp);
+    block_inv(st, S, bj, Mj, Xrj, info, prec, scr, W.get(NS1, bj), W.get(NS2, bj));
     if (corner) {
       // g_{0,j} = -(g_{0,i} a_ij) x^R_jj
       zgemm(st, S, b0, bj, bi, QX_OP_N, QX_OP_N, kOne, W.get(gc_cur, bmax), A.upper(i), kZero,
