@@ -304,10 +304,10 @@ __global__ __launch_bounds__(NT) void inv_panel(PanelArgs g) {
 // t holds columns 8 q .. 8 q + 7 (q = t % 4) of row t / 4 of the 32-column
 // panel, so each elimination step costs a thread 8 complex updates instead
 // of 32, and the four threads of a row are neighbouring lanes of one wave
-// (the active column's value reaches them by a DPP quad broadcast).  The column loop is
-// unrolled, so every register index is a compile-time constant and nothing
-// is rotated.  Pivot choice, outputs and the row / column gathers are those
-// of inv_panel.
+// (the active column's value reaches them by a DPP quad broadcast).  The
+// eight columns of a quarter are unrolled, so every register index is a
+// compile-time constant and nothing is rotated.  Pivot choice, outputs and
+// the row / column gathers are those of inv_panel.
 constexpr int NT4 = 1024;
 
 template <typename T>
