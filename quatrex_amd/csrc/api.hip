@@ -147,6 +147,8 @@ int beyn_moments(qx_handle* h, hipStream_t st, int64_t n, int b, int nq, double 
 thread_local KernelTimer* active_timer = nullptr;
 thread_local int active_gemm_tile = 0;
 
+thread_local int split_part = 0;
+
 void run_split(qx_handle* h, hipStream_t st, int64_t S,
                const std::function<void(hipStream_t, int64_t, int64_t)>& f) {
   int P = (int)std::min<int64_t>(h->parts, std::max<int64_t>(1, S / 4));
@@ -177,6 +179,7 @@ void run_split(qx_handle* h, hipStream_t st, int64_t S,
         active_timer = timer;
         active_gemm_tile = tile;
         QX_HIP_CHECK(hipSetDevice(h->device));
+        split_part = k;
         f(h->streams[k - 1], e0[k], e0[k + 1] - e0[k]);
       } catch (...) {
         err[k] = std::current_exception();
@@ -185,6 +188,7 @@ void run_split(qx_handle* h, hipStream_t st, int64_t S,
     });
   }
   try {
+    split_part = 0;
     f(st, 0, e0[1]);
   } catch (...) {
     err[0] = std::current_exception();

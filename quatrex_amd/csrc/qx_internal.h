@@ -222,6 +222,9 @@ extern thread_local int active_gemm_tile;
 // already queued on `st`, and `st` waits for all of them before returning.
 void run_split(qx_handle* h, hipStream_t st, int64_t S,
                const std::function<void(hipStream_t, int64_t, int64_t)>& f);
+// Index of the sub-batch the calling thread runs inside run_split (0 on the
+// caller's thread).
+extern thread_local int split_part;
 
 }  // namespace qx
 

@@ -11,6 +11,7 @@
 // they are exactly zero.
 #include <algorithm>
 #include <cstring>
+#include <future>
 
 #include "qx_internal.h"
 
@@ -251,7 +252,30 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       XR0(want_r ? lxr : nullptr, want_r ? xr : nullptr);
 
   const int gp = gemm_precision(prec);
+  // The odd sub-batches start after sub-batch 0's first block inverse, so
+  // that the sub-batches' latency-bound inverse phases do not coincide and
+  // the GEMMs of one fill the GPU during the inverse of another
+  // (QX_RGF_STAGGER=0, read per call, starts all together).
+  const char* stg_env = std::getenv("QX_RGF_STAGGER");
+  const bool stagger = !(stg_env && stg_env[0] == '0');
+  hipEvent_t stg_ev = nullptr;
+  if (stagger) QX_HIP_CHECK(hipEventCreateWithFlags(&stg_ev, hipEventDisableTiming));
+  std::promise<void> stg_set;
+  std::shared_future<void> stg_ready = stg_set.get_future().share();
   run_split(h, st0, Stot, [&](hipStream_t st, int64_t e0, int64_t S) {
+  const int part = split_part;
+  // sub-batch 0 releases the others once its first inverse is enqueued, or
+  // when it leaves early with an error
+  struct Release {
+    std::promise<void>* p;
+    ~Release() {
+      if (p) p->set_value();
+    }
+  } release{stagger && part == 0 ? &stg_set : nullptr};
+  if (stagger && part % 2 == 1) {
+    stg_ready.wait();
+    QX_HIP_CHECK(hipStreamWaitEvent(st, stg_ev, 0));
+  }
   Btd A = A0.sub(e0), SL = SL0.sub(e0), SG = SG0.sub(e0), XL = XL0.sub(e0), XG = XG0.sub(e0),
       XR = XR0.sub(e0);
   const Slots W{ws, Stot, (int64_t)bmax * bmax, e0};
@@ -300,6 +324,11 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const Mat X0 = W.get(sr[0], b0), M0 = zinv_input(b0, X0, scr);
     fold(st, S, b0, b0, A.diag(0), -1.0, OR.get(0, b0), M0);
     block_inv(st, S, b0, M0, X0, info, prec, scr, W.get(NS1, b0), W.get(NS2, b0));
+    if (release.p) {
+      QX_HIP_CHECK(hipEventRecord(stg_ev, st));
+      release.p->set_value();
+      release.p = nullptr;
+    }
     if (corner) ew_axpby(st, S, b0, b0, kOne, X0, kZero, Mat(), W.get(gc_cur, bmax));
     for (auto& q : qs) {
       const Mat T3 = W.get(T3s, b0), T6 = W.get(T6s, b0);
@@ -454,6 +483,8 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     }
   }
   });
+  // released once the waits enqueued on it have been satisfied
+  if (stg_ev) QX_HIP_CHECK(hipEventDestroy(stg_ev));
 }
 
 }  // namespace qx
