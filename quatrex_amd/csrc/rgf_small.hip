@@ -577,10 +577,14 @@ __global__ __launch_bounds__(kThreads) void inv_small_kernel(int64_t batch, int 
 
 const Cplx kOneS{1.0, 0.0}, kMinusS{-1.0, 0.0}, kZeroS{0.0, 0.0};
 
+// SP: the complex product split (4 = 4M, 3 = 3M), a template parameter so
+// that each inlined block product carries only its own MFMA sequence.
+template <int SP>
 __global__ __launch_bounds__(kThreads) void rgf_small_kernel(SmallArgs g) {
   __shared__ Shared sh;
   const int64_t e = blockIdx.x;
-  const int N = g.N, sp = g.split;
+  const int N = g.N;
+  constexpr int sp = SP;
   const bool want_r = (g.flags & QX_SOLVE_RETARDED) != 0;
   const bool want_r_off = (g.flags & QX_SOLVE_RETARDED_OFFDIAG) != 0;
   const bool want_cur = (g.flags & QX_SOLVE_CURRENT) != 0 && g.current;
@@ -782,10 +786,12 @@ struct SmallInvArgs {
   int* info;
 };
 
+template <int SP>
 __global__ __launch_bounds__(kThreads) void rgf_small_inv_kernel(SmallInvArgs g) {
   __shared__ Shared sh;
   const int64_t e = blockIdx.x;
-  const int N = g.N, sp = g.split;
+  const int N = g.N;
+  constexpr int sp = SP;
   int* info = g.info + e;
   z2* W = g.ws + e * (int64_t)g.NS * kSB * kSB;
   auto slot = [&](int s, int ld) { return M{W + (int64_t)s * kSB * kSB, ld}; };
@@ -959,7 +965,10 @@ bool rgf_small_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, cons
   KernelTimer* timer = active_timer;
   hipEvent_t ev0 = nullptr;
   if (timer) timer->begin(st, &ev0);
-  hipLaunchKernelGGL(rgf_small_kernel, dim3((unsigned)S), dim3(kThreads), 0, st, g);
+  if (g.split == 3)
+    hipLaunchKernelGGL(rgf_small_kernel<3>, dim3((unsigned)S), dim3(kThreads), 0, st, g);
+  else
+    hipLaunchKernelGGL(rgf_small_kernel<4>, dim3((unsigned)S), dim3(kThreads), 0, st, g);
   QX_HIP_CHECK(hipGetLastError());
   if (timer) {
     // algorithmic GEMM flops of the sweep (8 b^3 per product, products with
@@ -1008,7 +1017,10 @@ bool rgf_small_inv(qx_handle* h, hipStream_t st, const qx_btd_layout* la, const 
   KernelTimer* timer = active_timer;
   hipEvent_t ev0 = nullptr;
   if (timer) timer->begin(st, &ev0);
-  hipLaunchKernelGGL(rgf_small_inv_kernel, dim3((unsigned)S), dim3(kThreads), 0, st, g);
+  if (g.split == 3)
+    hipLaunchKernelGGL(rgf_small_inv_kernel<3>, dim3((unsigned)S), dim3(kThreads), 0, st, g);
+  else
+    hipLaunchKernelGGL(rgf_small_inv_kernel<4>, dim3((unsigned)S), dim3(kThreads), 0, st, g);
   QX_HIP_CHECK(hipGetLastError());
   if (timer) {
     double fl = 0.0;
