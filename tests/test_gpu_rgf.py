@@ -167,6 +167,52 @@ def test_deep_recursion_parity(device, precision):
     assert _per_block_worst(w, xi, orgf.RGF(2).selected_inv(oa)) < 1e-10
 
 
+def test_deep_recursion_with_obc_blocks(device):
+    """The deep case with boundary self-energies on the first and last
+    blocks (retarded, and skew-Hermitian lesser / greater as the contact
+    self-energies are) and every optional output, against the oracle."""
+    w = make_workload(seed=4, n_energies=2, n_blocks=12, block_size=128)
+    oa, osl, osg = _oracle_containers(w)
+    n, ne, b = w.n_blocks, 2, 128
+    rng = np.random.default_rng(9)
+    def block(scale):
+        return scale * (rng.standard_normal((ne, b, b)) + 1j * rng.standard_normal((ne, b, b)))
+    gam0, gam1 = block(0.05), block(0.05)
+    obc = {"retarded": {0: block(0.05), n - 1: block(0.05)},
+           "lesser": {0: 1j * (gam0 @ gam0.conj().transpose(0, 2, 1))},
+           "greater": {n - 1: -1j * (gam1 @ gam1.conj().transpose(0, 2, 1))}}
+    oobc, gobc = orgf.OBCBlocks(n), OBCBlocks(n)
+    for kind, blocks in obc.items():
+        for i, v in blocks.items():
+            getattr(oobc, kind)[i] = v
+            getattr(gobc, kind)[i] = torch.as_tensor(v, device=device)
+    rl, rg, rr, rcur = orgf.RGF(2).selected_solve(
+        oa, osl, osg, obc_blocks=oobc, return_retarded=True, return_current=True,
+        retarded_offdiag=True)
+    a, sl, sg = to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device)
+    xl, xg, xr, cur = RGF(2, retarded_offdiag=True).selected_solve(
+        a, sl, sg, obc_blocks=gobc, return_retarded=True, return_current=True)
+    for g, o in ((xl, rl), (xg, rg), (xr, rr)):
+        assert _per_block_worst(w, g, o) < 1e-10
+    assert rel_err(cur.cpu().numpy(), rcur) < 1e-10
+
+
+def test_sub_batch_offset_changes_only_timing(device, monkeypatch):
+    """The odd sub-batches start after sub-batch 0's first block inverse
+    (QX_RGF_STAGGER); the energies are independent, so the outputs are
+    bitwise those of the sub-batches started together."""
+    w = make_workload(seed=5, n_energies=16, n_blocks=6, block_size=64)
+    oa, osl, osg = _oracle_containers(w)
+    outs = {}
+    for stg in ("1", "0"):
+        monkeypatch.setenv("QX_RGF_STAGGER", stg)
+        a, sl, sg = to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device)
+        xl, xg, xr = RGF(16).selected_solve(a, sl, sg, return_retarded=True)
+        outs[stg] = [t.data.cpu().numpy() for t in (xl, xg, xr)]
+    for x1, x0 in zip(outs["1"], outs["0"]):
+        assert np.array_equal(x1.view(np.uint64), x0.view(np.uint64))
+
+
 def test_workload_device_assembly_matches(device, workload):
     """Workload.assemble (device, dense-block layout) equals the oracle containers."""
     w, (oa, osl, osg), _ = workload
