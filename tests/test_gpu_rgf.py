@@ -195,6 +195,9 @@ def test_deep_recursion_with_obc_blocks(device):
     for g, o in ((xl, rl), (xg, rg), (xr, rr)):
         assert _per_block_worst(w, g, o) < 1e-10
     assert rel_err(cur.cpu().numpy(), rcur) < 1e-10
+    # selected_inv with the retarded boundary blocks
+    xi = RGF(2).selected_inv(a, obc_blocks=gobc)
+    assert _per_block_worst(w, xi, orgf.RGF(2).selected_inv(oa, obc_blocks=oobc)) < 1e-10
 
 
 def test_sub_batch_offset_changes_only_timing(device, monkeypatch):
