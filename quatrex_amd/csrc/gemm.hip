@@ -75,16 +75,39 @@ struct GemmArgs {
   const z2* E;
   int64_t sE;
   int lde;
+  // Two products of one shape in one launch: matrices e >= half take every
+  // operand from the second base (A2 ...) at index e - half.  A single
+  // product sets half = batch and the second bases equal to the first.
+  int64_t half;
+  const z2 *A2, *B2, *Cin2, *E2;
+  z2 *C2, *Ct2;
 };
+
+// Operand bases of matrix e of a launch (see GemmArgs::half).
+struct GemmPtrs {
+  const z2 *A, *B, *Cin, *E;
+  z2 *C, *Ct;
+};
+__device__ __forceinline__ GemmPtrs gemm_ptrs(const GemmArgs& g, int64_t e) {
+  const bool second = e >= g.half;
+  const int64_t l = second ? e - g.half : e;
+  GemmPtrs p;
+  p.A = (second ? g.A2 : g.A) + l * g.sA;
+  p.B = (second ? g.B2 : g.B) + l * g.sB;
+  p.C = (second ? g.C2 : g.C) + l * g.sC;
+  p.Cin = (second ? g.Cin2 : g.Cin) + l * g.sCin;
+  p.E = g.E ? (second ? g.E2 : g.E) + l * g.sE : nullptr;
+  p.Ct = g.Ct ? (second ? g.Ct2 : g.Ct) + l * g.sCt : nullptr;
+  return p;
+}
 
 // beta-term source of element (row, col) of matrix e: Cin, or Cin + E - E^H
 // with the additions in the order of the Hermitian-part kernel
 // ((E - E^H) + Cin), so that the fused form rounds like the separate pass.
-__device__ __forceinline__ z2 beta_source(const GemmArgs& g, int64_t e, const z2* Ci, int row,
+__device__ __forceinline__ z2 beta_source(const GemmArgs& g, const z2* E, const z2* Ci, int row,
                                           int col) {
   const z2 c = Ci[(int64_t)row * g.ldcin + col];
-  if (!g.E) return c;
-  const z2* E = g.E + e * g.sE;
+  if (!E) return c;
   const z2 a = E[(int64_t)row * g.lde + col], b = E[(int64_t)col * g.lde + row];
   return make_double2((a.x - b.x) + c.x, (a.y + b.y) + c.y);
 }
@@ -138,10 +161,11 @@ void zgemm_kernel(GemmArgs g) {
   const int64_t e = id / tiles;
   const int t = id - (int)(e * tiles);
   const int m0 = (t / g.tilesN) * BM, n0 = (t % g.tilesN) * BN;
-  const z2* __restrict__ A = g.A + e * g.sA;
-  const z2* __restrict__ B = g.B + e * g.sB;
-  z2* C = g.C + e * g.sC;
-  const z2* Ci = g.Cin + e * g.sCin;
+  const GemmPtrs P = gemm_ptrs(g, e);
+  const z2* __restrict__ A = P.A;
+  const z2* __restrict__ B = P.B;
+  z2* C = P.C;
+  const z2* Ci = P.Cin;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w / WN, wn = w % WN;
@@ -299,12 +323,12 @@ void zgemm_kernel(GemmArgs g) {
           double outr = fma(g.ar, vr, -(g.ai * vi)), outi = fma(g.ar, vi, g.ai * vr);
           z2* dst = C + (int64_t)row * g.ldc + col;
           if (!g.beta_zero) {
-            const z2 c = beta_source(g, e, Ci, row, col);
+            const z2 c = beta_source(g, P.E, Ci, row, col);
             outr += fma(g.br, c.x, -(g.bi * c.y));
             outi += fma(g.br, c.y, g.bi * c.x);
           }
           *dst = make_double2(outr, outi);
-          if (g.Ct) g.Ct[e * g.sCt + (int64_t)col * g.ldct + row] = make_double2(-outr, outi);
+          if (P.Ct) P.Ct[(int64_t)col * g.ldct + row] = make_double2(-outr, outi);
         }
       }
 }
@@ -347,8 +371,9 @@ void zgemm_reg_kernel(GemmArgs g) {
   const int m0 = (t / g.tilesN) * BM, n0 = (t % g.tilesN) * BN;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wm = w >> 1, wn = w & 1;
-  const z2* __restrict__ A = g.A + e * g.sA;
-  const z2* __restrict__ B = g.B + e * g.sB;
+  const GemmPtrs P = gemm_ptrs(g, e);
+  const z2* __restrict__ A = P.A;
+  const z2* __restrict__ B = P.B;
   const int r16 = lane & 15, k4 = lane >> 4;
 
   // pa[i] / pb[j]: this lane's element at k = 0 (k index k4 added below)
@@ -483,8 +508,8 @@ void zgemm_reg_kernel(GemmArgs g) {
     }
   }
 
-  z2* C = g.C + e * g.sC;
-  const z2* Ci = g.Cin + e * g.sCin;
+  z2* C = P.C;
+  const z2* Ci = P.Cin;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -506,12 +531,12 @@ void zgemm_reg_kernel(GemmArgs g) {
           // explicit fma, as in zgemm_kernel
           double outr = fma(g.ar, vr, -(g.ai * vi)), outi = fma(g.ar, vi, g.ai * vr);
           if (!g.beta_zero) {
-            const z2 c = beta_source(g, e, Ci, row, col);
+            const z2 c = beta_source(g, P.E, Ci, row, col);
             outr += fma(g.br, c.x, -(g.bi * c.y));
             outi += fma(g.br, c.y, g.bi * c.x);
           }
           C[(int64_t)row * g.ldc + col] = make_double2(outr, outi);
-          if (g.Ct) g.Ct[e * g.sCt + (int64_t)col * g.ldct + row] = make_double2(-outr, outi);
+          if (P.Ct) P.Ct[(int64_t)col * g.ldct + row] = make_double2(-outr, outi);
         }
       }
 }
@@ -658,7 +683,8 @@ void launch_typed(hipStream_t s, int64_t batch, int opA, int opB, const GemmArgs
   // would spill): that combination stays on the 64 x 64 shape.
   constexpr bool big_ok = !(sizeof(T) == 8 && SPLIT == 3);
   if constexpr (big_ok) {
-    if (gemm_tile(g.m, g.n, batch) == 128) {
+    // the tile of one product alone, also in a paired launch
+    if (gemm_tile(g.m, g.n, g.half) == 128) {
       launch_shape<T, SPLIT, 128, 128, 4, 2>(s, batch, opA, opB, g);
       return;
     }
@@ -689,41 +715,73 @@ void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
 
 namespace {
 
-void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
-                Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
-                const Mat& C, int precision, const Mat& Ct, const Mat& E) {
-  QX_REQUIRE(!C.zero(), "zgemm: output is a zero matrix");
-  QX_REQUIRE(E.zero() || (m == n && !Cin.zero() && E.p != C.p && Cin.p != C.p),
+void check_operands(int m, int n, const GemmOperands& o) {
+  QX_REQUIRE(!o.C.zero(), "zgemm: output is a zero matrix");
+  QX_REQUIRE(o.E.zero() || (m == n && !o.Cin.zero() && o.E.p != o.C.p && o.Cin.p != o.C.p),
              "zgemm: the D + E - E^H beta term needs m == n, a non-zero D and an output "
              "aliasing neither");
-  QX_REQUIRE(Ct.zero() || (Ct.p != C.p && Ct.p != A.p && Ct.p != B.p && Ct.p != Cin.p),
+  QX_REQUIRE(o.Ct.zero() || (o.Ct.p != o.C.p && o.Ct.p != o.A.p && o.Ct.p != o.B.p &&
+                             o.Ct.p != o.Cin.p),
              "zgemm: the transposed output aliases an operand");
+}
+
+// Operands of two products can share one launch: same presence, matrix
+// stride and leading dimension.
+bool same_layout(const Mat& a, const Mat& b) {
+  return a.zero() == b.zero() && (a.zero() || (a.stride == b.stride && a.ld == b.ld));
+}
+
+// An output of one product is no operand of the other.
+bool independent(const GemmOperands& x, const GemmOperands& y) {
+  for (const z2* w : {x.C.p, x.Ct.p}) {
+    if (!w) continue;
+    for (const z2* r : {y.A.p, y.B.p, y.Cin.p, y.E.p, y.C.p, y.Ct.p})
+      if (r == w) return false;
+  }
+  return true;
+}
+
+// One launch of batch products (y null) or of two such groups of products
+// (x, y: 2 batch matrices).
+void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+                Cplx alpha, Cplx beta, const GemmOperands& x, const GemmOperands* y,
+                int precision) {
+  check_operands(m, n, x);
+  if (y) check_operands(m, n, *y);
   QX_REQUIRE(opA == QX_OP_N || opA == QX_OP_C, "zgemm: bad op_a");
   QX_REQUIRE(opB == QX_OP_N || opB == QX_OP_C, "zgemm: bad op_b");
   QX_REQUIRE(precision == QX_PREC_C128 || precision == QX_PREC_C64 ||
                  precision == QX_PREC_C128_3M,
              "zgemm: precision must be QX_PREC_C128, QX_PREC_C64 or QX_PREC_C128_3M");
   if (batch <= 0 || m <= 0 || n <= 0) return;
-  const bool beta_zero = (beta.re == 0.0 && beta.im == 0.0) || Cin.zero();
-  if (A.zero() || B.zero() || k <= 0) {
-    // op(A) op(B) == 0: C = beta Cin (Cin + E - E^H).
-    if (!E.zero() && !beta_zero) {
-      ew_herm_add(s, batch, n, Cin, 1.0, E, C);
-      if (beta.re != 1.0 || beta.im != 0.0)
-        ew_axpby(s, batch, m, n, beta, C, Cplx{0, 0}, Mat(), C);
-    } else {
-      ew_axpby(s, batch, m, n, beta_zero ? Cplx{0, 0} : beta, Cin, Cplx{0, 0}, Mat(), C);
+  if (y) {
+    // otherwise x, then y, as two launches in that order
+    const bool one_launch = independent(x, *y) && independent(*y, x) && !x.A.zero() &&
+                            !x.B.zero() && k > 0 && same_layout(x.A, y->A) &&
+                            same_layout(x.B, y->B) && same_layout(x.Cin, y->Cin) &&
+                            same_layout(x.C, y->C) && same_layout(x.Ct, y->Ct) &&
+                            same_layout(x.E, y->E);
+    if (!one_launch) {
+      zgemm_impl(s, batch, m, n, k, opA, opB, alpha, beta, x, nullptr, precision);
+      zgemm_impl(s, batch, m, n, k, opA, opB, alpha, beta, *y, nullptr, precision);
+      return;
     }
-    if (!Ct.zero()) ew_neg_h(s, batch, m, n, C, Ct);
+  }
+  const bool beta_zero = (beta.re == 0.0 && beta.im == 0.0) || x.Cin.zero();
+  if (x.A.zero() || x.B.zero() || k <= 0) {
+    // op(A) op(B) == 0: C = beta Cin (Cin + E - E^H).
+    if (!x.E.zero() && !beta_zero) {
+      ew_herm_add(s, batch, n, x.Cin, 1.0, x.E, x.C);
+      if (beta.re != 1.0 || beta.im != 0.0)
+        ew_axpby(s, batch, m, n, beta, x.C, Cplx{0, 0}, Mat(), x.C);
+    } else {
+      ew_axpby(s, batch, m, n, beta_zero ? Cplx{0, 0} : beta, x.Cin, Cplx{0, 0}, Mat(), x.C);
+    }
+    if (!x.Ct.zero()) ew_neg_h(s, batch, m, n, x.C, x.Ct);
     return;
   }
+  const GemmOperands& o2 = y ? *y : x;
   GemmArgs g;
-  g.E = E.p;
-  g.sE = E.stride;
-  g.lde = E.ld;
-  g.Ct = Ct.p;
-  g.sCt = Ct.stride;
-  g.ldct = Ct.ld;
   g.m = m;
   g.n = n;
   g.k = k;
@@ -731,30 +789,45 @@ void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int 
   g.ai = alpha.im;
   g.br = beta.re;
   g.bi = beta.im;
-  g.A = A.p;
-  g.sA = A.stride;
-  g.lda = A.ld;
-  g.B = B.p;
-  g.sB = B.stride;
-  g.ldb = B.ld;
-  g.C = C.p;
-  g.sC = C.stride;
-  g.ldc = C.ld;
-  g.Cin = Cin.zero() ? C.p : Cin.p;
-  g.sCin = Cin.zero() ? C.stride : Cin.stride;
-  g.ldcin = Cin.zero() ? C.ld : Cin.ld;
+  g.A = x.A.p;
+  g.sA = x.A.stride;
+  g.lda = x.A.ld;
+  g.B = x.B.p;
+  g.sB = x.B.stride;
+  g.ldb = x.B.ld;
+  g.C = x.C.p;
+  g.sC = x.C.stride;
+  g.ldc = x.C.ld;
+  const Mat& ci = x.Cin.zero() ? x.C : x.Cin;
+  g.Cin = ci.p;
+  g.sCin = ci.stride;
+  g.ldcin = ci.ld;
   g.beta_zero = beta_zero ? 1 : 0;
+  g.Ct = x.Ct.p;
+  g.sCt = x.Ct.stride;
+  g.ldct = x.Ct.ld;
+  g.E = x.E.p;
+  g.sE = x.E.stride;
+  g.lde = x.E.ld;
+  g.half = batch;
+  g.A2 = o2.A.p;
+  g.B2 = o2.B.p;
+  g.C2 = o2.C.p;
+  g.Cin2 = o2.Cin.zero() ? o2.C.p : o2.Cin.p;
+  g.Ct2 = o2.Ct.p;
+  g.E2 = o2.E.p;
+  const int64_t nb = y ? 2 * batch : batch;
   const int split = gemm_split(precision);
   if (precision == QX_PREC_C64) {
     if (split == 3)
-      launch_typed<float, 3>(s, batch, opA, opB, g);
+      launch_typed<float, 3>(s, nb, opA, opB, g);
     else
-      launch_typed<float, 4>(s, batch, opA, opB, g);
+      launch_typed<float, 4>(s, nb, opA, opB, g);
   } else {
     if (split == 3)
-      launch_typed<double, 3>(s, batch, opA, opB, g);
+      launch_typed<double, 3>(s, nb, opA, opB, g);
     else
-      launch_typed<double, 4>(s, batch, opA, opB, g);
+      launch_typed<double, 4>(s, nb, opA, opB, g);
   }
 }
 
@@ -763,13 +836,21 @@ void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int 
 void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
               Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
               const Mat& C, int precision, const Mat& Ct) {
-  zgemm_impl(s, batch, m, n, k, opA, opB, alpha, A, B, beta, Cin, C, precision, Ct, Mat());
+  zgemm_impl(s, batch, m, n, k, opA, opB, alpha, beta, GemmOperands{A, B, Cin, C, Ct, Mat()},
+             nullptr, precision);
 }
 
 void zgemm_hin(hipStream_t s, int64_t batch, int n, int k, int opA, int opB, Cplx alpha,
                const Mat& A, const Mat& B, Cplx beta, const Mat& D, const Mat& E, const Mat& C,
                int precision) {
-  zgemm_impl(s, batch, n, n, k, opA, opB, alpha, A, B, beta, D, C, precision, Mat(), E);
+  zgemm_impl(s, batch, n, n, k, opA, opB, alpha, beta, GemmOperands{A, B, D, C, Mat(), E},
+             nullptr, precision);
+}
+
+void zgemm_pair(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+                Cplx alpha, Cplx beta, const GemmOperands& x, const GemmOperands& y,
+                int precision) {
+  zgemm_impl(s, batch, m, n, k, opA, opB, alpha, beta, x, &y, precision);
 }
 
 }  // namespace qx

@@ -120,6 +120,20 @@ void zgemm_in(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int op
 void zgemm_hin(hipStream_t s, int64_t batch, int n, int k, int opA, int opB, Cplx alpha,
                const Mat& A, const Mat& B, Cplx beta, const Mat& D, const Mat& E, const Mat& C,
                int precision);
+// Operands of one product C = alpha op(A) op(B) + beta (Cin + E - E^H), E
+// optional (zero Mat), with the optional second output Ct = -C^H.
+struct GemmOperands {
+  Mat A, B, Cin, C, Ct, E;
+};
+// Two products of one shape, ops, alpha and beta (x and y, batch matrices
+// each) as a single launch of 2 batch matrices when every operand of x has
+// the presence, matrix stride and leading dimension of its counterpart in y
+// (an operand shared by both is passed twice) and neither product reads or
+// writes an output of the other; otherwise x, then y, as two launches.  Each
+// element is computed as by the single-product call.
+void zgemm_pair(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+                Cplx alpha, Cplx beta, const GemmOperands& x, const GemmOperands& y,
+                int precision);
 
 // Ozaki-split GEMM on bf16 matrix cores: C = alpha op(A) op(B) + beta Cin
 // from `slices` 7-bit digit slices per operand (2..7); ws holds the digits.

@@ -129,7 +129,7 @@ int block_limits(const qx_btd_layout* la, const char* what) {
 size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve) {
   int bmax = 1;
   for (int i = 0; i < la->n_blocks; ++i) bmax = std::max(bmax, (int)la->block_sizes[i]);
-  const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 22 : (int64_t)la->n_blocks + 10;  // see slot maps
+  const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 26 : (int64_t)la->n_blocks + 10;  // see slot maps
   size_t bytes = (size_t)(slots * la->n_stack * (int64_t)bmax * bmax) * sizeof(z2);
   // the fused small-block kernels keep 6 N + 16 (solve) or 2 N + 3 (inverse)
   // slots of 32 x 32 per energy
@@ -258,6 +258,8 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   // (QX_RGF_STAGGER=0, read per call, starts all together).
   const char* stg_env = std::getenv("QX_RGF_STAGGER");
   const bool stagger = !(stg_env && stg_env[0] == '0');
+  const char* pair_env = std::getenv("QX_RGF_PAIR");
+  const bool pair_on = !(pair_env && pair_env[0] == '0');
   hipEvent_t stg_ev = nullptr;
   if (stagger) QX_HIP_CHECK(hipEventCreateWithFlags(&stg_ev, hipEventDisableTiming));
   std::promise<void> stg_set;
@@ -307,6 +309,10 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
             C1s = 3 * N + 11, C2s = 3 * N + 12, C3s = 3 * N + 13, C4s = 3 * N + 14;
   const InvScratch scr = W.inv(3 * N + 15);
   const int NS1 = 3 * N + 20, NS2 = 3 * N + 21;  // Newton-Schulz temporaries
+  // G^> temporaries, so that each G^< product and its G^> counterpart can run
+  // as one launch (QX_RGF_PAIR=0, read per call: one launch each)
+  const int T3g = 3 * N + 22, T4g = 3 * N + 23, T5g = 3 * N + 24, T6g = 3 * N + 25;
+  int spare_g = T6g;
 
   struct Q {
     Btd* sig;
@@ -314,9 +320,33 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     std::vector<int>* slot;
     const Obc* obc;
     bool skip_lower;
+    int T3, T4, T5, T6;  // temporaries of this quantity
+    int* spare;          // free slot of the backward sweep, rotated with slot
   };
-  Q qs[2] = {{&SL, &XL, &sl_, &OL, (flags & QX_SOLVE_SKIP_LOWER_L) != 0},
-             {&SG, &XG, &sg_, &OG, (flags & QX_SOLVE_SKIP_LOWER_G) != 0}};
+  Q qs[2] = {{&SL, &XL, &sl_, &OL, (flags & QX_SOLVE_SKIP_LOWER_L) != 0, T3s, T4s, T5s, T6s,
+              &spare},
+             {&SG, &XG, &sg_, &OG, (flags & QX_SOLVE_SKIP_LOWER_G) != 0, T3g, T4g, T5g, T6g,
+              &spare_g}};
+  // prod(h0, h1, ...): the G^< product x (if h0) and its G^> counterpart y
+  // (if h1); one launch when both are wanted and pairing is on
+  auto one = [&](int m, int n, int k, int opA, int opB, Cplx alpha, Cplx beta,
+                 const GemmOperands& o) {
+    if (o.E.zero())
+      zgemm_in(st, S, m, n, k, opA, opB, alpha, o.A, o.B, beta, o.Cin, o.C, gp, o.Ct);
+    else
+      zgemm_hin(st, S, n, k, opA, opB, alpha, o.A, o.B, beta, o.Cin, o.E, o.C, gp);
+  };
+  auto prod = [&](bool h0, bool h1, int m, int n, int k, int opA, int opB, Cplx alpha,
+                  Cplx beta, const GemmOperands& x, const GemmOperands& y) {
+    if (h0 && h1 && pair_on) {
+      zgemm_pair(st, S, m, n, k, opA, opB, alpha, beta, x, y, gp);
+      return;
+    }
+    if (h0) one(m, n, k, opA, opB, alpha, beta, x);
+    if (h1) one(m, n, k, opA, opB, alpha, beta, y);
+  };
+  auto slot = [&](int qi, int blk, int b) { return W.get((*qs[qi].slot)[blk], b); };
+  const Mat none;
 
   // ---- block 0 (rgf.py:207-231) ----
   {
@@ -330,12 +360,18 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       release.p = nullptr;
     }
     if (corner) ew_axpby(st, S, b0, b0, kOne, X0, kZero, Mat(), W.get(gc_cur, bmax));
-    for (auto& q : qs) {
-      const Mat T3 = W.get(T3s, b0), T6 = W.get(T6s, b0);
-      fold(st, S, b0, b0, q.sig->diag(0), 1.0, q.obc->get(0, b0), T3);
-      zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_N, kOne, X0, T3, kZero, T6, gp);
-      zgemm(st, S, b0, b0, b0, QX_OP_N, QX_OP_C, kOne, T6, X0, kZero, W.get((*q.slot)[0], b0), gp);
+    Mat T3[2], T6[2], X[2];
+    for (int qi = 0; qi < 2; ++qi) {
+      const Q& q = qs[qi];
+      T3[qi] = W.get(q.T3, b0);
+      T6[qi] = W.get(q.T6, b0);
+      X[qi] = slot(qi, 0, b0);
+      fold(st, S, b0, b0, q.sig->diag(0), 1.0, q.obc->get(0, b0), T3[qi]);
     }
+    prod(true, true, b0, b0, b0, QX_OP_N, QX_OP_N, kOne, kZero,
+         {X0, T3[0], T6[0], T6[0], none, none}, {X0, T3[1], T6[1], T6[1], none, none});
+    prod(true, true, b0, b0, b0, QX_OP_N, QX_OP_C, kOne, kZero,
+         {T6[0], X0, X[0], X[0], none, none}, {T6[1], X0, X[1], X[1], none, none});
   }
 
   // ---- forward sweep (rgf.py:234-295) ----
@@ -363,23 +399,37 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
             W.get(gc_nxt, bmax), gp);
       std::swap(gc_cur, gc_nxt);
     }
-    for (auto& q : qs) {
-      const Mat T3 = W.get(T3s, bj), T4 = W.get(T4s, bi), T5 = W.get(T5s, bj), T6 = W.get(T6s, bj);
+    {
       // sigma_jj + obc + a_ji x_ii a_ji^H + u^H - u,  u = a_ji xr_ii sigma_ij
-      Mat Sjj = q.sig->diag(j);
-      if (!q.obc->get(j, bj).zero()) {
-        fold(st, S, bj, bj, q.sig->diag(j), 1.0, q.obc->get(j, bj), T3);
-        Sjj = T3;
+      Mat Sjj[2], T3[2], T4[2], T5[2], T6[2], Xi[2], Xj[2], sij[2];
+      for (int qi = 0; qi < 2; ++qi) {
+        const Q& q = qs[qi];
+        T3[qi] = W.get(q.T3, bj);
+        T4[qi] = W.get(q.T4, bi);
+        T5[qi] = W.get(q.T5, bj);
+        T6[qi] = W.get(q.T6, bj);
+        Xi[qi] = slot(qi, i, bi);
+        Xj[qi] = slot(qi, j, bj);
+        sij[qi] = q.sig->upper(i);
+        Sjj[qi] = q.sig->diag(j);
+        if (!q.obc->get(j, bj).zero()) {
+          fold(st, S, bj, bj, q.sig->diag(j), 1.0, q.obc->get(j, bj), T3[qi]);
+          Sjj[qi] = T3[qi];
+        }
       }
-      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), W.get((*q.slot)[i], bi), kZero, T4, gp);
-      zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, T4, A.lower(i), kOne, Sjj, T3, gp);
-      const Mat sij = q.sig->upper(i);
-      if (!sij.zero()) {
-        zgemm(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kOne, T1, sij, kZero, T5, gp);
-        ew_add_hdiff(st, S, bj, T5, T3);
-      }
-      zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_N, kOne, Xrj, T3, kZero, T6, gp);
-      zgemm(st, S, bj, bj, bj, QX_OP_N, QX_OP_C, kOne, T6, Xrj, kZero, W.get((*q.slot)[j], bj), gp);
+      const Mat Aji = A.lower(i);
+      prod(true, true, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, kZero,
+           {Aji, Xi[0], T4[0], T4[0], none, none}, {Aji, Xi[1], T4[1], T4[1], none, none});
+      prod(true, true, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, kOne,
+           {T4[0], Aji, Sjj[0], T3[0], none, none}, {T4[1], Aji, Sjj[1], T3[1], none, none});
+      prod(!sij[0].zero(), !sij[1].zero(), bj, bj, bi, QX_OP_N, QX_OP_N, kOne, kZero,
+           {T1, sij[0], T5[0], T5[0], none, none}, {T1, sij[1], T5[1], T5[1], none, none});
+      for (int qi = 0; qi < 2; ++qi)
+        if (!sij[qi].zero()) ew_add_hdiff(st, S, bj, T5[qi], T3[qi]);
+      prod(true, true, bj, bj, bj, QX_OP_N, QX_OP_N, kOne, kZero,
+           {Xrj, T3[0], T6[0], T6[0], none, none}, {Xrj, T3[1], T6[1], T6[1], none, none});
+      prod(true, true, bj, bj, bj, QX_OP_N, QX_OP_C, kOne, kZero,
+           {T6[0], Xrj, Xj[0], Xj[0], none, none}, {T6[1], Xrj, Xj[1], Xj[1], none, none});
     }
   }
 
@@ -435,33 +485,50 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
                  cur + i, N - 1);
     }
 
-    for (auto& q : qs) {
-      const Mat Xi = W.get((*q.slot)[i], bi), Xj = W.get((*q.slot)[j], bj);
-      const Mat T1 = W.get(T1s, bi), T5 = W.get(T5s, bj);
-      const Mat sij = q.sig->upper(i);
-      // temp_1x = B4 x_ii - xr_ii s_ij B3^H
-      zgemm(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, B4, Xi, kZero, T1, gp);
-      if (!sij.zero()) {
-        zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, sij, kZero, T5, gp);
-        zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, T5, B3, kOne, T1, gp);
+    {
+      // G^< temporaries T1, T5, T4 (x_ij when the output has no upper
+      // block); G^> ones in its own slots
+      Mat Xi[2], Xj[2], T1[2], T5[2], Z[2], Oij[2], low[2], sij[2];
+      bool hs[2];
+      for (int qi = 0; qi < 2; ++qi) {
+        const Q& q = qs[qi];
+        Xi[qi] = slot(qi, i, bi);
+        Xj[qi] = slot(qi, j, bj);
+        T1[qi] = W.get(qi == 0 ? T1s : q.T3, bi);
+        T5[qi] = W.get(q.T5, bj);
+        Z[qi] = W.get(*q.spare, bi);
+        sij[qi] = q.sig->upper(i);
+        hs[qi] = !sij[qi].zero();
+        Oij[qi] = q.out->upper(i);
+        if (Oij[qi].zero()) Oij[qi] = W.get(q.T4, bj);
+        low[qi] = (!q.skip_lower) ? q.out->lower(i) : Mat();
       }
-      // x_ii' = x_ii + (temp_1x - temp_1x^H) + temp_2x B1^H, the first two
-      // terms formed in the epilogue of the product below
-      const Mat Z = W.get(spare, bi);
+      // temp_1x = B4 x_ii - xr_ii s_ij B3^H
+      prod(true, true, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, kZero,
+           {B4, Xi[0], T1[0], T1[0], none, none}, {B4, Xi[1], T1[1], T1[1], none, none});
+      prod(hs[0], hs[1], bi, bj, bi, QX_OP_N, QX_OP_N, kOne, kZero,
+           {Xri, sij[0], T5[0], T5[0], none, none}, {Xri, sij[1], T5[1], T5[1], none, none});
+      prod(hs[0], hs[1], bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, kOne,
+           {T5[0], B3, T1[0], T1[0], none, none}, {T5[1], B3, T1[1], T1[1], none, none});
       // x_ij = -temp_2x - x_ii B2^H + xr_ii s_ij xr_jj^H,  temp_2x = B1 x_jj
-      Mat Oij = q.out->upper(i);
-      if (Oij.zero()) Oij = W.get(T4s, bj);
-      zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, B1, Xj, kZero, Oij, gp);  // -temp_2x
-      zgemm_hin(st, S, bi, bj, QX_OP_N, QX_OP_C, kMinus, Oij, B1, kOne, Xi, T1, Z, gp);
+      prod(true, true, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, kZero,
+           {B1, Xj[0], Oij[0], Oij[0], none, none}, {B1, Xj[1], Oij[1], Oij[1], none, none});
+      // x_ii' = x_ii + (temp_1x - temp_1x^H) + temp_2x B1^H, the first two
+      // terms formed in the epilogue of this product
+      prod(true, true, bi, bi, bj, QX_OP_N, QX_OP_C, kMinus, kOne,
+           {Oij[0], B1, Xi[0], Z[0], none, T1[0]}, {Oij[1], B1, Xi[1], Z[1], none, T1[1]});
       // the GEMM that completes x_ij also writes the lower block -x_ij^H
-      const Mat low = (!q.skip_lower) ? q.out->lower(i) : Mat();
-      zgemm_in(st, S, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, Xi, B2, kOne, Oij, Oij, gp,
-               sij.zero() ? low : Mat());
-      if (!sij.zero())
-        zgemm_in(st, S, bi, bj, bj, QX_OP_N, QX_OP_C, kOne, T5, Xrj, kOne, Oij, Oij, gp, low);
-      std::swap(spare, (*q.slot)[i]);
-      if (!q.out->diag(i).zero()) ew_antiherm(st, S, bi, 0.5, Z, q.out->diag(i));
-      if (dens && &q == &qs[0]) ew_diag_imag(st, S, bi, Z, dens + orb_off[i], n_orb);
+      prod(true, true, bi, bj, bi, QX_OP_N, QX_OP_C, kMinus, kOne,
+           {Xi[0], B2, Oij[0], Oij[0], hs[0] ? none : low[0], none},
+           {Xi[1], B2, Oij[1], Oij[1], hs[1] ? none : low[1], none});
+      prod(hs[0], hs[1], bi, bj, bj, QX_OP_N, QX_OP_C, kOne, kOne,
+           {T5[0], Xrj, Oij[0], Oij[0], low[0], none}, {T5[1], Xrj, Oij[1], Oij[1], low[1], none});
+      for (int qi = 0; qi < 2; ++qi) {
+        const Q& q = qs[qi];
+        std::swap(*q.spare, (*q.slot)[i]);
+        if (!q.out->diag(i).zero()) ew_antiherm(st, S, bi, 0.5, Z[qi], q.out->diag(i));
+        if (dens && qi == 0) ew_diag_imag(st, S, bi, Z[qi], dens + orb_off[i], n_orb);
+      }
     }
 
     // retarded: xr_ii <- xr_ii + B4 xr_ii ; off-diagonal blocks on request

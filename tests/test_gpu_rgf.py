@@ -216,6 +216,62 @@ def test_sub_batch_offset_changes_only_timing(device, monkeypatch):
         assert np.array_equal(x1.view(np.uint64), x0.view(np.uint64))
 
 
+@pytest.mark.parametrize("precision", ["c128", "c128-3m", "mixed"])
+@pytest.mark.parametrize("sigma_pattern", ["shared", "distinct"])
+def test_paired_lesser_greater_products_bitwise(device, monkeypatch, precision, sigma_pattern):
+    """Each G^< product and its G^> counterpart run as one launch
+    (QX_RGF_PAIR); every element is computed as by the separate launches, so
+    the outputs are bitwise those of QX_RGF_PAIR=0.  Mixed block sizes above
+    the fused-kernel limit, Σ with off-diagonal blocks, boundary blocks on
+    both ends, the current and the off-diagonal G^R blocks; complex128 also
+    against the oracle.  Σ^< and Σ^> share one sparsity pattern (their
+    products pair) or not (different nnz layouts: two launches each)."""
+    bs = np.array([48, 64, 64, 40, 64])
+    n = len(bs)
+    dense = bt_dense(bs, seed=3)
+    coo = sps.coo_matrix(dense)
+    oa = ODSDBCOO.from_sparray(coo, bs, (6,))
+    rng = np.random.default_rng(11)
+    s = sps.coo_matrix(bt_dense(bs, seed=4))
+    s = sps.coo_matrix(s - s.conj().T)
+    if sigma_pattern == "shared":
+        vals = rng.standard_normal(s.nnz) + 1j * rng.standard_normal(s.nnz)
+        s2 = sps.coo_matrix((vals, (s.row, s.col)), shape=s.shape)
+    else:
+        s2 = sps.coo_matrix(bt_dense(bs, seed=5))
+        s2 = sps.coo_matrix(s2 - s2.conj().T)
+    sig = [ODSDBCOO.from_sparray(m, bs, (6,)) for m in (s, s2)]
+    assert (sig[0].data.shape == sig[1].data.shape) == (sigma_pattern == "shared")
+    def block(b, scale=0.05):
+        return scale * (rng.standard_normal((6, b, b)) + 1j * rng.standard_normal((6, b, b)))
+    g0, g1 = block(bs[0]), block(bs[-1])
+    obc = {"retarded": {0: block(bs[0]), n - 1: block(bs[-1])},
+           "lesser": {0: 1j * (g0 @ g0.conj().transpose(0, 2, 1))},
+           "greater": {n - 1: -1j * (g1 @ g1.conj().transpose(0, 2, 1))}}
+    oobc, gobc = orgf.OBCBlocks(n), OBCBlocks(n)
+    for kind, blocks in obc.items():
+        for i, v in blocks.items():
+            getattr(oobc, kind)[i] = v
+            getattr(gobc, kind)[i] = torch.as_tensor(v, device=device)
+    outs = {}
+    for pair in ("1", "0"):
+        monkeypatch.setenv("QX_RGF_PAIR", pair)
+        a, sl, sg = (to_gpu(o, device) for o in (oa, *sig))
+        res = RGF(4, precision=precision, retarded_offdiag=True).selected_solve(
+            a, sl, sg, obc_blocks=gobc, return_retarded=True, return_current=True)
+        outs[pair] = [t.data.cpu().numpy() if hasattr(t, "data") and not torch.is_tensor(t)
+                      else t.cpu().numpy() for t in res]
+    for x1, x0 in zip(outs["1"], outs["0"]):
+        assert np.array_equal(x1.view(np.uint64), x0.view(np.uint64))
+    if precision == "c128":
+        refs = orgf.RGF(4).selected_solve(oa, *sig, obc_blocks=oobc, return_retarded=True,
+                                          return_current=True, retarded_offdiag=True)
+        for got, ref in zip(outs["1"][:3], refs[:3]):
+            scale = np.max(np.abs(ref.data))
+            assert np.allclose(got, ref.data, rtol=1e-10, atol=1e-10 * scale)
+        assert rel_err(outs["1"][3], refs[3]) < 1e-10
+
+
 def test_workload_device_assembly_matches(device, workload):
     """Workload.assemble (device, dense-block layout) equals the oracle containers."""
     w, (oa, osl, osg), _ = workload
