@@ -327,8 +327,9 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
               &spare},
              {&SG, &XG, &sg_, &OG, (flags & QX_SOLVE_SKIP_LOWER_G) != 0, T3g, T4g, T5g, T6g,
               &spare_g}};
-  // prod(h0, h1, ...): the G^< product x (if h0) and its G^> counterpart y
-  // (if h1); one launch when both are wanted and pairing is on
+  // prod(h0, h1, ...): product x (if h0) and product y (if h1) of one shape,
+  // typically a G^< product and its G^> counterpart; one launch when both
+  // are wanted and pairing is on
   auto one = [&](int m, int n, int k, int opA, int opB, Cplx alpha, Cplx beta,
                  const GemmOperands& o) {
     if (o.E.zero())
@@ -458,10 +459,19 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const Mat Xri = W.get(sr[i], bi), Xrj = gr[j];
     const Mat Aij = A.upper(i), Aji = A.lower(i);
     const Mat B1 = W.get(B1s, bj), B2 = W.get(B2s, bi), B3 = W.get(B3s, bj), B4 = W.get(B4s, bi);
-    zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, Aij, kZero, B1, gp);  // xr_ii a_ij
-    zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xrj, Aji, kZero, B2, gp);  // xr_jj a_ji
-    zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kOne, B1, Xrj, kZero, B3, gp);   // xr_ii a_ij xr_jj
-    zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kOne, B1, B2, kZero, B4, gp);    // ... a_ji
+    // B1 = xr_ii a_ij, B2 = xr_jj a_ji, B3 = B1 xr_jj (xr_ii a_ij xr_jj),
+    // B4 = B1 B2 (... a_ji): two pairs of one shape when bi == bj
+    if (bi == bj) {
+      prod(true, true, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, kZero,
+           {Xri, Aij, B1, B1, none, none}, {Xrj, Aji, B2, B2, none, none});
+      prod(true, true, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, kZero,
+           {B1, Xrj, B3, B3, none, none}, {B1, B2, B4, B4, none, none});
+    } else {
+      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xri, Aij, kZero, B1, gp);
+      zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xrj, Aji, kZero, B2, gp);
+      zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kOne, B1, Xrj, kZero, B3, gp);
+      zgemm(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kOne, B1, B2, kZero, B4, gp);
+    }
 
     if (want_cur) {
       // Meir-Wingreen layer current from the forward (left-connected) blocks
