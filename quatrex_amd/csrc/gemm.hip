@@ -76,39 +76,49 @@ struct GemmArgs {
   int64_t sE;
   int lde;
   // Two products of one shape in one launch: matrices e >= half take every
-  // operand from the second base (A2 ...) at index e - half.  A single
-  // product sets half = batch and the second bases equal to the first.
+  // operand from the second set (base, matrix stride, leading dimension) at
+  // index e - half.  A single product sets half = batch and the second set
+  // equal to the first.
   int64_t half;
   const z2 *A2, *B2, *Cin2, *E2;
   z2 *C2, *Ct2;
+  int64_t sA2, sB2, sC2, sCin2, sCt2, sE2;
+  int lda2, ldb2, ldc2, ldcin2, ldct2, lde2;
 };
 
-// Operand bases of matrix e of a launch (see GemmArgs::half).
+// Operands of matrix e of a launch (see GemmArgs::half); uniform over a
+// workgroup.
 struct GemmPtrs {
   const z2 *A, *B, *Cin, *E;
   z2 *C, *Ct;
+  int lda, ldb, ldc, ldcin, ldct, lde;
 };
 __device__ __forceinline__ GemmPtrs gemm_ptrs(const GemmArgs& g, int64_t e) {
   const bool second = e >= g.half;
   const int64_t l = second ? e - g.half : e;
   GemmPtrs p;
-  p.A = (second ? g.A2 : g.A) + l * g.sA;
-  p.B = (second ? g.B2 : g.B) + l * g.sB;
-  p.C = (second ? g.C2 : g.C) + l * g.sC;
-  p.Cin = (second ? g.Cin2 : g.Cin) + l * g.sCin;
-  p.E = g.E ? (second ? g.E2 : g.E) + l * g.sE : nullptr;
-  p.Ct = g.Ct ? (second ? g.Ct2 : g.Ct) + l * g.sCt : nullptr;
+  p.A = second ? g.A2 + l * g.sA2 : g.A + l * g.sA;
+  p.B = second ? g.B2 + l * g.sB2 : g.B + l * g.sB;
+  p.C = second ? g.C2 + l * g.sC2 : g.C + l * g.sC;
+  p.Cin = second ? g.Cin2 + l * g.sCin2 : g.Cin + l * g.sCin;
+  p.E = g.E ? (second ? g.E2 + l * g.sE2 : g.E + l * g.sE) : nullptr;
+  p.Ct = g.Ct ? (second ? g.Ct2 + l * g.sCt2 : g.Ct + l * g.sCt) : nullptr;
+  p.lda = second ? g.lda2 : g.lda;
+  p.ldb = second ? g.ldb2 : g.ldb;
+  p.ldc = second ? g.ldc2 : g.ldc;
+  p.ldcin = second ? g.ldcin2 : g.ldcin;
+  p.ldct = second ? g.ldct2 : g.ldct;
+  p.lde = second ? g.lde2 : g.lde;
   return p;
 }
 
 // beta-term source of element (row, col) of matrix e: Cin, or Cin + E - E^H
 // with the additions in the order of the Hermitian-part kernel
 // ((E - E^H) + Cin), so that the fused form rounds like the separate pass.
-__device__ __forceinline__ z2 beta_source(const GemmArgs& g, const z2* E, const z2* Ci, int row,
-                                          int col) {
-  const z2 c = Ci[(int64_t)row * g.ldcin + col];
-  if (!E) return c;
-  const z2 a = E[(int64_t)row * g.lde + col], b = E[(int64_t)col * g.lde + row];
+__device__ __forceinline__ z2 beta_source(const GemmPtrs& P, int row, int col) {
+  const z2 c = P.Cin[(int64_t)row * P.ldcin + col];
+  if (!P.E) return c;
+  const z2 a = P.E[(int64_t)row * P.lde + col], b = P.E[(int64_t)col * P.lde + row];
   return make_double2((a.x - b.x) + c.x, (a.y + b.y) + c.y);
 }
 
@@ -165,7 +175,6 @@ void zgemm_kernel(GemmArgs g) {
   const z2* __restrict__ A = P.A;
   const z2* __restrict__ B = P.B;
   z2* C = P.C;
-  const z2* Ci = P.Cin;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w / WN, wn = w % WN;
@@ -192,12 +201,12 @@ void zgemm_kernel(GemmArgs g) {
         const int row = idx / BK, col = idx % BK;
         const int gm = m0 + row, gk = k0 + col;
         ra[r] = zero;
-        if (gm < g.m && gk < g.k) ra[r] = A[(int64_t)gm * g.lda + gk];
+        if (gm < g.m && gk < g.k) ra[r] = A[(int64_t)gm * P.lda + gk];
       } else {  // A stored k x m, op = conj transpose
         const int kk = idx / BM, mm = idx % BM;
         const int gm = m0 + mm, gk = k0 + kk;
         ra[r] = zero;
-        if (gm < g.m && gk < g.k) ra[r] = A[(int64_t)gk * g.lda + gm];
+        if (gm < g.m && gk < g.k) ra[r] = A[(int64_t)gk * P.lda + gm];
       }
     }
 #pragma unroll
@@ -207,12 +216,12 @@ void zgemm_kernel(GemmArgs g) {
         const int kk = idx / BN, nn = idx % BN;
         const int gn = n0 + nn, gk = k0 + kk;
         rb[r] = zero;
-        if (gn < g.n && gk < g.k) rb[r] = B[(int64_t)gk * g.ldb + gn];
+        if (gn < g.n && gk < g.k) rb[r] = B[(int64_t)gk * P.ldb + gn];
       } else {  // B stored n x k, op = conj transpose
         const int nn = idx / BK, kk = idx % BK;
         const int gn = n0 + nn, gk = k0 + kk;
         rb[r] = zero;
-        if (gn < g.n && gk < g.k) rb[r] = B[(int64_t)gn * g.ldb + gk];
+        if (gn < g.n && gk < g.k) rb[r] = B[(int64_t)gn * P.ldb + gk];
       }
     }
   };
@@ -321,14 +330,14 @@ void zgemm_kernel(GemmArgs g) {
           }
           // explicit fma: the same rounding whatever the compiler contracts
           double outr = fma(g.ar, vr, -(g.ai * vi)), outi = fma(g.ar, vi, g.ai * vr);
-          z2* dst = C + (int64_t)row * g.ldc + col;
+          z2* dst = C + (int64_t)row * P.ldc + col;
           if (!g.beta_zero) {
-            const z2 c = beta_source(g, P.E, Ci, row, col);
+            const z2 c = beta_source(P, row, col);
             outr += fma(g.br, c.x, -(g.bi * c.y));
             outi += fma(g.br, c.y, g.bi * c.x);
           }
           *dst = make_double2(outr, outi);
-          if (P.Ct) P.Ct[(int64_t)col * g.ldct + row] = make_double2(-outr, outi);
+          if (P.Ct) P.Ct[(int64_t)col * P.ldct + row] = make_double2(-outr, outi);
         }
       }
 }
@@ -383,15 +392,15 @@ void zgemm_reg_kernel(GemmArgs g) {
   for (int i = 0; i < TM; ++i) {
     int row = m0 + wm * 16 * TM + i * 16 + r16;
     if (CHECK) row = min(row, g.m - 1);
-    pa[i] = CA ? A + row : A + (int64_t)row * g.lda;
+    pa[i] = CA ? A + row : A + (int64_t)row * P.lda;
   }
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
     int col = n0 + wn * 16 * TN + j * 16 + r16;
     if (CHECK) col = min(col, g.n - 1);
-    pb[j] = CB ? B + (int64_t)col * g.ldb : B + col;
+    pb[j] = CB ? B + (int64_t)col * P.ldb : B + col;
   }
-  const int64_t ka = CA ? g.lda : 1, kb_ = CB ? 1 : g.ldb;  // element step per k
+  const int64_t ka = CA ? P.lda : 1, kb_ = CB ? 1 : P.ldb;  // element step per k
   const int nk = (g.k + 3) / 4;  // k-steps, the last one possibly partial
   // k-steps of the pipelined loop: whole groups of D full k-steps.  The rest
   // (CHECK only) runs in the tail loop below.
@@ -509,7 +518,6 @@ void zgemm_reg_kernel(GemmArgs g) {
   }
 
   z2* C = P.C;
-  const z2* Ci = P.Cin;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -531,12 +539,12 @@ void zgemm_reg_kernel(GemmArgs g) {
           // explicit fma, as in zgemm_kernel
           double outr = fma(g.ar, vr, -(g.ai * vi)), outi = fma(g.ar, vi, g.ai * vr);
           if (!g.beta_zero) {
-            const z2 c = beta_source(g, P.E, Ci, row, col);
+            const z2 c = beta_source(P, row, col);
             outr += fma(g.br, c.x, -(g.bi * c.y));
             outi += fma(g.br, c.y, g.bi * c.x);
           }
-          C[(int64_t)row * g.ldc + col] = make_double2(outr, outi);
-          if (P.Ct) P.Ct[(int64_t)col * g.ldct + row] = make_double2(-outr, outi);
+          C[(int64_t)row * P.ldc + col] = make_double2(outr, outi);
+          if (P.Ct) P.Ct[(int64_t)col * P.ldct + row] = make_double2(-outr, outi);
         }
       }
 }
@@ -725,11 +733,9 @@ void check_operands(int m, int n, const GemmOperands& o) {
              "zgemm: the transposed output aliases an operand");
 }
 
-// Operands of two products can share one launch: same presence, matrix
-// stride and leading dimension.
-bool same_layout(const Mat& a, const Mat& b) {
-  return a.zero() == b.zero() && (a.zero() || (a.stride == b.stride && a.ld == b.ld));
-}
+// Operands of two products can share one launch when both or neither are
+// present (strides and leading dimensions may differ).
+bool same_layout(const Mat& a, const Mat& b) { return a.zero() == b.zero(); }
 
 // An output of one product is no operand of the other.
 bool independent(const GemmOperands& x, const GemmOperands& y) {
@@ -811,11 +817,24 @@ void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int 
   g.lde = x.E.ld;
   g.half = batch;
   g.A2 = o2.A.p;
+  g.sA2 = o2.A.stride;
+  g.lda2 = o2.A.ld;
   g.B2 = o2.B.p;
+  g.sB2 = o2.B.stride;
+  g.ldb2 = o2.B.ld;
   g.C2 = o2.C.p;
-  g.Cin2 = o2.Cin.zero() ? o2.C.p : o2.Cin.p;
+  g.sC2 = o2.C.stride;
+  g.ldc2 = o2.C.ld;
+  const Mat& ci2 = o2.Cin.zero() ? o2.C : o2.Cin;
+  g.Cin2 = ci2.p;
+  g.sCin2 = ci2.stride;
+  g.ldcin2 = ci2.ld;
   g.Ct2 = o2.Ct.p;
+  g.sCt2 = o2.Ct.stride;
+  g.ldct2 = o2.Ct.ld;
   g.E2 = o2.E.p;
+  g.sE2 = o2.E.stride;
+  g.lde2 = o2.E.ld;
   const int64_t nb = y ? 2 * batch : batch;
   const int split = gemm_split(precision);
   if (precision == QX_PREC_C64) {

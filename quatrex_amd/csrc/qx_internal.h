@@ -126,9 +126,10 @@ struct GemmOperands {
   Mat A, B, Cin, C, Ct, E;
 };
 // Two products of one shape, ops, alpha and beta (x and y, batch matrices
-// each) as a single launch of 2 batch matrices when every operand of x has
-// the presence, matrix stride and leading dimension of its counterpart in y
-// (an operand shared by both is passed twice) and neither product reads or
+// each) as a single launch of 2 batch matrices when every operand of x is
+// present exactly when its counterpart in y is (strides and leading
+// dimensions may differ; an operand shared by both is passed twice), op(A)
+// op(B) is not identically zero, and neither product reads or
 // writes an output of the other; otherwise x, then y, as two launches.  Each
 // element is computed as by the single-product call.
 void zgemm_pair(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
