@@ -81,22 +81,41 @@ struct HermArgs {
   const z2* D;
   int64_t sD;
   int ldd;
+  // Two groups of matrices in one launch: matrices e >= half use the second
+  // A, B and Y (index e - half); D is single-group only.  One group sets
+  // half = batch and the second set equal to the first.
+  int64_t half;
+  const z2* A2;
+  int64_t sA2;
+  int lda2;
+  const z2* B2;
+  int64_t sB2;
+  int ldb2;
+  z2* Y2;
+  int64_t sY2;
+  int ldy2;
 };
 
 __global__ __launch_bounds__(NT) void herm_kernel(HermArgs g) {
   __shared__ z2 tile[TS][TS + 1];
   const int tiles = g.tm * g.tn;
-  const int64_t e = blockIdx.x / tiles;
-  const int t = blockIdx.x - (int)(e * tiles);
+  const int64_t eg = blockIdx.x / tiles;
+  const int t = blockIdx.x - (int)(eg * tiles);
+  const bool second = eg >= g.half;
+  const int64_t e = second ? eg - g.half : eg;
+  const z2* A = second ? g.A2 + e * g.sA2 : (g.A ? g.A + e * g.sA : nullptr);
+  const z2* B = second ? g.B2 + e * g.sB2 : (g.B ? g.B + e * g.sB : nullptr);
+  z2* Y = second ? g.Y2 + e * g.sY2 : g.Y + e * g.sY;
+  const int lda = second ? g.lda2 : g.lda, ldb = second ? g.ldb2 : g.ldb,
+            ldy = second ? g.ldy2 : g.ldy;
   const int r0 = (t / g.tn) * TS, c0 = (t % g.tn) * TS;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   if (g.B) {
     // tile[j][i] = B[c0 + j][r0 + i]
-    const z2* B = g.B + e * g.sB;
     for (int j = ty; j < TS; j += 8) {
       const int br = c0 + j, bc = r0 + tx;
       z2 v = make_double2(0.0, 0.0);
-      if (br < g.n && bc < g.m) v = B[(int64_t)br * g.ldb + bc];
+      if (br < g.n && bc < g.m) v = B[(int64_t)br * ldb + bc];
       tile[j][tx] = v;
     }
   }
@@ -105,14 +124,14 @@ __global__ __launch_bounds__(NT) void herm_kernel(HermArgs g) {
     const int r = r0 + i, c = c0 + tx;
     if (r >= g.m || c >= g.n) continue;
     double yr = 0.0, yi = 0.0;
-    z2* y = g.Y + e * g.sY + (int64_t)r * g.ldy + c;
+    z2* y = Y + (int64_t)r * ldy + c;
     if (g.cy != 0.0) {
       const z2 v = *y;
       yr = g.cy * v.x;
       yi = g.cy * v.y;
     }
     if (g.A) {
-      const z2 a = g.A[e * g.sA + (int64_t)r * g.lda + c];
+      const z2 a = A[(int64_t)r * lda + c];
       yr += g.ca * a.x;
       yi += g.ca * a.y;
     }
@@ -130,14 +149,25 @@ __global__ __launch_bounds__(NT) void herm_kernel(HermArgs g) {
   }
 }
 
+// One launch over batch matrices, or over two groups of batch matrices when
+// A2 / B2 / Y2 are given (the second group's operands; D must then be zero).
 void herm(hipStream_t s, int64_t batch, int m, int n, double cy, double ca,
           const Mat& A, double cb, const Mat& B, const Mat& Y, double cc = 0.0,
-          const Mat& D = Mat()) {
+          const Mat& D = Mat(), const Mat* A2 = nullptr, const Mat* B2 = nullptr,
+          const Mat* Y2 = nullptr) {
   if (batch <= 0 || m <= 0 || n <= 0) return;
+  const bool two = Y2 != nullptr;
+  QX_REQUIRE(!two || (D.zero() && A2 && B2 && A2->zero() == A.zero() &&
+                      B2->zero() == B.zero() && !Y2->zero()),
+             "herm: a two-group launch needs matching operands and no D term");
+  const Mat& a2 = two ? *A2 : A;
+  const Mat& b2 = two ? *B2 : B;
+  const Mat& y2 = two ? *Y2 : Y;
   HermArgs g{m, n, cy, ca, cb, A.p, A.stride, A.ld, B.p, B.stride, B.ld,
              Y.p, Y.stride, Y.ld, (m + TS - 1) / TS, (n + TS - 1) / TS,
-             cc, D.p, D.stride, D.ld};
-  const int64_t blocks = batch * g.tm * g.tn;
+             cc, D.p, D.stride, D.ld, batch,
+             a2.p, a2.stride, a2.ld, b2.p, b2.stride, b2.ld, y2.p, y2.stride, y2.ld};
+  const int64_t blocks = (two ? 2 : 1) * batch * g.tm * g.tn;
   QX_REQUIRE(blocks < (int64_t(1) << 31), "herm: grid too large");
   hipLaunchKernelGGL(herm_kernel, dim3((unsigned)blocks), dim3(NT), 0, s, g);
   QX_HIP_CHECK(hipGetLastError());
@@ -422,6 +452,18 @@ void ew_antiherm(hipStream_t s, int64_t batch, int n, double scale,
                  const Mat& X, const Mat& Y) {
   QX_REQUIRE(X.p != Y.p, "antiherm: in-place use is not supported");
   herm(s, batch, n, n, 0.0, scale, X, -scale, X, Y);
+}
+
+void ew_antiherm_pair(hipStream_t s, int64_t batch, int n, double scale, const Mat& X0,
+                      const Mat& Y0, const Mat& X1, const Mat& Y1) {
+  QX_REQUIRE(X0.p != Y0.p && X1.p != Y1.p, "antiherm: in-place use is not supported");
+  if (Y0.p == Y1.p || Y0.p == X1.p || Y1.p == X0.p) {
+    // one group's output is the other's operand: in order
+    ew_antiherm(s, batch, n, scale, X0, Y0);
+    ew_antiherm(s, batch, n, scale, X1, Y1);
+    return;
+  }
+  herm(s, batch, n, n, 0.0, scale, X0, -scale, X0, Y0, 0.0, Mat(), &X1, &X1, &Y1);
 }
 
 void ew_herm_add(hipStream_t s, int64_t batch, int n, const Mat& D, double scale,

@@ -184,6 +184,10 @@ void ew_herm_add(hipStream_t s, int64_t batch, int n, const Mat& D, double scale
                  const Mat& X, const Mat& Y);
 void ew_antiherm(hipStream_t s, int64_t batch, int n, double scale,
                  const Mat& X, const Mat& Y);
+// Y0 = scale (X0 - X0^H) and Y1 = scale (X1 - X1^H) as one launch (in order,
+// as two launches, if an output of one is an operand of the other).
+void ew_antiherm_pair(hipStream_t s, int64_t batch, int n, double scale, const Mat& X0,
+                      const Mat& Y0, const Mat& X1, const Mat& Y1);
 // Y += T^H - T             (square)
 void ew_add_hdiff(hipStream_t s, int64_t batch, int n, const Mat& T,
                   const Mat& Y);

@@ -442,9 +442,14 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   {
     const int bn = bs(N - 1);
     if (dens) ew_diag_imag(st, S, bn, W.get(sl_[N - 1], bn), dens + orb_off[N - 1], n_orb);
-    for (auto& q : qs)
-      if (!q.out->diag(N - 1).zero())
-        ew_antiherm(st, S, bn, 0.5, W.get((*q.slot)[N - 1], bn), q.out->diag(N - 1));
+    const Mat d0 = qs[0].out->diag(N - 1), d1 = qs[1].out->diag(N - 1);
+    if (pair_on && !d0.zero() && !d1.zero()) {
+      ew_antiherm_pair(st, S, bn, 0.5, slot(0, N - 1, bn), d0, slot(1, N - 1, bn), d1);
+    } else {
+      for (auto& q : qs)
+        if (!q.out->diag(N - 1).zero())
+          ew_antiherm(st, S, bn, 0.5, W.get((*q.slot)[N - 1], bn), q.out->diag(N - 1));
+    }
     if (want_r && !XR.diag(N - 1).zero())
       ew_axpby(st, S, bn, bn, kOne, W.get(sr[N - 1], bn), kZero, Mat(), XR.diag(N - 1));
   }
@@ -533,10 +538,13 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
            {Xi[1], B2, Oij[1], Oij[1], hs[1] ? none : low[1], none});
       prod(hs[0], hs[1], bi, bj, bj, QX_OP_N, QX_OP_C, kOne, kOne,
            {T5[0], Xrj, Oij[0], Oij[0], low[0], none}, {T5[1], Xrj, Oij[1], Oij[1], low[1], none});
+      const Mat d0 = qs[0].out->diag(i), d1 = qs[1].out->diag(i);
+      const bool proj2 = pair_on && !d0.zero() && !d1.zero();
+      if (proj2) ew_antiherm_pair(st, S, bi, 0.5, Z[0], d0, Z[1], d1);
       for (int qi = 0; qi < 2; ++qi) {
         const Q& q = qs[qi];
         std::swap(*q.spare, (*q.slot)[i]);
-        if (!q.out->diag(i).zero()) ew_antiherm(st, S, bi, 0.5, Z[qi], q.out->diag(i));
+        if (!proj2 && !q.out->diag(i).zero()) ew_antiherm(st, S, bi, 0.5, Z[qi], q.out->diag(i));
         if (dens && qi == 0) ew_diag_imag(st, S, bi, Z[qi], dens + orb_off[i], n_orb);
       }
     }
