@@ -157,6 +157,9 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   const Btd A0(la, a), X0(lx, x);
 
   const int gp = gemm_precision(prec);
+  // QX_RGF_PAIR=0 (read per call): one launch per backward product
+  const char* pair_env = std::getenv("QX_RGF_PAIR");
+  const bool pair_on = !(pair_env && pair_env[0] == '0');
   run_split(h, st0, Stot, [&](hipStream_t st, int64_t e0, int64_t S) {
     const Btd A = A0.sub(e0), X = X0.sub(e0);
     const Slots W{ws, Stot, (int64_t)bmax * bmax, e0};
@@ -198,15 +201,29 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       const int j = i + 1, bi = bs(i), bj = bs(j);
       const Mat Xii = W.get(xd[i], bi), Xjj = W.get(xd[j], bj);
       const Mat T1 = W.get(t1, bi), T2 = W.get(t2, bj);
-      // x_ji = -x_jj a_ji x_ii
-      zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xjj, A.lower(i), kZero, T1, gp);
       Mat Xji = X.lower(i);
       if (Xji.zero()) Xji = W.get(t3, bi);
-      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, T1, Xii, kZero, Xji, gp);
-      // x_ij = -x_ii a_ij x_jj
-      zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xii, A.upper(i), kZero, T2, gp);
-      if (!X.upper(i).zero())
-        zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xjj, kZero, X.upper(i), gp);
+      const Mat Xij = X.upper(i);
+      if (pair_on && bi == bj) {
+        // the x_ji and x_ij chains below, each product pair as one launch
+        const Mat none;
+        zgemm_pair(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kOne, kZero,
+                   {Xjj, A.lower(i), T1, T1, none, none}, {Xii, A.upper(i), T2, T2, none, none},
+                   gp);
+        if (!Xij.zero())
+          zgemm_pair(st, S, bi, bi, bi, QX_OP_N, QX_OP_N, kMinus, kZero,
+                     {T1, Xii, Xji, Xji, none, none}, {T2, Xjj, Xij, Xij, none, none}, gp);
+        else
+          zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, T1, Xii, kZero, Xji, gp);
+      } else {
+        // x_ji = -x_jj a_ji x_ii
+        zgemm(st, S, bj, bi, bj, QX_OP_N, QX_OP_N, kOne, Xjj, A.lower(i), kZero, T1, gp);
+        zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kMinus, T1, Xii, kZero, Xji, gp);
+        // x_ij = -x_ii a_ij x_jj
+        zgemm(st, S, bi, bj, bi, QX_OP_N, QX_OP_N, kOne, Xii, A.upper(i), kZero, T2, gp);
+        if (!Xij.zero())
+          zgemm(st, S, bi, bj, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xjj, kZero, Xij, gp);
+      }
       // x_ii <- x_ii - x_ii a_ij x_ji
       const Mat Z = W.get(spare, bi);
       zgemm_in(st, S, bi, bi, bj, QX_OP_N, QX_OP_N, kMinus, T2, Xji, kOne, Xii, Z, gp);

@@ -272,6 +272,36 @@ def test_paired_lesser_greater_products_bitwise(device, monkeypatch, precision, 
         assert rel_err(outs["1"][3], refs[3]) < 1e-10
 
 
+@pytest.mark.parametrize("precision", ["c128", "c128-3m", "mixed"])
+def test_paired_selected_inv_bitwise(device, monkeypatch, precision):
+    """selected_inv runs x_jj a_ji with x_ii a_ij, and the two products that
+    complete x_ji and x_ij, as one launch each where neighbouring blocks are
+    of equal size (QX_RGF_PAIR); the outputs are bitwise those of
+    QX_RGF_PAIR=0.  Mixed sizes (pairs and single launches), retarded
+    boundary blocks; complex128 also against the oracle."""
+    bs = np.array([48, 64, 64, 40, 40, 64])
+    n = len(bs)
+    oa = ODSDBCOO.from_sparray(sps.coo_matrix(bt_dense(bs, seed=6)), bs, (5,))
+    rng = np.random.default_rng(12)
+    obc = {0: 0.05 * (rng.standard_normal((5, bs[0], bs[0])) + 1j * rng.standard_normal((5, bs[0], bs[0]))),
+           n - 1: 0.05 * (rng.standard_normal((5, bs[-1], bs[-1]))
+                          + 1j * rng.standard_normal((5, bs[-1], bs[-1])))}
+    oobc, gobc = orgf.OBCBlocks(n), OBCBlocks(n)
+    for i, v in obc.items():
+        oobc.retarded[i] = v
+        gobc.retarded[i] = torch.as_tensor(v, device=device)
+    outs = {}
+    for pair in ("1", "0"):
+        monkeypatch.setenv("QX_RGF_PAIR", pair)
+        x = RGF(5, precision=precision).selected_inv(to_gpu(oa, device), obc_blocks=gobc)
+        outs[pair] = x.data.cpu().numpy()
+    assert np.array_equal(outs["1"].view(np.uint64), outs["0"].view(np.uint64))
+    if precision == "c128":
+        ref = orgf.RGF(5).selected_inv(oa, obc_blocks=oobc)
+        scale = np.max(np.abs(ref.data))
+        assert np.allclose(outs["1"], ref.data, rtol=1e-10, atol=1e-10 * scale)
+
+
 def test_workload_device_assembly_matches(device, workload):
     """Workload.assemble (device, dense-block layout) equals the oracle containers."""
     w, (oa, osl, osg), _ = workload
