@@ -75,15 +75,19 @@ struct GemmArgs {
   const z2* E;
   int64_t sE;
   int lde;
-  // Two products of one shape in one launch: matrices e >= half take every
-  // operand from the second set (base, matrix stride, leading dimension) at
-  // index e - half.  A single product sets half = batch and the second set
-  // equal to the first.
+  // Up to three products of one shape in one launch: matrices half <= e <
+  // 2 half take every operand from the second set (base, matrix stride,
+  // leading dimension) at index e - half, matrices e >= 2 half from the
+  // third at e - 2 half.  Sets a launch does not use equal the first.
   int64_t half;
   const z2 *A2, *B2, *Cin2, *E2;
   z2 *C2, *Ct2;
   int64_t sA2, sB2, sC2, sCin2, sCt2, sE2;
   int lda2, ldb2, ldc2, ldcin2, ldct2, lde2;
+  const z2 *A3, *B3, *Cin3, *E3;
+  z2 *C3, *Ct3;
+  int64_t sA3, sB3, sC3, sCin3, sCt3, sE3;
+  int lda3, ldb3, ldc3, ldcin3, ldct3, lde3;
 };
 
 // Operands of matrix e of a launch (see GemmArgs::half); uniform over a
@@ -94,21 +98,24 @@ struct GemmPtrs {
   int lda, ldb, ldc, ldcin, ldct, lde;
 };
 __device__ __forceinline__ GemmPtrs gemm_ptrs(const GemmArgs& g, int64_t e) {
-  const bool second = e >= g.half;
-  const int64_t l = second ? e - g.half : e;
+  const int grp = e >= 2 * g.half ? 2 : (e >= g.half ? 1 : 0);
+  const int64_t l = e - grp * g.half;
+  // field of the group's set
+#define QX_SEL(f1, f2, f3) (grp == 2 ? (f3) : grp == 1 ? (f2) : (f1))
   GemmPtrs p;
-  p.A = second ? g.A2 + l * g.sA2 : g.A + l * g.sA;
-  p.B = second ? g.B2 + l * g.sB2 : g.B + l * g.sB;
-  p.C = second ? g.C2 + l * g.sC2 : g.C + l * g.sC;
-  p.Cin = second ? g.Cin2 + l * g.sCin2 : g.Cin + l * g.sCin;
-  p.E = g.E ? (second ? g.E2 + l * g.sE2 : g.E + l * g.sE) : nullptr;
-  p.Ct = g.Ct ? (second ? g.Ct2 + l * g.sCt2 : g.Ct + l * g.sCt) : nullptr;
-  p.lda = second ? g.lda2 : g.lda;
-  p.ldb = second ? g.ldb2 : g.ldb;
-  p.ldc = second ? g.ldc2 : g.ldc;
-  p.ldcin = second ? g.ldcin2 : g.ldcin;
-  p.ldct = second ? g.ldct2 : g.ldct;
-  p.lde = second ? g.lde2 : g.lde;
+  p.A = QX_SEL(g.A + l * g.sA, g.A2 + l * g.sA2, g.A3 + l * g.sA3);
+  p.B = QX_SEL(g.B + l * g.sB, g.B2 + l * g.sB2, g.B3 + l * g.sB3);
+  p.C = QX_SEL(g.C + l * g.sC, g.C2 + l * g.sC2, g.C3 + l * g.sC3);
+  p.Cin = QX_SEL(g.Cin + l * g.sCin, g.Cin2 + l * g.sCin2, g.Cin3 + l * g.sCin3);
+  p.E = g.E ? QX_SEL(g.E + l * g.sE, g.E2 + l * g.sE2, g.E3 + l * g.sE3) : nullptr;
+  p.Ct = g.Ct ? QX_SEL(g.Ct + l * g.sCt, g.Ct2 + l * g.sCt2, g.Ct3 + l * g.sCt3) : nullptr;
+  p.lda = QX_SEL(g.lda, g.lda2, g.lda3);
+  p.ldb = QX_SEL(g.ldb, g.ldb2, g.ldb3);
+  p.ldc = QX_SEL(g.ldc, g.ldc2, g.ldc3);
+  p.ldcin = QX_SEL(g.ldcin, g.ldcin2, g.ldcin3);
+  p.ldct = QX_SEL(g.ldct, g.ldct2, g.ldct3);
+  p.lde = QX_SEL(g.lde, g.lde2, g.lde3);
+#undef QX_SEL
   return p;
 }
 
@@ -747,29 +754,36 @@ bool independent(const GemmOperands& x, const GemmOperands& y) {
   return true;
 }
 
-// One launch of batch products (y null) or of two such groups of products
-// (x, y: 2 batch matrices).
+// One launch of batch products (y, z null) or of two or three such groups
+// of products (x, y[, z]: 2 or 3 batch matrices).
 void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
                 Cplx alpha, Cplx beta, const GemmOperands& x, const GemmOperands* y,
-                int precision) {
-  check_operands(m, n, x);
-  if (y) check_operands(m, n, *y);
+                int precision, const GemmOperands* z = nullptr) {
+  QX_REQUIRE(!z || y, "zgemm: a third group needs a second");
+  const GemmOperands* gs[3] = {&x, y, z};
+  const int ng = z ? 3 : (y ? 2 : 1);
+  for (int a = 0; a < ng; ++a) check_operands(m, n, *gs[a]);
   QX_REQUIRE(opA == QX_OP_N || opA == QX_OP_C, "zgemm: bad op_a");
   QX_REQUIRE(opB == QX_OP_N || opB == QX_OP_C, "zgemm: bad op_b");
   QX_REQUIRE(precision == QX_PREC_C128 || precision == QX_PREC_C64 ||
                  precision == QX_PREC_C128_3M,
              "zgemm: precision must be QX_PREC_C128, QX_PREC_C64 or QX_PREC_C128_3M");
   if (batch <= 0 || m <= 0 || n <= 0) return;
-  if (y) {
-    // otherwise x, then y, as two launches in that order
-    const bool one_launch = independent(x, *y) && independent(*y, x) && !x.A.zero() &&
-                            !x.B.zero() && k > 0 && same_layout(x.A, y->A) &&
-                            same_layout(x.B, y->B) && same_layout(x.Cin, y->Cin) &&
-                            same_layout(x.C, y->C) && same_layout(x.Ct, y->Ct) &&
-                            same_layout(x.E, y->E);
+  if (ng > 1) {
+    // otherwise x, y[, z] as separate launches in that order
+    bool one_launch = !x.A.zero() && !x.B.zero() && k > 0;
+    for (int a = 0; a < ng; ++a)
+      for (int b = 0; b < ng; ++b)
+        if (a != b) one_launch = one_launch && independent(*gs[a], *gs[b]);
+    for (int a = 1; a < ng; ++a) {
+      const GemmOperands& o = *gs[a];
+      one_launch = one_launch && same_layout(x.A, o.A) && same_layout(x.B, o.B) &&
+                   same_layout(x.Cin, o.Cin) && same_layout(x.C, o.C) &&
+                   same_layout(x.Ct, o.Ct) && same_layout(x.E, o.E);
+    }
     if (!one_launch) {
-      zgemm_impl(s, batch, m, n, k, opA, opB, alpha, beta, x, nullptr, precision);
-      zgemm_impl(s, batch, m, n, k, opA, opB, alpha, beta, *y, nullptr, precision);
+      for (int a = 0; a < ng; ++a)
+        zgemm_impl(s, batch, m, n, k, opA, opB, alpha, beta, *gs[a], nullptr, precision);
       return;
     }
   }
@@ -835,7 +849,27 @@ void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int 
   g.E2 = o2.E.p;
   g.sE2 = o2.E.stride;
   g.lde2 = o2.E.ld;
-  const int64_t nb = y ? 2 * batch : batch;
+  const GemmOperands& o3 = z ? *z : x;
+  g.A3 = o3.A.p;
+  g.sA3 = o3.A.stride;
+  g.lda3 = o3.A.ld;
+  g.B3 = o3.B.p;
+  g.sB3 = o3.B.stride;
+  g.ldb3 = o3.B.ld;
+  g.C3 = o3.C.p;
+  g.sC3 = o3.C.stride;
+  g.ldc3 = o3.C.ld;
+  const Mat& ci3 = o3.Cin.zero() ? o3.C : o3.Cin;
+  g.Cin3 = ci3.p;
+  g.sCin3 = ci3.stride;
+  g.ldcin3 = ci3.ld;
+  g.Ct3 = o3.Ct.p;
+  g.sCt3 = o3.Ct.stride;
+  g.ldct3 = o3.Ct.ld;
+  g.E3 = o3.E.p;
+  g.sE3 = o3.E.stride;
+  g.lde3 = o3.E.ld;
+  const int64_t nb = ng * batch;
   const int split = gemm_split(precision);
   if (precision == QX_PREC_C64) {
     if (split == 3)
@@ -870,6 +904,12 @@ void zgemm_pair(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int 
                 Cplx alpha, Cplx beta, const GemmOperands& x, const GemmOperands& y,
                 int precision) {
   zgemm_impl(s, batch, m, n, k, opA, opB, alpha, beta, x, &y, precision);
+}
+
+void zgemm_triple(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+                  Cplx alpha, Cplx beta, const GemmOperands& x, const GemmOperands& y,
+                  const GemmOperands& z, int precision) {
+  zgemm_impl(s, batch, m, n, k, opA, opB, alpha, beta, x, &y, precision, &z);
 }
 
 }  // namespace qx

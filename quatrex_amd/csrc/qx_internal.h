@@ -135,6 +135,10 @@ struct GemmOperands {
 void zgemm_pair(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
                 Cplx alpha, Cplx beta, const GemmOperands& x, const GemmOperands& y,
                 int precision);
+// Three products the same way (x, y, z: 3 batch matrices in one launch).
+void zgemm_triple(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
+                  Cplx alpha, Cplx beta, const GemmOperands& x, const GemmOperands& y,
+                  const GemmOperands& z, int precision);
 
 // Ozaki-split GEMM on bf16 matrix cores: C = alpha op(A) op(B) + beta Cin
 // from `slices` 7-bit digit slices per operand (2..7); ws holds the digits.

@@ -397,8 +397,19 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const int j = i + 1, bi = bs(i), bj = bs(j);
     const Mat Xri = W.get(sr[i], bi), Xrj = W.get(sr[j], bj);
     const Mat T1 = W.get(T1s, bi);
-    // a_ji xr_ii
-    zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), Xri, kZero, T1, gp);
+    // a_ji xr_ii; with pairing on, in one launch with a_ji x^<_ii and
+    // a_ji x^>_ii of the Σ side below (inputs of the previous step only)
+    const bool aji3 = pair_on;
+    if (aji3) {
+      const Mat Aji = A.lower(i), none;
+      const Mat t4l = W.get(qs[0].T4, bi), t4g = W.get(qs[1].T4, bi);
+      const Mat xl = slot(0, i, bi), xg = slot(1, i, bi);
+      zgemm_triple(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, kZero,
+                   {Aji, Xri, T1, T1, none, none}, {Aji, xl, t4l, t4l, none, none},
+                   {Aji, xg, t4g, t4g, none, none}, gp);
+    } else {
+      zgemm(st, S, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, A.lower(i), Xri, kZero, T1, gp);
+    }
     // xr_jj = inv(a_jj - obc - a_ji xr_ii a_ij), the argument formed where
     // the inverse starts
     const Mat Mj = zinv_input(bj, Xrj, scr);
@@ -436,8 +447,9 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
         }
       }
       const Mat Aji = A.lower(i);
-      prod(true, true, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, kZero,
-           {Aji, Xi[0], T4[0], T4[0], none, none}, {Aji, Xi[1], T4[1], T4[1], none, none});
+      if (!aji3)
+        prod(true, true, bj, bi, bi, QX_OP_N, QX_OP_N, kOne, kZero,
+             {Aji, Xi[0], T4[0], T4[0], none, none}, {Aji, Xi[1], T4[1], T4[1], none, none});
       prod(true, true, bj, bj, bi, QX_OP_N, QX_OP_C, kOne, kOne,
            {T4[0], Aji, Sjj[0], T3[0], none, none}, {T4[1], Aji, Sjj[1], T3[1], none, none});
       prod(!sij[0].zero(), !sij[1].zero(), bj, bj, bi, QX_OP_N, QX_OP_N, kOne, kZero,
