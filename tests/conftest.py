@@ -55,3 +55,33 @@ def rel_err(x, y):
     y = np.asarray(y)
     den = np.linalg.norm(y)
     return np.linalg.norm(x - y) / (den if den > 0 else 1.0)
+
+
+def oracle_btd(w, energies=None):
+    """(A, Sigma^<, Sigma^>) of a workload as oracle DSDBCOO stacks built block
+    by block (no dense matrix), for the energies given (default: all)."""
+    from oracle.dsdbcoo import DSDBCOO as ODSDBCOO
+    from quatrex_amd.datastructures import btd_pattern
+
+    e = w.energies if energies is None else np.asarray(energies)
+    bs = w.block_sizes
+    rows, cols, aoff = btd_pattern(bs, offdiag=True)
+    drows, dcols, doff = btd_pattern(bs, offdiag=False)
+    diag, upper, lower = w.a_blocks(e)
+    ne = len(e)
+    a = np.empty((ne, len(rows)), complex)
+    sl = np.empty((ne, len(drows)), complex)
+    sg = np.empty((ne, len(drows)), complex)
+    for i, b in enumerate(bs):
+        o = aoff[(i, i)]
+        a[:, o:o + b * b] = diag[i].reshape(ne, -1)
+        o = doff[(i, i)]
+        sl[:, o:o + b * b] = w.sl_diag[i].reshape(1, -1)
+        sg[:, o:o + b * b] = w.sg_diag[i].reshape(1, -1)
+        if i + 1 < len(bs):
+            o = aoff[(i, i + 1)]
+            a[:, o:o + b * bs[i + 1]] = upper[i].reshape(ne, -1)
+            o = aoff[(i + 1, i)]
+            a[:, o:o + b * bs[i + 1]] = lower[i].reshape(ne, -1)
+    mk = lambda d, r, c: ODSDBCOO(d, r, c, bs, (ne,))  # noqa: E731
+    return mk(a, rows, cols), mk(sl, drows, dcols), mk(sg, drows, dcols)

@@ -1,0 +1,211 @@
+"""Parity at the production shapes of the benchmark configurations.
+
+- Single-device production shape: 64 blocks of 256 (2 energies here), every
+  arithmetic against the NumPy oracle (rgf.py:115-414): per block of G^<,
+  G^> (all three block diagonals) and of diag G^R, the density Im diag G^<
+  and the transmission Tr[Gamma_L G^R_{0,N-1} Gamma_R G^A_{N-1,0}].
+- Open boundary at lead layer 256: Sancho-Rubio (sancho_rubio.py:33-86) on 4
+  energies and the Beyn contour moments (beyn.py:177-235) with 64 nodes on
+  the annuli r_o in {1.5, 2, 3}, r_i = 1 / r_o.
+- Uneven mix: one device per block size in {64, 128, 256, 416} in one
+  batch on concurrent lanes (eager and graph replay) against direct RGF
+  solves, and one energy of each device against the oracle.
+- Serving over the same mix: each request's result is bitwise the same
+  alone and in the mix.
+
+Bars (each written in its test):
+- complex128 (4M and 3M): 1e-10 relative Frobenius per block, density,
+  transmission.
+- mixed (complex64 products + FP64 Newton-Schulz inverses): 5e-2 per block,
+  2e-3 density, 2e-4 transmission.  At eta = 1e-3 and 64 blocks the complex64
+  product error (~1e-7) is amplified ~3e3 by the recursion; measured at this
+  shape: worst G^< block 8e-3, density 3.4e-4, transmission 1.5e-5.
+- Sancho-Rubio at b = 256: 2e-9 per energy.  This is the conditioning floor
+  of the oracle itself: the same oracle run with 1 instead of 8 BLAS threads,
+  or with a_ii perturbed by one unit in the last place, moves its result by
+  up to 4.0e-10 (median 9e-11) over 32 energies of this lead, and its own
+  fixed-point residual ||x - inv(a_ii - a_ji x a_ij)|| / ||x|| reaches
+  3.4e-10.  The same bound holds for the GPU result's fixed-point residual.
+- Beyn moments: 1e-10 (the oracle's own spread under a BLAS thread change is
+  5e-13 at these radii).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.nevp as onevp
+import oracle.obc as oobc
+import oracle.rgf as orgf
+from quatrex_amd.boundary_conditions import SanchoRubio
+from quatrex_amd.datastructures import DSDBCOO, empty_btd
+from quatrex_amd.engine import Device, UnevenBatch
+from quatrex_amd.greens_function_solver import RGF, transmission
+from quatrex_amd.nevp import Beyn
+from quatrex_amd.serving import Request, Server
+from quatrex_amd.workloads import lead_blocks, make_workload
+from tests.conftest import oracle_btd, rel_err
+
+pytestmark = pytest.mark.gpu
+
+N_PROD, B_PROD = 64, 256
+# (per-block, density, transmission) bars per arithmetic
+BARS = {"c128": (1e-10, 1e-10, 1e-10), "c128-3m": (1e-10, 1e-10, 1e-10),
+        "mixed": (5e-2, 2e-3, 2e-4)}
+
+
+def _gammas(w, ne):
+    gl = 1j * (w.sigma_b_left - w.sigma_b_left.conj().T)
+    gr = 1j * (w.sigma_b_right - w.sigma_b_right.conj().T)
+    return (np.broadcast_to(gl, (ne,) + gl.shape), np.broadcast_to(gr, (ne,) + gr.shape))
+
+
+@pytest.fixture(scope="module")
+def production():
+    """2 energies of the production shape and the oracle's outputs."""
+    w = make_workload(seed=7, n_energies=2, n_blocks=N_PROD, block_size=B_PROD)
+    oa, osl, osg = oracle_btd(w)
+    corner = np.zeros((2, B_PROD, B_PROD), complex)
+    rl, rg, rr = orgf.RGF(2).selected_solve(oa, osl, osg, return_retarded=True, corner=corner)
+    dens = np.stack([np.diagonal(rl.blocks[i, i], axis1=-2, axis2=-1).imag
+                     for i in range(N_PROD)], 1).reshape(2, -1)
+    t = orgf.transmission(corner, *(g[0] for g in _gammas(w, 2)))
+    return w, (rl, rg, rr), dens, t
+
+
+@pytest.mark.parametrize("precision", list(BARS))
+def test_production_shape_against_oracle(device, production, precision):
+    w, (rl, rg, rr), dens_ref, t_ref = production
+    bar_block, bar_dens, bar_t = BARS[precision]
+    a, sl, sg = w.assemble(device)
+    xl, xg = DSDBCOO.zeros_like(a), DSDBCOO.zeros_like(a)
+    xr, _ = empty_btd(w.block_sizes, (2,), offdiag=False, device=device)
+    dens = torch.zeros((2, N_PROD * B_PROD), dtype=torch.float64, device=device)
+    corner = torch.zeros((2, B_PROD, B_PROD), dtype=torch.complex128, device=device)
+    RGF(2, precision=precision).selected_solve(a, sl, sg, out=(xl, xg, xr), return_retarded=True,
+                                               corner=corner, density=dens)
+    t = transmission(corner, *_gammas(w, 2)).cpu().numpy()
+    worst = {"lesser": 0.0, "greater": 0.0, "retarded_diag": 0.0}
+    for g, o, name in ((xl, rl, "lesser"), (xg, rg, "greater")):
+        for i in range(N_PROD):
+            for j in (i - 1, i, i + 1):
+                if 0 <= j < N_PROD:
+                    worst[name] = max(worst[name], rel_err(g.blocks[i, j].cpu().numpy(),
+                                                           o.blocks[i, j]))
+    for i in range(N_PROD):
+        worst["retarded_diag"] = max(worst["retarded_diag"],
+                                     rel_err(xr.blocks[i, i].cpu().numpy(), rr.blocks[i, i]))
+    d_err = rel_err(dens.cpu().numpy(), dens_ref)
+    t_err = float(np.max(np.abs(t - t_ref) / np.abs(t_ref)))
+    print(f"{precision}: worst blocks {worst}, density {d_err:.2e}, transmission {t_err:.2e}")
+    assert max(worst.values()) < bar_block
+    assert d_err < bar_dens
+    assert t_err < bar_t
+
+
+@pytest.fixture(scope="module")
+def lead256():
+    m_ji, m_ii, m_ij, _ = lead_blocks(seed=1, n_energies=4, block_size=256)
+    return m_ji, m_ii, m_ij
+
+
+def test_sancho_rubio_lead_layer_256(device, lead256):
+    m_ji, m_ii, m_ij = lead256
+    o = oobc.SanchoRubio(max_iterations=100, convergence_tol=1e-12)
+    ref = o(m_ii, m_ij, m_ji)
+    g = SanchoRubio(max_iterations=100, convergence_tol=1e-12)
+    x = g(*(torch.as_tensor(v, device=device) for v in (m_ii, m_ij, m_ji))).cpu().numpy()
+    assert g.iterations == o.iterations
+    err = np.linalg.norm(x - ref, axis=(-2, -1)) / np.linalg.norm(ref, axis=(-2, -1))
+    res = np.linalg.inv(m_ii - m_ji @ x @ m_ij) - x
+    res = np.linalg.norm(res, axis=(-2, -1)) / np.linalg.norm(x, axis=(-2, -1))
+    print(f"Sancho-Rubio b=256: error vs oracle {err}, fixed-point residual {res}")
+    assert err.max() < 2e-9
+    assert res.max() < 2e-9
+
+
+@pytest.mark.parametrize("r_o", [1.5, 2.0, 3.0])
+def test_beyn_moments_lead_layer_256(device, lead256, r_o):
+    m_ji, m_ii, m_ij = (v[:2] for v in lead256)
+    ob = onevp.Beyn(r_o=r_o, r_i=1 / r_o, m_0=32, num_quad_points=64)
+    q0r, q1r = ob._contour_integrate((m_ji, m_ii, m_ij))
+    gb = Beyn(r_o=r_o, r_i=1 / r_o, m_0=32, num_quad_points=64)
+    q0, q1 = gb._contour_integrate([torch.as_tensor(v, device=device) for v in (m_ji, m_ii, m_ij)])
+    assert rel_err(q0.cpu().numpy(), q0r) < 1e-10
+    assert rel_err(q1.cpu().numpy(), q1r) < 1e-10
+
+
+# one device per block size of the uneven mix: (b, N, energies, seed)
+MIX = [(64, 4, 2, 31), (128, 3, 2, 32), (256, 3, 2, 33), (416, 3, 2, 34)]
+
+
+def _direct(w, device):
+    a, sl, sg = w.assemble(device)
+    ne, n_orb = len(w.energies), int(np.sum(w.block_sizes))
+    dens = torch.zeros((ne, n_orb), dtype=torch.float64, device=device)
+    corner = torch.zeros((ne, w.block_sizes[0], w.block_sizes[-1]), dtype=torch.complex128,
+                         device=device)
+    RGF(ne).selected_solve(a, sl, sg, density=dens, corner=corner)
+    return dens.cpu().numpy(), transmission(corner, *_gammas(w, ne)).cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def mix(device):
+    ws = [make_workload(seed=s, n_energies=e, n_blocks=n, block_size=b) for b, n, e, s in MIX]
+    return ws, [_direct(w, device) for w in ws]
+
+
+def _check(devs, refs, tol):
+    for d, (dens, t) in zip(devs, refs):
+        assert np.allclose(d.density.cpu().numpy(), dens, rtol=tol, atol=tol * np.abs(dens).max())
+        assert np.allclose(d.transmission.cpu().numpy(), t, rtol=tol)
+
+
+def test_uneven_mix_block_sizes(device, mix):
+    ws, refs = mix
+    # the direct solves against the oracle on one energy of each device
+    for w, (dens, t) in zip(ws, refs):
+        oa, osl, osg = oracle_btd(w, w.energies[:1])
+        corner = np.zeros((1, w.block_sizes[0], w.block_sizes[-1]), complex)
+        rl = orgf.RGF(1).selected_solve(oa, osl, osg, corner=corner)[0]
+        od = np.concatenate([np.diagonal(rl.blocks[i, i], axis1=-2, axis2=-1).imag
+                             for i in range(w.n_blocks)], -1)[0]
+        ot = orgf.transmission(corner, *(g[0] for g in _gammas(w, 1)))
+        assert rel_err(dens[0], od) < 1e-10
+        assert abs(t[0] - ot[0]) / abs(ot[0]) < 1e-10
+    devs = [Device(w, device, f"b{w.block_sizes[0]}") for w in ws]
+    ub = UnevenBatch(devs, n_lanes=4, default_chunk=2)
+    ub.run_eager()
+    ub.check()
+    _check(devs, refs, 1e-12)
+    ub.capture()
+    for _ in range(2):
+        for d in devs:
+            d.density.zero_()
+            d.transmission.zero_()
+        ub.replay()
+        ub.check()
+        _check(devs, refs, 1e-12)
+        torch.ones(1000, device=device).sum().item()
+
+
+def test_serving_mix_block_sizes_alone_equals_mix(device, mix):
+    ws, _ = mix
+    wl = {f"b{w.block_sizes[0]}": w for w in ws}
+
+    def server(lanes):
+        srv = Server(device, n_lanes=lanes, hbm_budget_bytes=64 << 30, chunk=2)
+        for k, w in wl.items():
+            srv.add_device(k, w)
+        return srv
+
+    rng = np.random.default_rng(5)
+    reqs = [Request(k, rng.uniform(-1, 1, 3) + 1e-3j) for k in list(wl) * 2]
+    stats = server(4).run(reqs, realtime=False)
+    assert stats["requests"] == len(reqs)
+    alone_srv = server(1)
+    for r in reqs[:4]:
+        alone = Request(r.key, r.energies.copy())
+        alone_srv.run([alone], realtime=False)
+        assert torch.equal(alone.density, r.density)
+        assert torch.equal(alone.transmission, r.transmission)
