@@ -149,6 +149,22 @@ int qx_inv_batched(qx_handle* handle, void* stream, int64_t batch, int n,
                    int lds, int64_t stride_s, void* x, int ldx,
                    int64_t stride_x, int precision);
 
+/* Batched inverse that allocates no device memory (no counterpart in the
+ * reference: for callers that cannot allocate, such as code under HIP graph
+ * capture or tight serving loops).  Same result as qx_inv_batched, computed
+ * by the same kernels, with its scratch (ping-pong buffer, pivots, status
+ * flags) in a fixed array of QX_NOALLOC_SCRATCH_BYTES per GPU instead of the
+ * handle's workspace: batch * (16 n^2 + 12 n + 4) bytes plus alignment must
+ * fit (e.g. 63 matrices of 256, 15 of 512), else QX_EINVAL and nothing runs.
+ * info (nullable, device, batch ints) receives a nonzero flag per exactly
+ * singular matrix.  All calls on one GPU share its array: calls that overlap
+ * in time on one GPU race.  No kernel timing. */
+#define QX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int qx_inv_batched_noalloc(qx_handle* handle, void* stream, int64_t batch, int n,
+                           const void* a, int lda, int64_t stride_a, const void* s,
+                           int lds, int64_t stride_s, void* x, int ldx,
+                           int64_t stride_x, int* info, int precision);
+
 /* Sancho-Rubio surface Green's function on (n, b, b) stacks.  Replaces
  * SanchoRubio.__call__ (src/qttools/boundary_conditions/obc/sancho_rubio.py:33-86):
  * batch-global stop when max_e ||(|alpha|+|beta|)||_F / 2 < tol.  Writes

@@ -315,6 +315,11 @@ int qx_create(qx_handle** handle, int device) {
   *handle = nullptr;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return QX_EHIP;
+  try {
+    qx::noalloc_init(device);
+  } catch (const qx::Error& e) {
+    return e.code;
+  }
   qx_handle* h = new qx_handle();
   h->device = device;
   if (const char* env = std::getenv("QX_RGF_STREAMS")) {

@@ -238,6 +238,10 @@ extern thread_local KernelTimer* active_timer;
 // GEMM tile shape of the calling handle: 0 automatic, 64 or 128.
 extern thread_local int active_gemm_tile;
 
+// Looks up this device's copy of the fixed scratch array of
+// qx_inv_batched_noalloc (once per device; called by qx_create).
+void noalloc_init(int device);
+
 // Splits a batch of S matrices into up to `handle->parts` contiguous
 // sub-batches and runs f(stream, e0, n) for each concurrently: sub-batch 0 on
 // the caller's stream `st` in this thread, the others on handle-owned streams
