@@ -12,19 +12,22 @@ the inputs.
 
 The primary arithmetic (``--precision``, default complex128 with the 4M
 GEMM split) gives ``value``; every mode of ``--compare`` (default:
-c128-3m = complex128 with the 3M split, and mixed = complex64 products and
-inverses + one FP64 Newton-Schulz step per block inverse) is timed the
+c128-3m = complex128 with the 3M split, mixed = complex64 products and
+inverses + one FP64 Newton-Schulz step per block inverse, and oz2-12 =
+block products on int8 matrix cores by modular arithmetic with 12 moduli
+and FP64 inverses) is timed the
 same way and reported beside it with its speed-up and its error against the
 complex128 GPU result (per block, electron density diag G^<, transmission
 T(E) = Tr[Gamma_L G^R_{0,N-1} Gamma_R G^A_{N-1,0}]) on the first
 ``--accuracy-energies`` energies.  The CPU baseline leg times the NumPy
-oracle on a bounded energy sample and compares both GPU modes with it on
+oracle on a bounded energy sample and compares every GPU mode with it on
 those energies.
 
 Printed: one JSON line with energies/s (whole job), ms/step, per-call
-latency percentiles, the roofline of the dominant kernel (the batched MFMA
-GEMM; achieved = algorithmic 8 m n k flops per launch / HIP-event launch
-time over the timed region) and the CPU baseline.
+latency percentiles, the roofline of the step (matrix-core flops of every
+GEMM and inverse launch, counted per launch with HIP events over the timed
+region, / wall time; the dominant GEMM kernel's per-launch figures beside
+it) and the CPU baseline.
 """
 
 import argparse
@@ -41,6 +44,9 @@ if str(ROOT) not in sys.path:
 METRIC = "RGF energies/s at 64 blocks×256, mixed vs c128 GPU; p50/p99 solve latency"
 # MI355X dense matrix peaks of the GEMM arithmetic each mode runs on
 PEAK_TFLOPS = {"c128": 78.6, "c128-3m": 78.6, "c64": 157.3, "mixed": 157.3}
+# modular-arithmetic products on int8 cores: rated against the FP64 matrix
+# peak of the arithmetic they replace (complex flops 8 m n k per product)
+PEAK_TFLOPS.update({f"oz2-{n}": 78.6 for n in range(8, 21)})
 HBM_PEAK_GBS = 8000.0
 
 
@@ -131,6 +137,25 @@ TRAFFIC_FILE = ROOT / "profiles" / "zgemm_256_hbm_traffic.json"
 GEMM_KERNEL_KEY = "zgemm_reg_kernel<double,N,N,4M,64x64>"
 
 
+STEP_COUNTERS_FILE = ROOT / "profiles" / "rgf_step_64x256_counters.json"
+
+
+def measured_step_traffic(precision: str, required_step_flops: float):
+    """HBM bytes per step from the committed whole-step counter run (rocprofv3
+    FETCH_SIZE / WRITE_SIZE summed over every dispatch of one step, calibrated
+    on a copy) when it was taken at this precision and step size; else None."""
+    try:
+        rec = json.loads(STEP_COUNTERS_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    m = rec.get("modes", {}).get(precision)
+    if not m or abs(rec.get("required_flops_per_step", -1.0) - required_step_flops) > 1.0:
+        return None
+    return {"bytes_per_step": m["traffic_bytes_per_step"],
+            "source": f"rocprofv3 FETCH_SIZE + WRITE_SIZE over one step: "
+                      f"profiles/{STEP_COUNTERS_FILE.name}"}
+
+
 def measured_traffic(flops_per_launch: float):
     """HBM bytes per launch of the dominant GEMM from the committed counter
     measurement (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated on a copy), when
@@ -146,10 +171,12 @@ def measured_traffic(flops_per_launch: float):
     return None
 
 
-def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak, dominant_flops):
+def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak, dominant_flops,
+              required_step_flops):
     """Warm-up + timed steps of one arithmetic mode; returns its record.
     dominant_flops: flops of one launch of the dominant GEMM shape (one
-    sub-batch of b x b x b products), the shape the traffic was measured at."""
+    sub-batch of b x b x b products), the shape the traffic was measured at;
+    required_step_flops: flops one step requires (all energies)."""
     import numpy as np
     import torch
 
@@ -193,30 +220,49 @@ def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak, dominant_flop
                           "p99": round(float(np.percentile(lat_ms, 99)), 2)}}
     if timer:
         g, inv = timer.stats["gemm"], timer.stats["inv"]
-        # Sub-batches run on concurrent streams, so launches overlap: the GEMM
-        # throughput is its flops over the wall time during which at least one
-        # GEMM launch was in flight (union of the HIP-event intervals).
-        achieved = g["flops"] / (g["busy_ms"] / 1e3) / 1e12
+        # The roofline covers the whole step: the matrix-core flops of every
+        # GEMM and inverse launch of the timed steps, as launched (8 m n k per
+        # complex product, 8 n^3 per inverse), over the wall time of the timed
+        # steps.  The dominant kernel's per-launch figures (comparable with
+        # rocprofv3's average duration) are reported beside it: its launches
+        # from the 4 sub-batch streams overlap, so one launch's rate is its
+        # share of the chip, not the chip's rate.
+        executed = g["flops"] + inv["flops"]
+        achieved = executed / elapsed / 1e12
         per_launch = g["flops"] / max(g["launches"], 1)
         avg_ms = g["ms"] / max(g["launches"], 1)
-        traffic = measured_traffic(dominant_flops) if rgf.precision == "c128" else None
+        pl_tflops = per_launch / (avg_ms / 1e3) / 1e12
+        busy_union = g["flops"] / (g["busy_ms"] / 1e3) / 1e12
+        step_traffic = measured_step_traffic(rgf.precision, required_step_flops)
         rec["roofline"] = {
             "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
-            "kernel": "zgemm_reg_kernel (batched complex GEMM, register-direct, 64x64 tiles)",
-            "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch shape, calibrated "
-                               "on a 1 GiB copy: profiles/zgemm_256_hbm_traffic.json")
-            if traffic else None,
-            "traffic_launch_flops": dominant_flops if traffic else None,
-            "flops_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
-            "per_launch_tflops": round(per_launch / (avg_ms / 1e3) / 1e12, 3),
-            "busy_ms": round(g["busy_ms"], 2)}
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+            "traffic": step_traffic["bytes_per_step"] if step_traffic else None,
+            "method": ("whole step: flops of every GEMM and inverse launch of the timed steps "
+                       "(8 m n k per complex product, 8 n^3 per inverse, as launched) / wall "
+                       "time of the timed steps; traffic = HBM bytes per step"),
+            "executed_flops_per_step": executed / steps,
+            "required_flops_per_step": required_step_flops,
+            "traffic_source": step_traffic["source"] if step_traffic else None,
+            "dominant_kernel": {
+                "name": "zgemm_reg_kernel (batched complex GEMM, register-direct)",
+                "launches": g["launches"], "flops_per_launch": per_launch,
+                "avg_launch_ms": round(avg_ms, 4),
+                "per_launch_tflops": round(pl_tflops, 3),
+                "per_launch_frac": round(pl_tflops / peak, 4),
+                "launches_in_flight": round(g["ms"] / max(g["busy_ms"], 1e-9), 3),
+                "busy_union_tflops": round(busy_union, 3),
+                "traffic_per_launch": (measured_traffic(dominant_flops)
+                                       if rgf.precision == "c128" else None),
+                "traffic_launch_flops": dominant_flops}}
         rec["kernels"] = {
             "gemm": {"launches": g["launches"], "sum_ms": round(g["ms"], 2),
-                     "busy_ms": round(g["busy_ms"], 2), "tflops": round(achieved, 3)},
+                     "busy_ms": round(g["busy_ms"], 2), "flops": g["flops"],
+                     "busy_union_tflops": round(busy_union, 3)},
             "inverse": {"launches": inv["launches"], "sum_ms": round(inv["ms"], 2),
-                        "busy_ms": round(inv["busy_ms"], 2),
-                        "tflops": round(inv["flops"] / max(inv["busy_ms"], 1e-9) / 1e9, 3)},
+                        "busy_ms": round(inv["busy_ms"], 2), "flops": inv["flops"],
+                        "busy_union_tflops": round(inv["flops"] / max(inv["busy_ms"], 1e-9) / 1e9,
+                                                   3)},
             "wall_ms": round(elapsed * 1e3, 2),
         }
     return rec
@@ -456,7 +502,7 @@ def uneven_workload(args):
             "energies_per_s": round(ub.total_energies / t_graph, 3),
             "eager_energies_per_s": round(ub.total_energies / t_eager, 3),
             "graph_s": round(t_graph, 3), "eager_s": round(t_eager, 3),
-            "algorithmic_tflops": round(ub.total_flops / t_graph / 1e12, 3),
+            "required_tflops": round(ub.total_flops / t_graph / 1e12, 3),
             "lane_load_balance": round(min(ub.lane_flops) / max(ub.lane_flops), 3),
             "items": sum(len(l.items) for l in ub.lanes),
             "density": [d.density.clone() for d in devices],
@@ -592,7 +638,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="c128", choices=list(PEAK_TFLOPS))
-    ap.add_argument("--compare", default="c128-3m,mixed",
+    ap.add_argument("--compare", default="c128-3m,mixed,oz2-12",
                     help="comma-separated modes timed beside the primary one ('' for none)")
     ap.add_argument("--energies", type=int, default=512)
     ap.add_argument("--blocks", type=int, default=64)
@@ -648,7 +694,7 @@ def main():
 
     from quatrex_amd.datastructures import DSDBCOO, empty_btd
     from quatrex_amd.greens_function_solver import RGF
-    from quatrex_amd.workloads import make_workload, solve_flops
+    from quatrex_amd.workloads import make_workload, required_flops
 
     dev = torch.device("cuda:0")
     E, N, b, chunk = args.energies, args.blocks, args.block_size, args.chunk
@@ -675,7 +721,7 @@ def main():
         # the RGF splits a call into 4 concurrent sub-batches (QX_RGF_STREAMS)
         dominant = 8.0 * b ** 3 * ((chunk + 3) // 4)
         legs[m] = timed_leg(rgf, views, (xl, xg, xr), args.steps, args.warmup, dev,
-                            not args.no_timer, peak, dominant)
+                            not args.no_timer, peak, dominant, E * required_flops(N, b))
         legs[m]["value"] = E * args.steps / legs[m]["elapsed"]
         log(f"{m}: {legs[m]['value']:.1f} energies/s")
     del xl, xg, xr
@@ -697,7 +743,6 @@ def main():
             oref = None
 
     prim = legs[args.precision]
-    flops = solve_flops(N, b)
     result = {
         "metric": METRIC,
         "value": round(prim["value"], 3),
@@ -710,15 +755,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"c128": "f64", "c128-3m": "f64", "c64": "f32",
-                  "mixed": "f32+f64"}[args.precision],
+        "dtype": ("f64+i8" if args.precision.startswith("oz2-") else
+                  {"c128": "f64", "c128-3m": "f64", "c64": "f32",
+                   "mixed": "f32+f64"}[args.precision]),
         "data": "synthetic",
         "config": {"workload": f"RGF selected_solve, {E} energies, {N} blocks of {b}, "
                                f"complex128 storage, {args.precision} arithmetic",
                    "energies": E, "n_blocks": N, "block_size": b, "precision": args.precision,
                    "energies_per_call": chunk},
         "latency_ms": dict(prim["latency_ms"], per=f"selected_solve call of {chunk} energies"),
-        "algorithmic_tflops": round(prim["value"] * flops / 1e12, 3),
         "roofline": prim.get("roofline"),
         "kernels": prim.get("kernels"),
         "cpu_baseline": baseline,
@@ -729,7 +774,9 @@ def main():
              "speedup_vs_primary": round(legs[m]["value"] / prim["value"], 3),
              "latency_ms": legs[m]["latency_ms"]}
         if "roofline" in legs[m]:
-            r["gemm_tflops"] = legs[m]["roofline"]["achieved"]
+            r["solve_tflops"] = legs[m]["roofline"]["achieved"]
+            r["solve_frac"] = legs[m]["roofline"]["frac"]
+            r["gemm_busy_union_tflops"] = legs[m]["kernels"]["gemm"]["busy_union_tflops"]
             r["inverse_busy_ms"] = legs[m]["kernels"]["inverse"]["busy_ms"]
         if acc and m != "c128":
             r["error_vs_c128_gpu"] = accuracy_report(w, acc[m], acc["c128"])

@@ -45,6 +45,14 @@ extern "C" {
                              /* split (3/4 of the matrix-core work, rounding  */
                              /* error a small factor above the 4M split)      */
 
+/* complex128 storage and inverses; block products on int8 matrix cores by
+ * modular arithmetic (Ozaki scheme II) with n pairwise coprime moduli,
+ * 8 <= n <= 20: every line of a product's operands is scaled to integers of
+ * T bits (T = 42 for 12 moduli at k <= 256), so the relative error per
+ * product is ~2^-T of the line maxima.  Accepted by qx_zgemm_batched and the
+ * RGF calls. */
+#define QX_PREC_OZAKI2(n) (16 + (n))
+
 /* GEMM operand ops */
 #define QX_OP_N 0
 #define QX_OP_C 2            /* conjugate transpose                          */

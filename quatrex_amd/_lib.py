@@ -31,8 +31,16 @@ PREC_C128 = 0
 PREC_C64 = 1
 PREC_MIXED = 2
 PREC_C128_3M = 3
+
+def PREC_OZAKI2(n: int) -> int:
+    """QX_PREC_OZAKI2(n): complex128 data, block products on int8 matrix cores
+    by modular arithmetic with n moduli (8..20)."""
+    return 16 + n
+
+
 PRECISIONS = {"c128": PREC_C128, "complex128": PREC_C128, "c64": PREC_C64, "complex64": PREC_C64,
               "c128-3m": PREC_C128_3M}
+PRECISIONS.update({f"oz2-{n}": PREC_OZAKI2(n) for n in range(8, 21)})
 # RGF also accepts the mixed mode (complex64 products, FP64 Newton-Schulz
 # step on every block inverse).
 RGF_PRECISIONS = dict(PRECISIONS, mixed=PREC_MIXED)

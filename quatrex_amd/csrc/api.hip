@@ -10,7 +10,7 @@
 #include "qx_internal.h"
 
 namespace qx {
-size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve);
+size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve, int prec);
 void rgf_selected_inv(qx_handle* h, hipStream_t st, const qx_btd_layout* la, const void* a,
                       const void* const* obc_r, const qx_btd_layout* lx, void* x, int prec);
 void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, const void* a,
@@ -145,6 +145,7 @@ int beyn_moments(qx_handle* h, hipStream_t st, int64_t n, int b, int nq, double 
 }  // namespace
 
 thread_local KernelTimer* active_timer = nullptr;
+thread_local OzWorkspace active_oz_ws;
 thread_local int active_gemm_tile = 0;
 
 thread_local int split_part = 0;
@@ -402,7 +403,7 @@ int qx_workspace_query(qx_handle* h, const qx_btd_layout* la, int precision, siz
   (void)precision;
   return guarded(h, [&]() {
     QX_REQUIRE(la && bytes, "workspace_query: null argument");
-    *bytes = qx::rgf_workspace_bytes(la, true);
+    *bytes = qx::rgf_workspace_bytes(la, true, precision);
     return QX_OK;
   });
 }
@@ -459,6 +460,14 @@ int qx_zgemm_batched(qx_handle* h, void* stream, int64_t batch, int m, int n, in
                      int ldc, int64_t stride_c, int precision) {
   return guarded(h, [&]() {
     QX_REQUIRE(alpha && beta, "zgemm: null alpha/beta");
+    struct OzScope {
+      ~OzScope() { qx::active_oz_ws = qx::OzWorkspace{}; }
+    } oz_scope;
+    if (const int nmod = qx::oz2_moduli(precision)) {
+      QX_REQUIRE(h, "null handle");
+      const size_t bytes = qx::oz2_workspace_bytes(batch, m, n, k, nmod);
+      qx::active_oz_ws = qx::OzWorkspace{h->ensure_ws(bytes), h->ws_bytes};
+    }
     qx::zgemm(static_cast<hipStream_t>(stream), batch, m, n, k, op_a, op_b,
               qx::Cplx{alpha[0], alpha[1]}, qx::Mat(a, stride_a, lda), qx::Mat(b, stride_b, ldb),
               qx::Cplx{beta[0], beta[1]}, qx::Mat(c, stride_c, ldc), precision);

@@ -766,9 +766,18 @@ void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int 
   QX_REQUIRE(opA == QX_OP_N || opA == QX_OP_C, "zgemm: bad op_a");
   QX_REQUIRE(opB == QX_OP_N || opB == QX_OP_C, "zgemm: bad op_b");
   QX_REQUIRE(precision == QX_PREC_C128 || precision == QX_PREC_C64 ||
-                 precision == QX_PREC_C128_3M,
-             "zgemm: precision must be QX_PREC_C128, QX_PREC_C64 or QX_PREC_C128_3M");
+                 precision == QX_PREC_C128_3M || oz2_moduli(precision) > 0,
+             "zgemm: precision must be QX_PREC_C128, QX_PREC_C64, QX_PREC_C128_3M or "
+             "QX_PREC_OZAKI2(8..20)");
   if (batch <= 0 || m <= 0 || n <= 0) return;
+  if (const int nmod = oz2_moduli(precision)) {
+    // one launch sequence per product, in order, on the calling thread's
+    // residue workspace
+    for (int a = 0; a < ng; ++a)
+      zgemm_oz2(s, batch, m, n, k, opA, opB, alpha, beta, *gs[a], nmod, active_oz_ws.p,
+                active_oz_ws.bytes);
+    return;
+  }
   if (ng > 1) {
     // otherwise x, y[, z] as separate launches in that order
     bool one_launch = !x.A.zero() && !x.B.zero() && k > 0;

@@ -147,6 +147,23 @@ void zgemm_ozaki(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int
                  Cplx alpha, const Mat& A, const Mat& B, Cplx beta, const Mat& Cin,
                  const Mat& C, int slices, void* ws, size_t ws_bytes);
 
+// Modular-arithmetic (Ozaki scheme II) GEMM on int8 matrix cores for
+// complex128 data with nmod moduli (8..20); ws holds the residues.
+size_t oz2_workspace_bytes(int64_t batch, int m, int n, int k, int nmod);
+void zgemm_oz2(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB, Cplx alpha,
+               Cplx beta, const GemmOperands& x, int nmod, void* ws, size_t ws_bytes);
+// Number of moduli of an Ozaki-II precision code, or 0 for any other code.
+inline int oz2_moduli(int precision) {
+  return (precision >= QX_PREC_OZAKI2(8) && precision <= QX_PREC_OZAKI2(20)) ? precision - 16 : 0;
+}
+// Residue workspace of the calling thread for Ozaki-II products (set by the
+// RGF drivers per sub-batch, by qx_zgemm_batched per call).
+struct OzWorkspace {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+extern thread_local OzWorkspace active_oz_ws;
+
 // X = inv(A - S) with partial pivoting (S may be zero).  info[e] is set to a
 // nonzero value for an exactly singular matrix.  n <= kInvMaxN.  The scratch
 // holds, per matrix, an n x n complex buffer (buf + e * buf_stride) and 3 n

@@ -84,13 +84,32 @@ void fold(hipStream_t st, int64_t S, int m, int n, const Mat& a, double sign,
   ew_axpby(st, S, m, n, kOne, a, Cplx{sign, 0.0}, s, dst);
 }
 
-// Arithmetic of the block products: QX_PREC_MIXED runs them in complex64.
+// Arithmetic of the block products: QX_PREC_MIXED runs them in complex64,
+// QX_PREC_OZAKI2(n) on int8 matrix cores (the block inverses stay FP64).
 int gemm_precision(int prec) {
   QX_REQUIRE(prec == QX_PREC_C128 || prec == QX_PREC_C64 || prec == QX_PREC_MIXED ||
-                 prec == QX_PREC_C128_3M,
+                 prec == QX_PREC_C128_3M || oz2_moduli(prec) > 0,
              "unknown precision selector");
   return prec == QX_PREC_MIXED ? QX_PREC_C64 : prec;
 }
+
+// Residue workspace of one matrix for Ozaki-II products of blocks <= bmax.
+size_t oz2_bytes_per_matrix(int prec, int bmax) {
+  const int nmod = oz2_moduli(prec);
+  if (!nmod) return 0;
+  return (oz2_workspace_bytes(1, bmax, bmax, bmax, nmod) + 255) / 256 * 256;
+}
+
+// Sets the calling thread's Ozaki-II residue workspace to the part of
+// sub-batch [e0, e0 + S) and clears it on scope exit.
+struct OzSubBatch {
+  OzSubBatch(int prec, void* region, int bmax, int64_t e0, int64_t S) {
+    const size_t per = oz2_bytes_per_matrix(prec, bmax);
+    if (per)
+      active_oz_ws = OzWorkspace{static_cast<char*>(region) + e0 * per, (size_t)S * per};
+  }
+  ~OzSubBatch() { active_oz_ws = OzWorkspace{}; }
+};
 
 // In-place block inverse X <- inv(X) in the arithmetic `prec` selects.
 // QX_PREC_MIXED inverts in complex64, then applies one Newton-Schulz step
@@ -126,11 +145,18 @@ int block_limits(const qx_btd_layout* la, const char* what) {
 
 }  // namespace
 
-size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve) {
+// Bytes of the slot stacks (before the Ozaki-II residue region).
+size_t rgf_slot_bytes(const qx_btd_layout* la, bool solve, int bmax) {
+  const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 26 : (int64_t)la->n_blocks + 10;  // see slot maps
+  return (size_t)(slots * la->n_stack * (int64_t)bmax * bmax) * sizeof(z2);
+}
+
+size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve, int prec) {
   int bmax = 1;
   for (int i = 0; i < la->n_blocks; ++i) bmax = std::max(bmax, (int)la->block_sizes[i]);
-  const int64_t slots = solve ? 3 * (int64_t)la->n_blocks + 26 : (int64_t)la->n_blocks + 10;  // see slot maps
-  size_t bytes = (size_t)(slots * la->n_stack * (int64_t)bmax * bmax) * sizeof(z2);
+  size_t bytes = rgf_slot_bytes(la, solve, bmax);
+  // Ozaki-II products: residues of one product per matrix of the call
+  bytes += (size_t)la->n_stack * oz2_bytes_per_matrix(prec, bmax) + (oz2_moduli(prec) ? 512 : 0);
   // the fused small-block kernels keep 6 N + 16 (solve) or 2 N + 3 (inverse)
   // slots of 32 x 32 per energy
   if (la->n_blocks <= 16 && bmax <= 32) {
@@ -152,15 +178,17 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   const int bmax = block_limits(la, "selected_inv");
   gemm_precision(prec);  // validates the selector
   if (rgf_small_inv(h, st0, la, a, obc_r, lx, x, prec)) return;
-  z2* ws = static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, false)));
+  z2* ws = static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, false, prec)));
   int* info_all = h->ensure_info(Stot);
   const Btd A0(la, a), X0(lx, x);
+  char* oz_region = reinterpret_cast<char*>(ws) + rgf_slot_bytes(la, false, bmax);
 
   const int gp = gemm_precision(prec);
   // QX_RGF_PAIR=0 (read per call): one launch per backward product
   const char* pair_env = std::getenv("QX_RGF_PAIR");
   const bool pair_on = !(pair_env && pair_env[0] == '0');
   run_split(h, st0, Stot, [&](hipStream_t st, int64_t e0, int64_t S) {
+    const OzSubBatch oz(prec, oz_region, bmax, e0, S);
     const Btd A = A0.sub(e0), X = X0.sub(e0);
     const Slots W{ws, Stot, (int64_t)bmax * bmax, e0};
     const Obc OR{obc_r, e0};
@@ -263,8 +291,9 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
                       want_r ? lxr : nullptr, want_r ? xr : nullptr, want_cur ? current : nullptr,
                       corner, density, flags, prec))
     return;
-  z2* ws = static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, true)));
+  z2* ws = static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, true, prec)));
   int* info_all = h->ensure_info(Stot);
+  char* oz_region = reinterpret_cast<char*>(ws) + rgf_slot_bytes(la, true, bmax);
   const Btd A0(la, a), SL0(lsl, sl), SG0(lsg, sg), XL0(lxl, xl), XG0(lxg, xg),
       XR0(want_r ? lxr : nullptr, want_r ? xr : nullptr);
 
@@ -282,6 +311,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   std::promise<void> stg_set;
   std::shared_future<void> stg_ready = stg_set.get_future().share();
   run_split(h, st0, Stot, [&](hipStream_t st, int64_t e0, int64_t S) {
+  const OzSubBatch oz(prec, oz_region, bmax, e0, S);
   const int part = split_part;
   // sub-batch 0 releases the others once its first inverse is enqueued, or
   // when it leaves early with an error

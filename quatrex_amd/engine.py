@@ -45,7 +45,7 @@ import torch
 from quatrex_amd import _lib
 from quatrex_amd.datastructures import DSDBCOO, btd_from_blocks, btd_pattern
 from quatrex_amd.greens_function_solver import RGF, transmission
-from quatrex_amd.workloads import Workload, make_workload, solve_flops
+from quatrex_amd.workloads import Workload, make_workload, required_flops
 
 
 def _f(x, dev):
@@ -119,7 +119,7 @@ class Device:
         return max(self.block_sizes)
 
     def flops_per_energy(self) -> float:
-        return solve_flops(self.n_blocks, self.b)
+        return required_flops(self.n_blocks, self.b)
 
     def lane_bytes_per_energy(self) -> int:
         """HBM a lane needs per energy of a chunk: A, two block-diagonal

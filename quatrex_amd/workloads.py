@@ -146,9 +146,25 @@ def lead_blocks(seed: int = 0, n_energies: int = 1024, block_size: int = 256, et
 
 
 def solve_flops(n_blocks: int, b: int) -> float:
-    """Algorithmic flops of one selected_solve energy (G^R, G^<, G^>).
-
-    The reference algorithm (rgf.py:115-414) issues 4 + 33 (N - 1) block
-    products and N block inverses per energy, each 8 b^3 real flops.
+    """Flops of one selected_solve energy (G^R, G^<, G^>) as the reference
+    algorithm (rgf.py:115-414) issues them: 4 + 33 (N - 1) block products and
+    N block inverses per energy, each 8 b^3 real flops.  With the
+    block-diagonal Sigma^<> of these workloads 10 of the 33 products per
+    block step multiply a zero block; see ``required_flops``.
     """
     return 8.0 * b ** 3 * ((4 + 33 * (n_blocks - 1)) + n_blocks)
+
+
+def required_flops(n_blocks: int, b: int) -> float:
+    """Flops one selected_solve energy (G^R diagonal, G^<, G^>) requires
+    when Sigma^< and Sigma^> are block-diagonal, as in these workloads: the
+    reference's products with Sigma_{i,i+1} (rgf.py:270, 283, 336-346, 358-368)
+    are products with a zero block.  Per block step 10 products and one
+    inverse in the forward sweep (a_ji x^R_ii, a_jj - t a_ij, and per lesser /
+    greater quantity a_ji x_ii, (.) a_ji^H + Sigma_jj, x^R_jj (.),
+    (.) x^R_jj^H) and 13 products in the backward sweep (x^R_ii a_ij,
+    x^R_jj a_ji and the two products with them, per quantity four, and the
+    G^R_ii update); block 0 has 4 products and one inverse.  Each product
+    and each inverse is 8 b^3 real flops.
+    """
+    return 8.0 * b ** 3 * ((4 + 23 * (n_blocks - 1)) + n_blocks)

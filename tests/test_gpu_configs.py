@@ -51,7 +51,7 @@ pytestmark = pytest.mark.gpu
 N_PROD, B_PROD = 64, 256
 # (per-block, density, transmission) bars per arithmetic
 BARS = {"c128": (1e-10, 1e-10, 1e-10), "c128-3m": (1e-10, 1e-10, 1e-10),
-        "mixed": (5e-2, 2e-3, 2e-4)}
+        "mixed": (5e-2, 2e-3, 2e-4), "oz2-12": (1e-6, 1e-8, 1e-8)}
 
 
 def _gammas(w, ne):
