@@ -26,11 +26,13 @@
 extern "C" {
 #endif
 
-#define QX_ABI_VERSION 2
+#define QX_ABI_VERSION 3
 
 /* status codes */
 #define QX_OK 0
 #define QX_NOT_CONVERGED 1   /* warning: iteration limit reached            */
+#define QX_FP64_FALLBACK 2   /* warning: mixed mode redid block inverses in  */
+                             /* FP64 (Newton-Schulz residual check failed)   */
 #define QX_EINVAL (-1)       /* invalid argument                            */
 #define QX_ENOMEM (-2)       /* device allocation failed                    */
 #define QX_EHIP (-3)         /* HIP runtime error                           */
@@ -102,7 +104,10 @@ int qx_timer_start(qx_handle* handle, void* stream);
 int qx_timer_stop(qx_handle* handle, void* stream, double* stats);
 
 /* Wait for `stream`, then report (and clear) device-side errors recorded
- * since the last check: QX_ESINGULAR if an inversion met a zero pivot. */
+ * since the last check: QX_ESINGULAR if an inversion met a zero pivot, else
+ * QX_FP64_FALLBACK if a QX_PREC_MIXED call replaced a complex64 block inverse
+ * whose Newton-Schulz residual ||I - M X0||_F was not below the tolerance
+ * (env QX_MIXED_TOL, default 1e-2) by the FP64 pivoted inverse. */
 int qx_sync_check(qx_handle* handle, void* stream);
 
 /* RGF selected inversion.  Replaces RGF.selected_inv
@@ -128,7 +133,11 @@ int qx_rgf_selected_inv(qx_handle* handle, void* stream,
  * density (nullable): (n_stack, sum of block sizes) doubles, receives
  * Im diag(G^<) as each diagonal block is finished (the quantity of
  * quatrex/core/observables.py:57-86 without overlap), without a second pass
- * over the output. */
+ * over the output.
+ * ldos (nullable): same layout, receives the electron local density of
+ * states -Im diag(G^R) / (2 pi) (quatrex/core/scba.py:572-576 without
+ * overlap) as each diagonal G^R block is finished, whether or not
+ * QX_SOLVE_RETARDED is set. */
 int qx_rgf_selected_solve(qx_handle* handle, void* stream,
                           const qx_btd_layout* la, const void* a,
                           const qx_btd_layout* lsl, const void* sigma_lesser,
@@ -139,7 +148,7 @@ int qx_rgf_selected_solve(qx_handle* handle, void* stream,
                           const qx_btd_layout* lxg, void* x_greater,
                           const qx_btd_layout* lxr, void* x_retarded,
                           void* current, void* corner, double* density,
-                          uint32_t flags, int precision);
+                          double* ldos, uint32_t flags, int precision);
 
 /* Batched complex GEMM C = alpha op(A) op(B) + beta C, op in {N, C}.
  * Replaces the batched `@` of the reference (cuBLAS zgemm strided-batched).

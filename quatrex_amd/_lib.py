@@ -21,12 +21,13 @@ LIB_PATH = Path(os.environ.get("QX_LIBRARY") or
 
 QX_OK = 0
 QX_NOT_CONVERGED = 1
+QX_FP64_FALLBACK = 2
 QX_EINVAL = -1
 QX_ENOMEM = -2
 QX_EHIP = -3
 QX_ESINGULAR = -4
 
-ABI_VERSION = 2  # QX_ABI_VERSION of include/quatrex_amd.h
+ABI_VERSION = 3  # QX_ABI_VERSION of include/quatrex_amd.h
 PREC_C128 = 0
 PREC_C64 = 1
 PREC_MIXED = 2
@@ -123,7 +124,7 @@ def load() -> ctypes.CDLL:
         lib.qx_rgf_selected_inv.argtypes = [vp, vp, lay_p, vp, vp, lay_p, vp, i32]
         lib.qx_rgf_selected_solve.argtypes = [
             vp, vp, lay_p, vp, lay_p, vp, lay_p, vp, vp, vp, vp, lay_p, vp, lay_p, vp,
-            lay_p, vp, vp, vp, vp, ctypes.c_uint32, i32]
+            lay_p, vp, vp, vp, vp, vp, ctypes.c_uint32, i32]
         lib.qx_zgemm_batched.argtypes = [
             vp, vp, i64, i32, i32, i32, i32, i32, ctypes.POINTER(f64), vp, i32, i64, vp, i32,
             i64, ctypes.POINTER(f64), vp, i32, i64, i32]
@@ -225,6 +226,9 @@ def check(rc: int, h, what: str) -> int:
         return rc
     msg = load().qx_last_error(h).decode(errors="replace")
     if rc == QX_NOT_CONVERGED:
+        return rc
+    if rc == QX_FP64_FALLBACK:
+        warnings.warn(f"{what}: {msg}", RuntimeWarning, stacklevel=3)
         return rc
     if rc == QX_EINVAL:
         raise ValueError(f"{what}: {msg}")

@@ -18,8 +18,8 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, c
                         const void* sg, const void* const* obc_r, const void* const* obc_l,
                         const void* const* obc_g, const qx_btd_layout* lxl, void* xl,
                         const qx_btd_layout* lxg, void* xg, const qx_btd_layout* lxr, void* xr,
-                        void* current, void* corner, double* density, uint32_t flags,
-                        int prec);
+                        void* current, void* corner, double* density, double* ldos,
+                        uint32_t flags, int prec);
 
 namespace {
 
@@ -272,6 +272,22 @@ int* qx_handle::ensure_info(size_t n) {
   return info;
 }
 
+int* qx_handle::ensure_mask(size_t n) {
+  if (n > mask_len) {
+    const size_t len = std::max<size_t>(n, 1024);
+    int* p = nullptr;
+    if (hipMalloc(&p, len * sizeof(int)) != hipSuccess)
+      throw qx::Error{QX_ENOMEM, "mask allocation failed"};
+    if (mask) {
+      QX_HIP_CHECK(hipDeviceSynchronize());
+      QX_HIP_CHECK(hipFree(mask));
+    }
+    mask = p;
+    mask_len = len;
+  }
+  return mask;
+}
+
 namespace {
 
 // Per-call thread-local state taken from the handle: kernel timer and GEMM
@@ -393,6 +409,7 @@ int qx_destroy(qx_handle* h) {
   if (h->fork) (void)hipEventDestroy(h->fork);
   if (h->ws) (void)hipFree(h->ws);
   if (h->info) (void)hipFree(h->info);
+  if (h->mask) (void)hipFree(h->mask);
   delete h;
   return QX_OK;
 }
@@ -415,14 +432,23 @@ int qx_sync_check(qx_handle* h, void* stream) {
     if (!h->info) return QX_OK;
     std::vector<int> v(h->info_len);
     QX_HIP_CHECK(hipMemcpy(v.data(), h->info, h->info_len * sizeof(int), hipMemcpyDeviceToHost));
-    int any = 0;
-    for (int x : v) any |= x;
+    int any = 0, fallbacks = 0;
+    for (int x : v) {
+      any |= x;
+      fallbacks += (x & 2) ? 1 : 0;
+    }
     if (any) {
       hipLaunchKernelGGL(qx::fill_int_kernel, dim3(64), dim3(256), 0, st, h->info,
                          (int64_t)h->info_len, 0);
       QX_HIP_CHECK(hipStreamSynchronize(st));
-      h->last_error = "Singular matrix";
-      return QX_ESINGULAR;
+      if (any & 1) {
+        h->last_error = "Singular matrix";
+        return QX_ESINGULAR;
+      }
+      h->last_error = std::to_string(fallbacks) +
+                      " matrices had a complex64 block inverse redone in FP64 (residual of the "
+                      "Newton-Schulz check above tolerance or not finite)";
+      return QX_FP64_FALLBACK;
     }
     return QX_OK;
   });
@@ -443,13 +469,13 @@ int qx_rgf_selected_solve(qx_handle* h, void* stream, const qx_btd_layout* la, c
                           const void* sg, const void* const* obc_r, const void* const* obc_l,
                           const void* const* obc_g, const qx_btd_layout* lxl, void* xl,
                           const qx_btd_layout* lxg, void* xg, const qx_btd_layout* lxr, void* xr,
-                          void* current, void* corner, double* density, uint32_t flags,
-                          int precision) {
+                          void* current, void* corner, double* density, double* ldos,
+                          uint32_t flags, int precision) {
   return guarded(h, [&]() {
     QX_REQUIRE(h, "null handle");
     qx::rgf_selected_solve(h, static_cast<hipStream_t>(stream), la, a, lsl, sl, lsg, sg, obc_r,
-                           obc_l, obc_g, lxl, xl, lxg, xg, lxr, xr, current, corner, density, flags,
-                           precision);
+                           obc_l, obc_g, lxl, xl, lxg, xg, lxr, xr, current, corner, density, ldos,
+                           flags, precision);
     return QX_OK;
   });
 }

@@ -207,12 +207,13 @@ __global__ __launch_bounds__(NT) void contact_sigma_kernel(const z2* S, int64_t 
 }
 
 __global__ __launch_bounds__(NT) void diag_imag_kernel(const z2* X, int64_t sX, int ldx, int n,
-                                                       double* out, int64_t so, int64_t total) {
+                                                       double* out, int64_t so, int64_t total,
+                                                       double scale) {
   for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * NT) {
     const int64_t e = i / n;
     const int r = (int)(i - e * n);
-    out[e * so + r] = X[e * sX + (int64_t)r * ldx + r].y;
+    out[e * so + r] = scale * X[e * sX + (int64_t)r * ldx + r].y;
   }
 }
 
@@ -308,6 +309,36 @@ __global__ __launch_bounds__(NT) void absnorm_kernel(int n, const z2* A, int64_t
   }
   acc = block_sum(acc, red);
   if (threadIdx.x == 0) out[e] = sqrt(acc);
+}
+
+// Newton-Schulz check of one complex64 block inverse per workgroup (see
+// ew_ns_check).  Every thread reads the flag after the block sum, so the
+// zeroing pass is uniform over the workgroup.
+__global__ __launch_bounds__(NT) void ns_check_kernel(int n, z2* R, int64_t sR, int ldr,
+                                                      double tol, int* flagged, int* info) {
+  __shared__ double red[NT / 64];
+  __shared__ int flag;
+  const int64_t e = blockIdx.x;
+  z2* Re = R + e * sR;
+  double acc = 0.0;
+  for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+    const int r = idx / n, c = idx - r * n;
+    const z2 v = Re[(int64_t)r * ldr + c];
+    acc += v.x * v.x + v.y * v.y;
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    // NaN and Inf fail the comparison
+    flag = sqrt(acc) < tol ? 0 : 1;
+    flagged[e] = flag;
+    if (flag) info[e] |= 2;
+  }
+  __syncthreads();
+  if (flag)
+    for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+      const int r = idx / n, c = idx - r * n;
+      Re[(int64_t)r * ldr + c] = make_double2(0.0, 0.0);
+    }
 }
 
 struct ContourArgs {
@@ -420,12 +451,12 @@ void ew_contact_sigma(hipStream_t s, int64_t batch, int n, const Mat& S, const d
 }
 
 void ew_diag_imag(hipStream_t s, int64_t batch, int n, const Mat& X, double* out,
-                  int64_t out_stride) {
+                  int64_t out_stride, double scale) {
   if (batch <= 0 || n <= 0) return;
   QX_REQUIRE(!X.zero() && out, "diag_imag: null argument");
   const int64_t total = batch * n;
   hipLaunchKernelGGL(diag_imag_kernel, dim3(grid_for(total)), dim3(NT), 0, s, X.p, X.stride, X.ld,
-                     n, out, out_stride, total);
+                     n, out, out_stride, total, scale);
   QX_HIP_CHECK(hipGetLastError());
 }
 
@@ -501,6 +532,15 @@ void absnorm2(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& B,
   if (batch <= 0) return;
   hipLaunchKernelGGL(absnorm_kernel, dim3((unsigned)batch), dim3(NT), 0, s, n, A.p,
                      A.stride, A.ld, B.p, B.stride, B.ld, out);
+  QX_HIP_CHECK(hipGetLastError());
+}
+
+void ew_ns_check(hipStream_t s, int64_t batch, int n, const Mat& R, double tol, int* flagged,
+                 int* info) {
+  if (batch <= 0) return;
+  QX_REQUIRE(batch < (int64_t(1) << 31), "ns_check: batch too large");
+  hipLaunchKernelGGL(ns_check_kernel, dim3((unsigned)batch), dim3(NT), 0, s, n, R.p, R.stride,
+                     R.ld, tol, flagged, info);
   QX_HIP_CHECK(hipGetLastError());
 }
 

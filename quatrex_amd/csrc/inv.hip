@@ -114,6 +114,7 @@ struct PanelArgs {
   int* ints;  // per matrix: perm[n], sig[n], idx[n]
   int64_t sI;
   int* info;
+  const int* mask;  // nullable: matrices with mask[e] == 0 are left alone
 };
 
 // Row and column gathers of a finished panel.  Each position is traced
@@ -157,6 +158,7 @@ __global__ __launch_bounds__(NT) void inv_panel(PanelArgs g) {
   __shared__ unsigned red[NT / 64];
   __shared__ int perm_l[NB];
   const int64_t e = blockIdx.x;
+  if (g.mask && !g.mask[e]) return;  // uniform over the workgroup
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int n = g.n, k0 = g.k0, nbw = g.nbw;
   const z2* Xc = g.Xc + e * g.sXc;
@@ -317,6 +319,7 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
   __shared__ unsigned red[NT4 / 64];
   __shared__ int perm_l[NB];
   const int64_t e = blockIdx.x;
+  if (g.mask && !g.mask[e]) return;  // uniform over the workgroup
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int r = tid >> 2, q = tid & 3;
   const int n = g.n, k0 = g.k0, nbw = g.nbw;
@@ -456,6 +459,7 @@ struct UpdArgs {
   int ldn;
   const int* ints;
   int64_t sI;
+  const int* mask;  // nullable, as PanelArgs::mask
 };
 
 // Xn[r, c] = (r in K ? 0 : Xc[sig r, c]) + sum_k Xn[r, k0+k] Xc[sig(k0+k), c]
@@ -475,6 +479,7 @@ __global__ __launch_bounds__(NT) void inv_update(UpdArgs g) {
   const int n = g.n, k0 = g.k0, nbw = g.nbw;
   // a column tile that lies inside the panel has nothing to do
   if (c0 >= k0 && c0 + 64 <= k0 + nbw) return;
+  if (g.mask && !g.mask[e]) return;
   const z2* Xc = g.Xc + e * g.sXc;
   z2* Xn = g.Xn + e * g.sXn;
   const int* sig = g.ints + e * g.sI + n;
@@ -559,15 +564,37 @@ __global__ __launch_bounds__(NT) void inv_update(UpdArgs g) {
 // X[r, c] = Y[r, idx c]; X may not alias Y.
 __global__ __launch_bounds__(NT) void inv_gather(int n, const z2* Y, int64_t sY, int ldy, z2* X,
                                                  int64_t sX, int ldx, const int* ints, int64_t sI,
-                                                 int64_t total) {
+                                                 int64_t total, const int* mask) {
   const int64_t nn = (int64_t)n * n;
   for (int64_t q = blockIdx.x * (int64_t)NT + threadIdx.x; q < total;
        q += (int64_t)gridDim.x * NT) {
     const int64_t e = q / nn;
+    if (mask && !mask[e]) continue;
     const int64_t rc = q - e * nn;
     const int r = (int)(rc / n), c = (int)(rc - (int64_t)r * n);
     const int* idx = ints + e * sI + 2 * n;
     X[e * sX + (int64_t)r * ldx + c] = Y[e * sY + (int64_t)r * ldy + idx[c]];
+  }
+}
+
+// Y = A - S (S nullable) for the matrices with mask[e] != 0.
+__global__ __launch_bounds__(NT) void inv_masked_start(int n, const z2* A, int64_t sA, int lda,
+                                                       const z2* S, int64_t sS, int lds, z2* Y,
+                                                       int64_t sY, int ldy, int64_t total,
+                                                       const int* mask) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t q = blockIdx.x * (int64_t)NT + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * NT) {
+    const int64_t e = q / nn;
+    if (!mask[e]) continue;
+    const int64_t rc = q - e * nn;
+    const int r = (int)(rc / n), c = (int)(rc - (int64_t)r * n);
+    z2 v = A[e * sA + (int64_t)r * lda + c];
+    if (S) {
+      const z2 w = S[e * sS + (int64_t)r * lds + c];
+      v = make_double2(v.x - w.x, v.y - w.y);
+    }
+    Y[e * sY + (int64_t)r * ldy + c] = v;
   }
 }
 
@@ -583,7 +610,7 @@ bool use_panel4() {
 
 template <typename T, int NB, int RPT>
 void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScratch& scr,
-                 int s0, int* info) {
+                 int s0, int* info, const int* mask) {
   const int npan = (n + NB - 1) / NB;
   const Mat B[2] = {X, Mat(scr.buf, scr.buf_stride, n)};
   int cur = s0;
@@ -591,14 +618,14 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
   for (int p = 0; p < npan; ++p) {
     const int k0 = p * NB, nbw = min(NB, n - k0), nxt = 1 - cur;
     PanelArgs pa{n, k0, nbw, B[cur].p, B[cur].stride, B[cur].ld,
-                 B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, info};
+                 B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, info, mask};
     if (NB == 32 && RPT == 1 && use_panel4())
       hipLaunchKernelGGL((inv_panel4<T>), dim3((unsigned)batch), dim3(NT4), 0, s, pa);
     else
       hipLaunchKernelGGL((inv_panel<T, NB, RPT>), dim3((unsigned)batch), dim3(NT), 0, s, pa);
     if (n > nbw) {
       UpdArgs ua{n, k0, nbw, tn * tn, B[cur].p, B[cur].stride, B[cur].ld,
-                 B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride};
+                 B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, mask};
       const int64_t blocks = batch * tn * tn;
       QX_REQUIRE(blocks < (int64_t(1) << 31), "inv: grid too large");
       hipLaunchKernelGGL((inv_update<T, NB>), dim3((unsigned)blocks), dim3(NT), 0, s, ua);
@@ -610,7 +637,8 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
   int64_t blocks = (total + NT - 1) / NT;
   if (blocks > 65536 * 4) blocks = 65536 * 4;
   hipLaunchKernelGGL(inv_gather, dim3((unsigned)blocks), dim3(NT), 0, s, n, B[1].p,
-                     B[1].stride, B[1].ld, X.p, X.stride, X.ld, scr.ints, scr.int_stride, total);
+                     B[1].stride, B[1].ld, X.p, X.stride, X.ld, scr.ints, scr.int_stride, total,
+                     mask);
 }
 
 }  // namespace
@@ -635,7 +663,7 @@ Mat zinv_input(int n, const Mat& X, const InvScratch& scr) {
 }
 
 void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const Mat& X,
-          int* info, int precision, const InvScratch& scr) {
+          int* info, int precision, const InvScratch& scr, const int* mask) {
   QX_REQUIRE(n >= 1 && n <= kInvMaxN, "inv: block size must be in [1, 512]");
   QX_REQUIRE(!A.zero() && !X.zero(), "inv: null operand");
   QX_REQUIRE(scr.buf && scr.ints && scr.buf_stride >= (int64_t)n * n &&
@@ -652,7 +680,7 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const
   if (timer) timer->begin(s, &ev0);
 
   // n <= 32 in complex128: one wave per matrix, one launch
-  if (precision != QX_PREC_C64 && inv_small(s, batch, n, A, S, X, info)) {
+  if (precision != QX_PREC_C64 && inv_small(s, batch, n, A, S, X, info, mask)) {
     if (timer) timer->end(s, ev0, KernelTimer::INV, 8.0 * (double)n * n * n * batch);
     return;
   }
@@ -666,16 +694,27 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const
   const Mat start = s0 == 0 ? X : W;
   const bool in_place =
       (A.p == start.p) && S.zero() && A.ld == start.ld && A.stride == start.stride;
-  if (!in_place) ew_axpby(s, batch, n, n, Cplx{1.0, 0.0}, A, Cplx{-1.0, 0.0}, S, start);
+  if (!in_place) {
+    if (mask) {
+      const int64_t total = batch * (int64_t)n * n;
+      int64_t blocks = (total + NT - 1) / NT;
+      if (blocks > 65536 * 4) blocks = 65536 * 4;
+      hipLaunchKernelGGL(inv_masked_start, dim3((unsigned)blocks), dim3(NT), 0, s, n, A.p,
+                         A.stride, A.ld, S.p, S.stride, S.ld, start.p, start.stride, start.ld,
+                         total, mask);
+    } else {
+      ew_axpby(s, batch, n, n, Cplx{1.0, 0.0}, A, Cplx{-1.0, 0.0}, S, start);
+    }
+  }
 
   if (precision == QX_PREC_C64) {
-    if (big) run_inverse<float, 16, 2>(s, batch, n, X, scr, s0, info);
-    else if (NB == 16) run_inverse<float, 16, 1>(s, batch, n, X, scr, s0, info);
-    else run_inverse<float, 32, 1>(s, batch, n, X, scr, s0, info);
+    if (big) run_inverse<float, 16, 2>(s, batch, n, X, scr, s0, info, mask);
+    else if (NB == 16) run_inverse<float, 16, 1>(s, batch, n, X, scr, s0, info, mask);
+    else run_inverse<float, 32, 1>(s, batch, n, X, scr, s0, info, mask);
   } else {
-    if (big) run_inverse<double, 16, 2>(s, batch, n, X, scr, s0, info);
-    else if (NB == 16) run_inverse<double, 16, 1>(s, batch, n, X, scr, s0, info);
-    else run_inverse<double, 32, 1>(s, batch, n, X, scr, s0, info);
+    if (big) run_inverse<double, 16, 2>(s, batch, n, X, scr, s0, info, mask);
+    else if (NB == 16) run_inverse<double, 16, 1>(s, batch, n, X, scr, s0, info, mask);
+    else run_inverse<double, 32, 1>(s, batch, n, X, scr, s0, info, mask);
   }
   QX_HIP_CHECK(hipGetLastError());
   if (timer) timer->end(s, ev0, KernelTimer::INV, 8.0 * (double)n * n * n * batch);

@@ -89,7 +89,7 @@ struct Error {
 // Batched X = inv(A - S) for blocks n <= 32, one wave per matrix
 // (rgf_small.hip); returns false, having done nothing, for larger n.
 bool inv_small(hipStream_t st, int64_t batch, int n, const Mat& A, const Mat& S, const Mat& X,
-               int* info);
+               int* info, const int* mask = nullptr);
 
 // Fused selected inversion for stacks of small blocks (rgf_small.hip);
 // returns false, having done nothing, when the call is not eligible.
@@ -103,7 +103,8 @@ bool rgf_small_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, cons
                      const void* sg, const void* const* obc_r, const void* const* obc_l,
                      const void* const* obc_g, const qx_btd_layout* lxl, void* xl,
                      const qx_btd_layout* lxg, void* xg, const qx_btd_layout* lxr, void* xr,
-                     void* current, void* corner, double* density, uint32_t flags, int prec);
+                     void* current, void* corner, double* density, double* ldos,
+                     uint32_t flags, int prec);
 
 // C = alpha op(A) op(B) + beta C for `batch` matrices; m x n output, inner k.
 void zgemm(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB,
@@ -176,8 +177,11 @@ struct InvScratch {
   int* ints;
   int64_t int_stride;
 };
+// A non-null mask restricts the inverse to the matrices with mask[e] != 0;
+// the others (X included) are left untouched.
 void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S,
-          const Mat& X, int* info, int precision, const InvScratch& scr);
+          const Mat& X, int* info, int precision, const InvScratch& scr,
+          const int* mask = nullptr);
 // The buffer an inverse into X starts from (X itself or the scratch buffer):
 // a matrix formed there and passed as A (S zero) is inverted into X without
 // the initial copy.
@@ -195,9 +199,15 @@ void ew_contact_sigma(hipStream_t s, int64_t batch, int n, const Mat& S, const d
 // A[e * sA + k] = z_e S_k + T_k - Sig[e * sSig + k] (S, T, Sig nullable = 0).
 void ew_assemble(hipStream_t s, int64_t batch, int64_t nnz, const z2* z, const z2* S,
                  const z2* T, const z2* Sig, int64_t sSig, z2* A, int64_t sA);
-// out[e * out_stride + r] = Im X_e[r][r], r < n.
+// out[e * out_stride + r] = scale * Im X_e[r][r], r < n.
 void ew_diag_imag(hipStream_t s, int64_t batch, int n, const Mat& X, double* out,
-                  int64_t out_stride);
+                  int64_t out_stride, double scale = 1.0);
+// Newton-Schulz safety check of a complex64 block inverse X0 of M: R holds
+// I - M X0 (formed in FP64).  Per matrix, flagged[e] = !(||R||_F < tol)
+// (NaN-safe); a flagged matrix gets R = 0 (so the Newton-Schulz update
+// leaves X unchanged) and info[e] |= 2.
+void ew_ns_check(hipStream_t s, int64_t batch, int n, const Mat& R, double tol, int* flagged,
+                 int* info);
 // Y = I - Y for a stack of n x n matrices.
 void ew_ident_sub(hipStream_t s, int64_t batch, int n, const Mat& Y);
 // Y = D + scale (X - X^H) (Y may alias D, not X).
@@ -277,8 +287,10 @@ struct qx_handle {
   int device = 0;
   void* ws = nullptr;
   size_t ws_bytes = 0;
-  int* info = nullptr;  // device error flags
+  int* info = nullptr;  // device error flags: bit 0 singular, bit 1 FP64 fallback
   size_t info_len = 0;
+  int* mask = nullptr;  // per-matrix selection flags (device)
+  size_t mask_len = 0;
   std::string last_error;
   qx::KernelTimer* timer = nullptr;
   int parts = 4;                      // concurrent sub-batches of one RGF call
@@ -288,4 +300,5 @@ struct qx_handle {
   hipEvent_t fork = nullptr;
   void* ensure_ws(size_t bytes);
   int* ensure_info(size_t n);
+  int* ensure_mask(size_t n);
 };

@@ -10,6 +10,7 @@
 // stored (e.g. Sigma_{i,i+1} of a block-diagonal self-energy) are skipped:
 // they are exactly zero.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <future>
 
@@ -119,17 +120,33 @@ struct OzSubBatch {
 // temporaries.
 // X = inv(M) for M in Ain, which is X or zinv_input(n, X, scr): the products
 // that form M write it there, so the inverse starts without a copy.
+//
+// Safety net of the mixed mode: since I - M X1 = (I - M X0)^2 for the
+// Newton-Schulz iterate X1, ||R0||_F = ||I - M X0||_F < tol bounds the
+// residual of X1 by tol^2.  A matrix whose complex64 inverse misses that
+// (or is not finite: entries outside the FP32 range, a pivot lost to FP32
+// rounding) is inverted again by the FP64 pivoted inverse, selected per
+// matrix on the device through `mask`, and flagged with info bit 1
+// (reported by qx_sync_check as QX_FP64_FALLBACK).
 void block_inv(hipStream_t st, int64_t S, int n, const Mat& Ain, const Mat& X, int* info,
-               int prec, const InvScratch& scr, const Mat& Mc, const Mat& R) {
+               int prec, const InvScratch& scr, const Mat& Mc, const Mat& R, int* mask) {
   if (prec != QX_PREC_MIXED) {
     zinv(st, S, n, Ain, Mat(), X, info, prec == QX_PREC_C64 ? QX_PREC_C64 : QX_PREC_C128, scr);
     return;
   }
+  static const double tol = [] {
+    const char* e = std::getenv("QX_MIXED_TOL");
+    const double v = e ? std::atof(e) : 0.0;
+    return v > 0.0 ? v : 1e-2;
+  }();
   ew_axpby(st, S, n, n, kOne, Ain, kZero, Mat(), Mc);  // M
   zinv(st, S, n, Ain, Mat(), X, info, QX_PREC_C64, scr);
   zgemm(st, S, n, n, n, QX_OP_N, QX_OP_N, kOne, Mc, X, kZero, R, QX_PREC_C128);
-  ew_ident_sub(st, S, n, R);                          // R = I - M X
-  ew_axpby(st, S, n, n, kOne, X, kZero, Mat(), Mc);  // X0
+  ew_ident_sub(st, S, n, R);  // R = I - M X0
+  ew_ns_check(st, S, n, R, tol, mask, info);
+  zinv(st, S, n, Mc, Mat(), X, info, QX_PREC_C128, scr, mask);  // flagged: FP64 inverse
+  ew_axpby(st, S, n, n, kOne, X, kZero, Mat(), Mc);  // X0 (FP64 inverse where flagged)
+  // X0 + X0 R; a flagged matrix has R = 0 and keeps its FP64 inverse
   zgemm(st, S, n, n, n, QX_OP_N, QX_OP_N, kOne, Mc, R, kOne, X, QX_PREC_C64);
 }
 
@@ -180,6 +197,7 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   if (rgf_small_inv(h, st0, la, a, obc_r, lx, x, prec)) return;
   z2* ws = static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, false, prec)));
   int* info_all = h->ensure_info(Stot);
+  int* mask_all = prec == QX_PREC_MIXED ? h->ensure_mask(Stot) : nullptr;
   const Btd A0(la, a), X0(lx, x);
   char* oz_region = reinterpret_cast<char*>(ws) + rgf_slot_bytes(la, false, bmax);
 
@@ -193,6 +211,7 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const Slots W{ws, Stot, (int64_t)bmax * bmax, e0};
     const Obc OR{obc_r, e0};
     int* info = info_all + e0;
+    int* mask = mask_all ? mask_all + e0 : nullptr;
     std::vector<int> xd(N);
     for (int i = 0; i < N; ++i) xd[i] = i;
     int spare = N;
@@ -204,7 +223,8 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     {
       const Mat X0 = W.get(xd[0], bs(0)), M0 = zinv_input(bs(0), X0, scr);
       fold(st, S, bs(0), bs(0), A.diag(0), -1.0, OR.get(0, bs(0)), M0);
-      block_inv(st, S, bs(0), M0, X0, info, prec, scr, W.get(N + 6, bs(0)), W.get(N + 7, bs(0)));
+      block_inv(st, S, bs(0), M0, X0, info, prec, scr, W.get(N + 6, bs(0)), W.get(N + 7, bs(0)),
+                mask);
     }
     for (int i = 0; i + 1 < N; ++i) {
       const int j = i + 1, bi = bs(i), bj = bs(j);
@@ -217,7 +237,7 @@ void rgf_selected_inv(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
         Ajj = Mj;
       }
       zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Ajj, Mj, gp);
-      block_inv(st, S, bj, Mj, Xj, info, prec, scr, W.get(N + 6, bj), W.get(N + 7, bj));
+      block_inv(st, S, bj, Mj, Xj, info, prec, scr, W.get(N + 6, bj), W.get(N + 7, bj), mask);
     }
     // rgf.py:94-95
     if (!X.diag(N - 1).zero())
@@ -269,7 +289,8 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
                         const void* const* obc_g, const qx_btd_layout* lxl,
                         void* xl, const qx_btd_layout* lxg, void* xg,
                         const qx_btd_layout* lxr, void* xr, void* current,
-                        void* corner, double* density, uint32_t flags, int prec) {
+                        void* corner, double* density, double* ldos, uint32_t flags,
+                        int prec) {
   QX_REQUIRE(la && a, "selected_solve: null A");
   QX_REQUIRE(lxl && xl && lxg && xg, "selected_solve: null lesser/greater output");
   const bool want_r = (flags & QX_SOLVE_RETARDED) != 0;
@@ -289,10 +310,11 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   const int bmax = block_limits(la, "selected_solve");
   if (rgf_small_solve(h, st0, la, a, lsl, sl, lsg, sg, obc_r, obc_l, obc_g, lxl, xl, lxg, xg,
                       want_r ? lxr : nullptr, want_r ? xr : nullptr, want_cur ? current : nullptr,
-                      corner, density, flags, prec))
+                      corner, density, ldos, flags, prec))
     return;
   z2* ws = static_cast<z2*>(h->ensure_ws(rgf_workspace_bytes(la, true, prec)));
   int* info_all = h->ensure_info(Stot);
+  int* mask_all = prec == QX_PREC_MIXED ? h->ensure_mask(Stot) : nullptr;
   char* oz_region = reinterpret_cast<char*>(ws) + rgf_slot_bytes(la, true, bmax);
   const Btd A0(la, a), SL0(lsl, sl), SG0(lsg, sg), XL0(lxl, xl), XG0(lxg, xg),
       XR0(want_r ? lxr : nullptr, want_r ? xr : nullptr);
@@ -330,12 +352,16 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
   const Slots W{ws, Stot, (int64_t)bmax * bmax, e0};
   const Obc OR{obc_r, e0}, OL{obc_l, e0}, OG{obc_g, e0};
   int* info = info_all + e0;
+  int* mask = mask_all ? mask_all + e0 : nullptr;
   z2* cur = current ? static_cast<z2*>(current) + e0 * (N - 1) : nullptr;
   // density Im diag(G^<): row of n_orb per matrix, block i at orb_off[i]
   std::vector<int64_t> orb_off(N + 1, 0);
   for (int i = 0; i < N; ++i) orb_off[i + 1] = orb_off[i] + A.bs(i);
   const int64_t n_orb = orb_off[N];
   double* dens = density ? density + e0 * n_orb : nullptr;
+  // LDOS -Im diag(G^R) / (2 pi), same layout, as each G^R_ii is finished
+  double* lds = ldos ? ldos + e0 * n_orb : nullptr;
+  constexpr double kLdos = -0.15915494309189533577;  // -1 / (2 pi)
   // corner block G^R_{0,N-1}: ping-pong slots + product temp, b0 x b_j
   // matrices with leading dimension bmax
   const int b0 = A.bs(0), bl = A.bs(N - 1);
@@ -401,7 +427,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     const int b0 = bs(0);
     const Mat X0 = W.get(sr[0], b0), M0 = zinv_input(b0, X0, scr);
     fold(st, S, b0, b0, A.diag(0), -1.0, OR.get(0, b0), M0);
-    block_inv(st, S, b0, M0, X0, info, prec, scr, W.get(NS1, b0), W.get(NS2, b0));
+    block_inv(st, S, b0, M0, X0, info, prec, scr, W.get(NS1, b0), W.get(NS2, b0), mask);
     if (release.p) {
       QX_HIP_CHECK(hipEventRecord(stg_ev, st));
       release.p->set_value();
@@ -449,7 +475,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       Ajj = Mj;
     }
     zgemm_in(st, S, bj, bj, bi, QX_OP_N, QX_OP_N, kMinus, T1, A.upper(i), kOne, Ajj, Mj, gp);
-    block_inv(st, S, bj, Mj, Xrj, info, prec, scr, W.get(NS1, bj), W.get(NS2, bj));
+    block_inv(st, S, bj, Mj, Xrj, info, prec, scr, W.get(NS1, bj), W.get(NS2, bj), mask);
     if (corner) {
       // g_{0,j} = -(g_{0,i} a_ij) x^R_jj
       zgemm(st, S, b0, bj, bi, QX_OP_N, QX_OP_N, kOne, W.get(gc_cur, bmax), A.upper(i), kZero,
@@ -511,6 +537,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
     }
     if (want_r && !XR.diag(N - 1).zero())
       ew_axpby(st, S, bn, bn, kOne, W.get(sr[N - 1], bn), kZero, Mat(), XR.diag(N - 1));
+    if (lds) ew_diag_imag(st, S, bn, W.get(sr[N - 1], bn), lds + orb_off[N - 1], n_orb, kLdos);
   }
 
   // ---- backward sweep (rgf.py:308-404) ----
@@ -625,6 +652,7 @@ void rgf_selected_solve(qx_handle* h, hipStream_t st0, const qx_btd_layout* la,
       std::swap(spare, sr[i]);
       gr[i] = Z;
     }
+    if (lds) ew_diag_imag(st, S, bi, gr[i], lds + orb_off[i], n_orb, kLdos);
   }
   });
   // released once the waits enqueued on it have been satisfied

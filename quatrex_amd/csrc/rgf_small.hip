@@ -555,9 +555,11 @@ __device__ __forceinline__ void sinv(int n, M X, int* info, Shared& sh) {
 __global__ __launch_bounds__(kThreads) void inv_small_wg_kernel(int n, const z2* A, int64_t sA,
                                                                 int lda, const z2* S, int64_t sS,
                                                                 int lds, z2* X, int64_t sX,
-                                                                int ldx, int* info) {
+                                                                int ldx, int* info,
+                                                                const int* mask) {
   __shared__ WgInvShared sh;
   const int64_t e = blockIdx.x;
+  if (mask && !mask[e]) return;  // uniform over the workgroup
   sinv_wg(n, A + e * sA, lda, S ? S + e * sS : nullptr, lds, X + e * sX, ldx, info + e, sh);
 }
 
@@ -566,11 +568,12 @@ __global__ __launch_bounds__(kThreads) void inv_small_wg_kernel(int n, const z2*
 __global__ __launch_bounds__(kThreads) void inv_small_kernel(int64_t batch, int n, const z2* A,
                                                              int64_t sA, int lda, const z2* S,
                                                              int64_t sS, int lds, z2* X,
-                                                             int64_t sX, int ldx, int* info) {
+                                                             int64_t sX, int ldx, int* info,
+                                                             const int* mask) {
   __shared__ WaveShared sh[kThreads / 64];
   const int w = threadIdx.x >> 6;
   const int64_t e = (int64_t)blockIdx.x * (kThreads / 64) + w;
-  if (e < batch)
+  if (e < batch && !(mask && !mask[e]))
     sinv_wave(n, A + e * sA, lda, S ? S + e * sS : nullptr, lds, X + e * sX, ldx, info + e,
               sh[w]);
 }
@@ -881,7 +884,7 @@ bool fits32(const qx_btd_layout* l, int N) {
 // 49 / 69 us at 256, 207 vs 623 / 296 us at 4096).  QX_INV_SMALL=0 sends every
 // size to the split panel kernels, 1 to the per-wave kernel.
 bool inv_small(hipStream_t st, int64_t batch, int n, const Mat& A, const Mat& S, const Mat& X,
-               int* info) {
+               int* info, const int* mask) {
   const char* env = std::getenv("QX_INV_SMALL");
   const int mode = env ? std::atoi(env) : -1;
   if (mode == 0) return false;
@@ -892,10 +895,11 @@ bool inv_small(hipStream_t st, int64_t batch, int n, const Mat& A, const Mat& S,
   if (per_wave) {
     const int64_t blocks = (batch + kThreads / 64 - 1) / (kThreads / 64);
     hipLaunchKernelGGL(inv_small_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, st, batch,
-                       n, A.p, A.stride, A.ld, S.p, S.stride, S.ld, X.p, X.stride, X.ld, info);
+                       n, A.p, A.stride, A.ld, S.p, S.stride, S.ld, X.p, X.stride, X.ld, info,
+                       mask);
   } else {
     hipLaunchKernelGGL(inv_small_wg_kernel, dim3((unsigned)batch), dim3(kThreads), 0, st, n,
-                       A.p, A.stride, A.ld, S.p, S.stride, S.ld, X.p, X.stride, X.ld, info);
+                       A.p, A.stride, A.ld, S.p, S.stride, S.ld, X.p, X.stride, X.ld, info, mask);
   }
   QX_HIP_CHECK(hipGetLastError());
   return true;
@@ -909,9 +913,11 @@ bool rgf_small_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, cons
                      const void* sg, const void* const* obc_r, const void* const* obc_l,
                      const void* const* obc_g, const qx_btd_layout* lxl, void* xl,
                      const qx_btd_layout* lxg, void* xg, const qx_btd_layout* lxr, void* xr,
-                     void* current, void* corner, double* density, uint32_t flags, int prec) {
+                     void* current, void* corner, double* density, double* ldos,
+                     uint32_t flags, int prec) {
   const char* env = std::getenv("QX_RGF_FUSED");
   if (env && std::atoi(env) == 0) return false;
+  if (ldos) return false;  // the per-product path writes the LDOS
   if (prec != QX_PREC_C128 && prec != QX_PREC_C128_3M) return false;
   const int N = la->n_blocks;
   if (N < 1 || N > kSN) return false;

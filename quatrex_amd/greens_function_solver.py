@@ -236,7 +236,8 @@ class RGF(GFSolver):
                        obc_blocks: OBCBlocks | None = None, out=None,
                        return_retarded: bool = False, return_current: bool = False,
                        *, corner: torch.Tensor | None = None,
-                       density: torch.Tensor | None = None):
+                       density: torch.Tensor | None = None,
+                       ldos: torch.Tensor | None = None):
         """rgf.py:115-414.
 
         Extensions (keyword-only, filled in place):
@@ -244,7 +245,9 @@ class RGF(GFSolver):
         G^R_{0,N-1} for the transmission; ``density`` (float64 device tensor
         (*stack, sum of block sizes)) receives Im diag(G^<), the electron
         density of quatrex/core/observables.py:57-86 (no overlap), written as
-        each diagonal block is finished.
+        each diagonal block is finished; ``ldos`` (same shape) receives the
+        electron LDOS -Im diag(G^R) / (2 pi) of quatrex/core/scba.py:572-576
+        (no overlap), whether or not ``return_retarded`` is set.
         """
         lib = _lib.load()
         _lib.require_device_tensor(a.data, "a.data")
@@ -305,15 +308,19 @@ class RGF(GFSolver):
                                      "(*stack, b_0, b_{N-1})")
                 corner_ptr = ctypes.c_void_p(corner.data_ptr()
                                              + s0 * inner * nl * n0 * corner.element_size())
-            dens_ptr = None
-            if density is not None:
-                n_orb = int(np.sum(a.block_sizes))
-                if (density.dtype != torch.float64 or not density.is_contiguous()
-                        or tuple(density.shape) != tuple(a.shape[:-2]) + (n_orb,)):
-                    raise ValueError("density must be a contiguous float64 tensor of shape "
+            n_orb = int(np.sum(a.block_sizes))
+
+            def orbital_ptr(t, name):
+                if t is None:
+                    return None
+                if (t.dtype != torch.float64 or not t.is_contiguous()
+                        or tuple(t.shape) != tuple(a.shape[:-2]) + (n_orb,)):
+                    raise ValueError(f"{name} must be a contiguous float64 tensor of shape "
                                      "(*stack, sum of block sizes)")
-                dens_ptr = ctypes.c_void_p(density.data_ptr()
-                                           + s0 * inner * n_orb * density.element_size())
+                return ctypes.c_void_p(t.data_ptr() + s0 * inner * n_orb * t.element_size())
+
+            dens_ptr = orbital_ptr(density, "density")
+            ldos_ptr = orbital_ptr(ldos, "ldos")
             cur_ptr = None
             if current is not None:
                 cur_ptr = ctypes.c_void_p(current.data_ptr()
@@ -322,7 +329,7 @@ class RGF(GFSolver):
                 h, st, A.lay(), A.ptr(), SL.lay(), SL.ptr(), SG.lay(), SG.ptr(), o_r, o_l, o_g,
                 XL.lay(), XL.ptr(), XG.lay(), XG.ptr(),
                 XR.lay() if XR else None, XR.ptr() if XR else None, cur_ptr, corner_ptr,
-                dens_ptr, flags, prec)
+                dens_ptr, ldos_ptr, flags, prec)
             _lib.check(rc, h, "RGF.selected_solve")
             for X in (XL, XG, XR):
                 if X is not None:

@@ -96,7 +96,7 @@ def test_library_exports_declared_symbols():
         assert hasattr(lib, name), name
     assert set(declared) == set(_lib.EXPORTED_SYMBOLS)
     lib.qx_abi_version.restype = ctypes.c_int
-    assert lib.qx_abi_version() == _lib.ABI_VERSION == 2
+    assert lib.qx_abi_version() == _lib.ABI_VERSION == 3
 
 
 def test_solver_refuses_host_tensors():

@@ -1,0 +1,117 @@
+"""FP64 safety net of the mixed arithmetic (complex64 block inverses with one
+Newton-Schulz step).
+
+I - M X1 = (I - M X0)^2 for the Newton-Schulz iterate, so a complex64
+inverse X0 whose residual ||I - M X0||_F is not below the tolerance (1e-2)
+cannot be repaired by the step; such a block is inverted again by the FP64
+pivoted inverse (selected per matrix on the device) and reported as a
+RuntimeWarning.  A block with condition number ~1e9 triggers it: the
+complex64 LU of it has an O(1) residual.
+
+- one block: the mixed selected inversion of an ill-conditioned block equals
+  the complex128 one bitwise (the FP64 inverse is kept, the Newton-Schulz
+  update adds exactly zero), only that matrix of the batch falls back, and
+  the well-conditioned matrices keep their complex64 + Newton-Schulz values;
+- a 3-block selected solve whose first diagonal block is ill-conditioned,
+  against the NumPy oracle at the mixed arithmetic's bar.  The couplings are
+  weak (1e-6), so G^R_00 ~ inv(A_00) carries no cancellation that complex64
+  products could not follow; without the FP64 fallback the O(1) residual of
+  the complex64 inverse of A_00 would give an O(1) error in G^R_00.
+"""
+
+import warnings
+
+import numpy as np
+import pytest
+import scipy.sparse as sps
+import torch
+
+import oracle.rgf as orgf
+from oracle.dsdbcoo import DSDBCOO as ODSDBCOO
+from quatrex_amd.datastructures import DSDBCOO
+from quatrex_amd.greens_function_solver import RGF
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _ill(rng, n, cond):
+    """n x n complex matrix with singular values from 1 down to 1 / cond."""
+    u, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    v, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    return (u * np.logspace(0, -np.log10(cond), n)) @ v.conj().T
+
+
+def _well(rng, n):
+    return rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 2 * n * np.eye(n)
+
+
+def _to_gpu(o, dev):
+    return DSDBCOO(torch.as_tensor(o.data, device=dev), o.rows, o.cols, o.block_sizes,
+                   o.global_stack_shape)
+
+
+def _one_block(mats):
+    n = mats[0].shape[0]
+    data = np.stack([m.reshape(-1) for m in mats])
+    rows = np.repeat(np.arange(n), n).astype(np.int32)
+    cols = np.tile(np.arange(n), n).astype(np.int32)
+    return ODSDBCOO(data, rows, cols, np.array([n]), (len(mats),))
+
+
+@pytest.mark.parametrize("n", [24, 96])
+def test_single_block_falls_back_to_fp64(device, n):
+    rng = np.random.default_rng(n)
+    mats = [_well(rng, n), _ill(rng, n, 1e9), _well(rng, n)]
+    a = _to_gpu(_one_block(mats), device)
+    x128 = RGF(3, precision="c128").selected_inv(a).data.cpu().numpy()
+    with pytest.warns(RuntimeWarning, match="1 matrices"):
+        xm = RGF(3, precision="mixed").selected_inv(a).data.cpu().numpy()
+    assert np.array_equal(xm[1], x128[1])
+    assert rel_err(xm[1].reshape(n, n), np.linalg.inv(mats[1])) < 1e-5
+    for e in (0, 2):
+        # complex64 inverse + one Newton-Schulz step, no fallback
+        assert not np.array_equal(xm[e], x128[e])
+        assert rel_err(xm[e].reshape(n, n), np.linalg.inv(mats[e])) < 1e-10
+    # a well-conditioned batch raises no warning
+    b = _to_gpu(_one_block([mats[0], mats[2]]), device)
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        RGF(2, precision="mixed").selected_inv(b)
+
+
+def test_selected_solve_with_ill_conditioned_block(device):
+    rng = np.random.default_rng(11)
+    b, nb, ne = 32, 3, 2
+    bs = np.array([b] * nb)
+    n = b * nb
+    dense = np.zeros((ne, n, n), complex)
+    for e in range(ne):
+        for i in range(nb):
+            s = slice(i * b, (i + 1) * b)
+            dense[e, s, s] = _ill(rng, b, 1e9) if i == 0 else _well(rng, b)
+            if i + 1 < nb:
+                t = slice((i + 1) * b, (i + 2) * b)
+                dense[e, s, t] = 1e-6 * (rng.standard_normal((b, b)) + 1j * rng.standard_normal((b, b)))
+                dense[e, t, s] = 1e-6 * (rng.standard_normal((b, b)) + 1j * rng.standard_normal((b, b)))
+    sig = np.zeros_like(dense)
+    for e in range(ne):
+        for i in range(nb):
+            s = slice(i * b, (i + 1) * b)
+            c = rng.standard_normal((b, b)) + 1j * rng.standard_normal((b, b))
+            sig[e, s, s] = 1j * c @ c.conj().T / b
+    pat = sps.coo_matrix(np.abs(dense).sum(0) > 0)
+    oa = ODSDBCOO.from_sparray(pat.astype(complex), bs, (ne,))
+    oa.data[:] = dense[:, oa.rows, oa.cols]
+    osg = ODSDBCOO.from_sparray(pat.astype(complex), bs, (ne,))
+    osg.data[:] = sig[:, osg.rows, osg.cols]
+    rl, rg, rr = orgf.RGF(ne).selected_solve(oa, osg, osg, return_retarded=True)
+    a, s_ = _to_gpu(oa, device), _to_gpu(osg, device)
+    with pytest.warns(RuntimeWarning, match="FP64"):
+        xl, xg, xr = RGF(ne, precision="mixed").selected_solve(a, s_, s_, return_retarded=True)
+    for g, o in ((xl, rl), (xg, rg), (xr, rr)):
+        for i in range(nb):
+            for j in (i - 1, i, i + 1):
+                if 0 <= j < nb:
+                    # the mixed arithmetic's per-block bar
+                    assert rel_err(g.blocks[i, j].cpu().numpy(), o.blocks[i, j]) < 5e-2

@@ -496,3 +496,21 @@ def test_edge_block_structures(device, block_sizes):
     got = RGF().selected_solve(a, bl, bl, return_retarded=True)
     for g, o in zip(got, refs):
         assert close(g, o, 1e-10)
+
+
+@pytest.mark.parametrize("shape", [(16, 8, 32, 0), (2, 6, 96, 3)], ids=["8x32", "6x96"])
+def test_ldos_against_oracle(device, shape):
+    """LDOS -Im diag(G^R) / (2 pi) (quatrex/core/scba.py:572-576) written by
+    the sweep, with and without the retarded output, against the oracle's
+    diagonal G^R blocks."""
+    ne, nb, b, seed = shape
+    w = make_workload(seed=seed, n_energies=ne, n_blocks=nb, block_size=b)
+    oa, osl, osg = _oracle_containers(w)
+    rr = orgf.RGF(ne).selected_solve(oa, osl, osg, return_retarded=True)[2]
+    ref = -np.concatenate([np.diagonal(rr.blocks[i, i], axis1=-2, axis2=-1).imag
+                           for i in range(nb)], -1) / (2 * np.pi)
+    a, sl, sg = to_gpu(oa, device), to_gpu(osl, device), to_gpu(osg, device)
+    for ret in (True, False):
+        ldos = torch.zeros((ne, nb * b), dtype=torch.float64, device=device)
+        RGF(ne).selected_solve(a, sl, sg, return_retarded=ret, ldos=ldos)
+        assert rel_err(ldos.cpu().numpy(), ref) < 1e-10
