@@ -173,6 +173,18 @@ def new_handle(device: torch.device) -> ctypes.c_void_p:
     return h
 
 
+def workspace_bytes(h, block_sizes, n_stack: int, precision: str = "c128") -> int:
+    """Bytes of handle workspace an RGF selected_solve of ``n_stack``
+    matrices with these block sizes grows the handle to (qx_workspace_query)."""
+    n = len(block_sizes)
+    lay, keep = make_layout(block_sizes, n_stack, 0, np.zeros(n), np.zeros(max(n - 1, 0)),
+                            np.zeros(max(n - 1, 0)))
+    out = ctypes.c_size_t(0)
+    check(load().qx_workspace_query(h, ctypes.byref(lay), int(RGF_PRECISIONS[precision]),
+                                    ctypes.byref(out)), h, "workspace_query")
+    return int(out.value)
+
+
 def set_tuning(h, streams: int, gemm_tile: int = 0) -> None:
     """Sub-batch streams of one RGF call and GEMM kernel (GEMM_*; 0 automatic)."""
     check(load().qx_set_tuning(h, int(streams), int(gemm_tile)), h, "set_tuning")

@@ -98,6 +98,7 @@ class Device:
         # container of this device
         self._meta = {True: ({}, {}), False: ({}, {})}
         self._sigma_views: dict = {}
+        self._ws_bytes: dict = {}
 
     def container(self, data: torch.Tensor, diag_only: bool) -> DSDBCOO:
         r, c = (self.drows, self.dcols) if diag_only else (self.rows, self.cols)
@@ -121,15 +122,27 @@ class Device:
     def flops_per_energy(self) -> float:
         return required_flops(self.n_blocks, self.b)
 
-    def lane_bytes_per_energy(self) -> int:
-        """HBM a lane needs per energy of a chunk: A, two block-diagonal
-        outputs, the corner block and the RGF workspace slots (the fused
-        small-block solve keeps 6 N + 16 slots of 32 x 32)."""
-        b2 = self.b * self.b
-        ws = (3 * self.n_blocks + 22) * b2
-        if self.n_blocks <= 16 and self.b <= 32:
-            ws = max(ws, (6 * self.n_blocks + 16) * 1024)
-        return 16 * (self.nnz + 2 * self.nnz_d + b2 + ws)
+    def pool_bytes_per_energy(self) -> int:
+        """Lane pool bytes per energy of a chunk (A, two block-diagonal
+        outputs, the corner block) plus the transient Gamma and transmission
+        temporaries of the chunk (six b0 x b0 blocks)."""
+        b0 = self.block_sizes[0]
+        return 16 * (self.nnz + 2 * self.nnz_d + self.block_sizes[-1] * b0 + 6 * b0 * b0)
+
+    def workspace_bytes(self, n_energies: int, precision: str = "c128") -> int:
+        """Handle workspace of a chunk of ``n_energies`` (qx_workspace_query)."""
+        key = (n_energies, precision)
+        v = self._ws_bytes.get(key)
+        if v is None:
+            v = _lib.workspace_bytes(_lib.handle(self.device), self.block_sizes, n_energies,
+                                     precision)
+            self._ws_bytes[key] = v
+        return v
+
+    def lane_bytes_per_energy(self, precision: str = "c128") -> int:
+        """HBM a lane needs per energy of a chunk: its pool share and the RGF
+        workspace (exact per energy: the workspace is linear in the chunk)."""
+        return self.pool_bytes_per_energy() + self.workspace_bytes(1, precision)
 
     def resident_bytes(self) -> int:
         return 16 * (2 * self.nnz + 2 * self.nnz_d) + 24 * self.n_energies * self.n_orb
@@ -158,6 +171,16 @@ class Lane:
         if elems > self._pool.numel():
             self._views.clear()
             self._pool = torch.empty(elems, dtype=torch.complex128, device=self.device)
+
+    def release(self) -> None:
+        """Give back the pool and the handle's workspace (the handle is
+        recreated; its workspace grows again on the next call)."""
+        self.stream.synchronize()
+        self._views.clear()
+        self._pool = torch.empty(0, dtype=torch.complex128, device=self.device)
+        _lib.load().qx_destroy(self.h)
+        self.h = _lib.new_handle(self.device)
+        _lib.set_tuning(self.h, streams=1, gemm_tile=0)
 
     def views(self, d: Device, c: int):
         """(A, X^<, X^>, corner) containers for a chunk of c energies of d."""
@@ -320,7 +343,7 @@ class UnevenBatch:
 
     def chunk_for(self, d: Device) -> int:
         c = int(self.tuning.get(d.b, {}).get("chunk", self.default_chunk))
-        cap = max(1, self.lane_budget_bytes // d.lane_bytes_per_energy())
+        cap = max(1, self.lane_budget_bytes // d.lane_bytes_per_energy(self.precision))
         return max(1, min(c, cap, d.n_energies))
 
     def plan(self) -> None:
@@ -472,7 +495,7 @@ def autotune(block_sizes, device, precision: str = "c128", n_blocks: int = 6,
         for chunk in CHUNKS:
             d = make_device(seed=b, n_energies=chunk, n_blocks=n_blocks, block_size=b,
                             device=dev)
-            if chunk * d.lane_bytes_per_energy() > budget_bytes:
+            if chunk * d.lane_bytes_per_energy(precision) > budget_bytes:
                 continue
             for tile in TILES:
                 tune = {"gemm_tile": tile, "chunk": chunk}

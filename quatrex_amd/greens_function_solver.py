@@ -100,23 +100,30 @@ class _Bound:
         upper = np.full(max(n - 1, 0), -1, np.int64)
         lower = np.full(max(n - 1, 0), -1, np.int64)
         arrs = {"d": diag, "u": upper, "l": lower}
-        # A read-only stack broadcast over its first axis (stride 0, e.g. an
-        # energy-independent self-energy expanded to the batch) is passed
+        # A read-only stack broadcast over its only stack axis (stride 0, e.g.
+        # an energy-independent self-energy expanded to the batch) is passed
         # with stack stride 0: every matrix of the batch reads the same data.
-        bcast = (not output and mat.data.dim() == 2 and mat.data.shape[0] > 1
-                 and mat.data.stride(0) == 0)
+        # Any other non-contiguous input (a broadcast over the first of
+        # several stack axes repeats with the period of the inner axes, which
+        # one stack stride cannot express) is copied contiguously for this
+        # batch; outputs must be contiguous.
+        data = mat.data
+        bcast = (not output and data.dim() == 2 and data.shape[0] > 1 and data.stride(0) == 0
+                 and data.stride(1) == 1)
+        self._copy = None
         if not staged:
             for (kind, i), off in direct.items():
                 arrs[kind][i] = off
             if bcast:
-                self.base, stride = mat.data.data_ptr(), 0
-            else:
-                if mat.data.stride(-1) != 1 or (mat.data.dim() >= 2
-                                                and mat.data.stride(-2) != mat.nnz):
-                    raise ValueError("DSDBCOO.data must be contiguous (or a stride-0 "
-                                     "broadcast of one stack entry for inputs)")
-                self.base = mat.data.data_ptr() + s0 * inner * mat.nnz * mat.data.element_size()
+                self.base, stride = data.data_ptr(), 0
+            elif data.is_contiguous():
+                self.base = data.data_ptr() + s0 * inner * mat.nnz * data.element_size()
                 stride = mat.nnz
+            elif output:
+                raise ValueError("an output DSDBCOO.data must be contiguous")
+            else:
+                self._copy = data[s0:s1].reshape(self.S, mat.nnz).contiguous()
+                self.base, stride = self._copy.data_ptr(), mat.nnz
         else:
             # all blocks of this container go through one dense staging buffer
             keys = sorted(set(direct) | set(staged), key=lambda k: (k[1], k[0]))

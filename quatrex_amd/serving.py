@@ -69,6 +69,13 @@ def next_admission(needs, used: int, budget: int):
     return None
 
 
+def lane_increment(retained: tuple, need: tuple) -> int:
+    """Bytes a lane retaining (pool, workspace) = ``retained`` adds when it
+    serves a request needing ``need``: each buffer only grows, to the larger
+    of the two, and is kept after the request."""
+    return max(0, need[0] - retained[0]) + max(0, need[1] - retained[1])
+
+
 @dataclass
 class Server:
     device: torch.device
@@ -84,6 +91,8 @@ class Server:
         self.devices: dict[str, Device] = {}
         self.lanes = [Lane(self.device, i) for i in range(self.n_lanes)]
         self.resident = 0
+        # (pool, workspace) bytes each lane keeps between requests
+        self.retained = [(0, 0)] * self.n_lanes
 
     def add_device(self, key: str, workload) -> Device:
         """Register a device; its Hamiltonian and Sigma blocks stay resident."""
@@ -94,13 +103,24 @@ class Server:
 
     def _chunk(self, d: Device) -> int:
         c = int(self.tuning.get(d.b, {}).get("chunk", self.chunk))
-        return max(1, min(c, self.lane_budget_bytes // d.lane_bytes_per_energy()))
+        return max(1, min(c, self.lane_budget_bytes // d.lane_bytes_per_energy(self.precision)))
 
-    def _need(self, req: Request) -> int:
+    def _lane_need(self, req: Request) -> tuple:
+        """(pool, workspace) bytes a lane grows to for the request's chunks."""
         d = self.devices[req.key]
         c = min(self._chunk(d), len(req.energies))
-        return c * d.lane_bytes_per_energy() + 24 * len(req.energies) * (d.n_orb + 1) + \
+        return c * d.pool_bytes_per_energy(), d.workspace_bytes(c, self.precision)
+
+    def _extra(self, req: Request) -> int:
+        """Bytes a request holds while it is served: its energies and
+        results, and its own Sigma blocks."""
+        d = self.devices[req.key]
+        return 24 * len(req.energies) * (d.n_orb + 1) + \
             (64 * d.nnz_d if req.sigma_lesser is not None else 0)
+
+    def _need(self, req: Request) -> int:
+        """Bytes a request adds on a lane that retains nothing."""
+        return sum(self._lane_need(req)) + self._extra(req)
 
     def _work(self, lane: Lane, req: Request) -> None:
         d = self.devices[req.key]
@@ -135,7 +155,16 @@ class Server:
         cond = threading.Condition()
         queue: list[Request] = []
         idle = list(range(self.n_lanes))
-        state = {"used": self.resident, "done": 0, "error": None}
+        # device bytes charged: resident devices, what every lane retains,
+        # and the requests in service
+        state = {"extra": 0, "done": 0, "error": None}
+        charged = lambda: self.resident + sum(map(sum, self.retained)) + state["extra"]  # noqa: E731
+
+        def best_lane(req):
+            """Idle lane that grows least for req, and the bytes it adds."""
+            need = self._lane_need(req)
+            return min(((lane_increment(self.retained[k], need), k) for k in idle),
+                       default=(None, None))
         t_start = time.perf_counter()
         now = lambda: time.perf_counter() - t_start
 
@@ -146,7 +175,7 @@ class Server:
                 state["error"] = e
             with cond:
                 req.t_done = now()
-                state["used"] -= self._need(req)
+                state["extra"] -= self._extra(req)
                 state["done"] += 1
                 idle.append(k)
                 cond.notify_all()
@@ -161,16 +190,27 @@ class Server:
                 while i < len(pending) and pending[i].arrival <= t:
                     queue.append(pending[i])
                     i += 1
-                # admit: first request in arrival order that fits, while a
-                # lane is idle
+                # admit: first request in arrival order that fits, on the
+                # idle lane that grows least, while a lane is idle; if none
+                # fits, idle lanes give back what they retain
                 while idle and queue:
-                    q = next_admission([self._need(r) for r in queue], state["used"],
-                                       self.hbm_budget_bytes)
+                    adds = [best_lane(r)[0] + self._extra(r) for r in queue]
+                    q = next_admission(adds, charged(), self.hbm_budget_bytes)
                     if q is None:
-                        break
+                        held = [k for k in idle if sum(self.retained[k]) > 0]
+                        if not held:
+                            break
+                        k = max(held, key=lambda j: sum(self.retained[j]))
+                        self.lanes[k].release()
+                        self.retained[k] = (0, 0)
+                        continue
                     req = queue.pop(q)
-                    k = idle.pop(0)
-                    state["used"] += self._need(req)
+                    _, k = best_lane(req)
+                    idle.remove(k)
+                    need = self._lane_need(req)
+                    self.retained[k] = (max(self.retained[k][0], need[0]),
+                                        max(self.retained[k][1], need[1]))
+                    state["extra"] += self._extra(req)
                     req.t_admit, req.lane = now(), k
                     th = threading.Thread(target=lane_main, args=(k, req))
                     threads.append(th)

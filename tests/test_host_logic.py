@@ -66,3 +66,46 @@ def test_admission_first_fit_in_arrival_order():
     assert next_admission([13], used=0, budget=12) is None
     assert next_admission([], used=0, budget=12) is None
     assert next_admission([12], used=0, budget=12) == 0         # exact fit admitted
+
+
+def _bound_case(monkeypatch, data, stack_shape, s0, s1, output=False):
+    import torch  # noqa: F401
+
+    from quatrex_amd import _lib
+    from quatrex_amd.datastructures import DSDBCOO, btd_pattern
+    from quatrex_amd.greens_function_solver import _Bound
+
+    monkeypatch.setattr(_lib, "require_device_tensor", lambda t, what: None)
+    bs = [2, 3]
+    rows, cols, _ = btd_pattern(bs, offdiag=True)
+    m = DSDBCOO(data, rows, cols, bs, stack_shape)
+    return m, _Bound(m, s0, s1, output=output)
+
+
+@pytest.mark.parametrize("s0,s1", [(0, 4), (1, 3)])
+def test_bound_strides_for_contiguous_and_broadcast_stacks(monkeypatch, s0, s1):
+    """Block layouts handed to the C ABI address only memory the tensor owns:
+    contiguous stacks by offset and stride nnz; a stride-0 broadcast over the
+    only stack axis with stack stride 0; a broadcast over the first of two
+    stack axes (period K, not expressible by one stride) through a
+    contiguous copy of the batch; outputs only when contiguous."""
+    import torch
+
+    nnz = 2 * 2 + 2 * 3 * 2 + 3 * 3
+    E, K = 4, 3
+    full = torch.arange(E * K * nnz, dtype=torch.float64).to(torch.complex128).reshape(E, K, nnz)
+    m, b = _bound_case(monkeypatch, full, (E, K), s0, s1)
+    assert b.layout.stack_stride == nnz and b.layout.n_stack == (s1 - s0) * K
+    assert b.base == full.data_ptr() + s0 * K * nnz * 16
+    one = torch.arange(nnz, dtype=torch.float64).to(torch.complex128).reshape(1, nnz)
+    m, b = _bound_case(monkeypatch, one.expand(E, nnz), (E,), s0, s1)
+    assert b.layout.stack_stride == 0 and b.base == one.data_ptr()
+    exp = torch.arange(K * nnz, dtype=torch.float64).to(torch.complex128).reshape(1, K, nnz).expand(E, K, nnz)
+    m, b = _bound_case(monkeypatch, exp, (E, K), s0, s1)
+    assert m.data.stride(0) == 0  # the container keeps the broadcast
+    assert b.layout.stack_stride == nnz and b.layout.n_stack == (s1 - s0) * K
+    assert b._copy is not None and b.base == b._copy.data_ptr()
+    assert b._copy.numel() == (s1 - s0) * K * nnz
+    assert torch.equal(b._copy, exp[s0:s1].reshape(-1, nnz))
+    with pytest.raises(ValueError, match="contiguous"):
+        _bound_case(monkeypatch, exp, (E, K), s0, s1, output=True)
