@@ -10,15 +10,16 @@
 //    FP64 for T <= 52.  The truncation error is at most 2^(-T-1) of the
 //    line's largest entry.
 // 2. Residues.  For pairwise coprime moduli m_1..m_N <= 256 (product M) the
-//    symmetric residues of a'_re, a'_im and a'_re + a'_im modulo m_l are int8
-//    values; they are written as planes [l][re, im, re + im][line][k_pad],
-//    k contiguous and zero-padded to a multiple of 64.
-// 3. Products (oz2_gemm_kernel).  Per modulus the three products
-//    P1 = Re Re, P2 = Im Im, P3 = (Re + Im)(Re + Im) run on
-//    v_mfma_i32_16x16x64_i8 with exact int32 accumulation (|P| <= k 2^14).
-//    Karatsuba is exact here: modulo m_l, Re C' = P1 - P2 and
-//    Im C' = P3 - P1 - P2 with no cancellation error.
-// 4. Chinese remainder reconstruction in the epilogue.  With
+//    symmetric residues of a'_re and a'_im modulo m_l are int8 values; they
+//    are written as planes [l][re, im][line][k_pad], k contiguous and
+//    zero-padded to a multiple of 64.
+// 3. Products (oz2_mod_kernel).  Per modulus the four real products of the
+//    complex product, Re Re, Im Im, Re Im and Im Re, run on
+//    v_mfma_i32_16x16x64_i8 with exact int32 accumulation (|P| <= k 2^14);
+//    modulo m_l, Re C' = P_rr - P_ii and Im C' = P_ri + P_ir.  (The
+//    Karatsuba form would save one product in four, but its third operand
+//    plane costs more in residue traffic than the int8 matrix cores save.)
+// 4. Chinese remainder reconstruction (oz2_crt_kernel).  With
 //    y_l = (M / m_l)^(-1) mod m_l, the exact integer product satisfies
 //    C' / M = sum_l v_l / m_l (mod 1), v_l = (x_l y_l) mod m_l, x_l the
 //    residue of C' from step 3.  The sum is formed in FP64 (each term in
@@ -71,28 +72,26 @@ __device__ __forceinline__ int sym_mod(double x, double m, double inv_m, int lo)
   return r;
 }
 
-__device__ __forceinline__ int sym_fix(int r, int mi, int lo) {
-  r = r > lo + mi - 1 ? r - mi : r;
-  r = r < lo ? r + mi : r;
-  return r;
-}
-
-// One workgroup per 64 lines of one matrix.  Thread t owns one line and 16
-// consecutive values of k per 64-wide chunk of k: for line-contiguous storage
-// (element (line, k) at X[line * ld + k]) line t / 4, k offset 16 (t % 4); for
-// k-contiguous storage (element at X[k * ld + line]) line t % 64, k offset
-// 16 (t / 64), so consecutive threads read consecutive lines.
+// One workgroup per 8 lines of one matrix, covering the whole of k in
+// chunks of 256.  Thread t owns one line and 8 consecutive values of k per
+// chunk: for k-contiguous storage of the lines (element (line, k) at
+// X[line * ld + k]) line t / 32 and k offset 8 (t % 32), so the 32 threads of
+// a line read and write 256 contiguous values; for line-contiguous storage
+// (element at X[k * ld + line]) line t % 8 and k offset 8 (t / 8), so 8
+// consecutive threads read 8 consecutive lines.  The first pass finds each
+// line's largest magnitude, the second writes the residues.
 __global__ __launch_bounds__(256) void oz2_split_kernel(const z2* X, int64_t sX, int ld, int trans,
                                                         int conj, int lines, int k, int k_pad,
                                                         OzConsts c, int8_t* res, int64_t s_res,
                                                         int* expo, int64_t s_expo) {
+  constexpr int LPW = 8, EPT = 8;
   __shared__ double red[256];
-  const int groups = (lines + 63) / 64;
+  const int groups = (lines + LPW - 1) / LPW;
   const int64_t e = blockIdx.x / groups;
-  const int line0 = (int)(blockIdx.x - e * groups) * 64;
+  const int line0 = (int)(blockIdx.x - e * groups) * LPW;
   const int t = threadIdx.x;
-  const int ll = trans ? (t & 63) : (t >> 2);
-  const int kq = trans ? 16 * (t >> 6) : 16 * (t & 3);
+  const int ll = trans ? (t % LPW) : (t / (256 / LPW));
+  const int kq = EPT * (trans ? (t / LPW) : (t % (256 / LPW)));
   const int line = line0 + ll;
   const bool valid = line < lines;
   const z2* Xe = X + e * sX;
@@ -101,19 +100,21 @@ __global__ __launch_bounds__(256) void oz2_split_kernel(const z2* X, int64_t sX,
     return trans ? Xe[(int64_t)kk * ld + line] : Xe[(int64_t)line * ld + kk];
   };
   double mx = 0.0;
-  for (int kc = 0; kc < k_pad; kc += 64)
-#pragma unroll 4
-    for (int j = 0; j < 16; ++j) {
+  for (int kc = 0; kc < k_pad; kc += 256) {
+    if (kc + kq >= k_pad) continue;
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
       const z2 v = load(kc + kq + j);
       mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
     }
+  }
   red[t] = mx;
   __syncthreads();
-  // the four threads of this line
-  if (trans)
-    mx = fmax(fmax(red[ll], red[ll + 64]), fmax(red[ll + 128], red[ll + 192]));
-  else
-    mx = fmax(fmax(red[4 * ll], red[4 * ll + 1]), fmax(red[4 * ll + 2], red[4 * ll + 3]));
+  // the 32 threads of this line
+  mx = 0.0;
+#pragma unroll
+  for (int q = 0; q < 256 / LPW; ++q)
+    mx = fmax(mx, red[trans ? (ll + LPW * q) : ((256 / LPW) * ll + q)]);
   // smallest ex with mx < 2^ex (0 for an all-zero line)
   const int ex = mx > 0.0 ? ilogb(mx) + 1 : 0;
   if (valid && kq == 0) expo[e * s_expo + line] = ex;
@@ -122,38 +123,34 @@ __global__ __launch_bounds__(256) void oz2_split_kernel(const z2* X, int64_t sX,
   const double sgn = conj ? -1.0 : 1.0;
   const int64_t plane = (int64_t)lines * k_pad;
   int8_t* R = res + e * s_res + (int64_t)line * k_pad;
-  for (int kc = 0; kc < k_pad; kc += 64) {
-    double ar[16], ai[16];
+  typedef int v2i __attribute__((ext_vector_type(2)));
+  for (int kc = 0; kc < k_pad; kc += 256) {
+    if (kc + kq >= k_pad) continue;
+    double ar[EPT], ai[EPT];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < EPT; ++j) {
       const z2 v = load(kc + kq + j);
       ar[j] = rint(ldexp(v.x, sh));
       ai[j] = rint(ldexp(sgn * v.y, sh));
     }
     for (int l = 0; l < c.nmod; ++l) {
       const double m = c.m[l], im = c.inv_m[l];
-      const int lo = c.lo[l], mi = (int)m;
-      v4i pr, pi, ps;
+      const int lo = c.lo[l];
+      v2i pr, pi;
 #pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        unsigned br = 0, bi = 0, bs = 0;
+      for (int w = 0; w < EPT / 4; ++w) {
+        unsigned br = 0, bi = 0;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-          const int rr = sym_mod(ar[4 * w + b], m, im, lo);
-          const int ri = sym_mod(ai[4 * w + b], m, im, lo);
-          const int rs = sym_fix(rr + ri, mi, lo);
-          br |= (unsigned)(rr & 0xFF) << (8 * b);
-          bi |= (unsigned)(ri & 0xFF) << (8 * b);
-          bs |= (unsigned)(rs & 0xFF) << (8 * b);
+          br |= (unsigned)(sym_mod(ar[4 * w + b], m, im, lo) & 0xFF) << (8 * b);
+          bi |= (unsigned)(sym_mod(ai[4 * w + b], m, im, lo) & 0xFF) << (8 * b);
         }
         pr[w] = (int)br;
         pi[w] = (int)bi;
-        ps[w] = (int)bs;
       }
-      const int64_t o = (int64_t)(3 * l) * plane + kc + kq;
-      *reinterpret_cast<v4i*>(R + o) = pr;
-      *reinterpret_cast<v4i*>(R + o + plane) = pi;
-      *reinterpret_cast<v4i*>(R + o + 2 * plane) = ps;
+      const int64_t o = (int64_t)(2 * l) * plane + kc + kq;
+      *reinterpret_cast<v2i*>(R + o) = pr;
+      *reinterpret_cast<v2i*>(R + o + plane) = pi;
     }
   }
 }
@@ -161,9 +158,9 @@ __global__ __launch_bounds__(256) void oz2_split_kernel(const z2* X, int64_t sX,
 struct Oz2Args {
   int m, n, k_pad;
   double ar, ai, br, bi;
-  const int8_t* Ra;  // [batch][l][3][m][k_pad]
+  const int8_t* Ra;  // [batch][l][2][m][k_pad]
   int64_t sRa;
-  const int8_t* Rb;  // [batch][l][3][n][k_pad]
+  const int8_t* Rb;  // [batch][l][2][n][k_pad]
   int64_t sRb;
   const int* ea;  // [batch][m]
   const int* eb;  // [batch][n]
@@ -181,6 +178,7 @@ struct Oz2Args {
   int ldct;
   int tilesM, tilesN;
   int beta_zero;
+  int8_t* V;  // [matrix][tile][modulus][re, im][4096] residues of C'
   OzConsts c;
 };
 
@@ -191,20 +189,28 @@ __device__ inline int oz2_xcd_remap(int bid, int nblocks) {
   return x * per + min(x, rem) + j;
 }
 
-// 64 x 64 output tile per workgroup of 4 waves (2 x 2, 32 x 32 each: 2 x 2
-// MFMA tiles).  Operand fragments are read straight from global memory (L2).
-__global__ __launch_bounds__(256) void oz2_gemm_kernel(Oz2Args g) {
-  const int id = oz2_xcd_remap(blockIdx.x, gridDim.x);
+// Residues of the product modulo one modulus per workgroup: a 64 x 64
+// output tile of one matrix, 4 waves (2 x 2, 32 x 32 each: 2 x 2 MFMA tiles),
+// grid (matrix, tile, modulus).  Operand fragments are read straight from
+// global memory (L2); four waves per SIMD and the many short workgroups of
+// the grid hide the load latency.  The
+// residues v_l = (x_l y_l) mod m_l of Re C' and Im C' leave as int8 in the
+// MFMA result order: per (matrix, tile, modulus, re / im) 256 threads x 16
+// values, one 16-byte store per thread.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void oz2_mod_kernel(Oz2Args g) {
   const int tiles = g.tilesM * g.tilesN;
-  const int64_t e = id / tiles;
-  const int t = id - (int)(e * tiles);
+  const int64_t id = oz2_xcd_remap(blockIdx.x, gridDim.x);
+  const int l = (int)(id % g.c.nmod);
+  const int64_t et = id / g.c.nmod;  // matrix * tiles + tile
+  const int64_t e = et / tiles;
+  const int t = (int)(et - e * tiles);
   const int m0 = (t / g.tilesN) * 64, n0 = (t % g.tilesN) * 64;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int kg = 16 * (lane >> 4);
   const int64_t planeA = (int64_t)g.m * g.k_pad, planeB = (int64_t)g.n * g.k_pad;
-  const int8_t* Ra = g.Ra + e * g.sRa;
-  const int8_t* Rb = g.Rb + e * g.sRb;
+  const int8_t* A0 = g.Ra + e * g.sRa + (int64_t)(2 * l) * planeA;
+  const int8_t* B0 = g.Rb + e * g.sRb + (int64_t)(2 * l) * planeB;
   // fragment rows / columns, clamped into the matrix (ragged edges)
   int64_t offA[2], offB[2];
 #pragma unroll
@@ -214,69 +220,97 @@ __global__ __launch_bounds__(256) void oz2_gemm_kernel(Oz2Args g) {
     offA[i] = (int64_t)r * g.k_pad + kg;
     offB[i] = (int64_t)cc * g.k_pad + kg;
   }
-  double sr[2][2][4], si[2][2][4];
+  auto fetch = [&](int kk, v4i (&a)[2][2], v4i (&b)[2][2]) {
+#pragma unroll
+    for (int cpt = 0; cpt < 2; ++cpt)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a[cpt][i] = *reinterpret_cast<const v4i*>(A0 + cpt * planeA + offA[i] + 64 * kk);
+        b[cpt][i] = *reinterpret_cast<const v4i*>(B0 + cpt * planeB + offB[i] + 64 * kk);
+      }
+  };
+  v4i prr[2][2], pii[2][2], pri[2][2], pir[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 2; ++j) prr[i][j] = pii[i][j] = pri[i][j] = pir[i][j] = v4i{0, 0, 0, 0};
+  auto step = [&](const v4i (&a)[2][2], const v4i (&b)[2][2]) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) sr[i][j][q] = si[i][j][q] = 0.0;
-
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        prr[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[0][i], b[0][j], prr[i][j], 0, 0, 0);
+        pii[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[1][i], b[1][j], pii[i][j], 0, 0, 0);
+        pri[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[0][i], b[1][j], pri[i][j], 0, 0, 0);
+        pir[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[1][i], b[0][j], pir[i][j], 0, 0, 0);
+      }
+  };
+  // one fragment buffer: four waves per SIMD (128 registers) hide the load
+  // latency of one another
   const int nk = g.k_pad / 64;
-  for (int l = 0; l < g.c.nmod; ++l) {
-    v4i p1[2][2], p2[2][2], p3[2][2];
+  for (int kk = 0; kk < nk; ++kk) {
+    v4i fa[2][2], fb[2][2];
+    fetch(kk, fa, fb);
+    step(fa, fb);
+  }
+  const double m = g.c.m[l], im = g.c.inv_m[l], y = g.c.y[l];
+  v4i vr, vi;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) p1[i][j] = p2[i][j] = p3[i][j] = v4i{0, 0, 0, 0};
-    const int8_t* A0 = Ra + (int64_t)(3 * l) * planeA;
-    const int8_t* B0 = Rb + (int64_t)(3 * l) * planeB;
-    for (int kk = 0; kk < nk; ++kk) {
-      v4i a[3][2], b[3][2];
+    for (int j = 0; j < 2; ++j) {
+      unsigned br = 0, bi = 0;
 #pragma unroll
-      for (int cpt = 0; cpt < 3; ++cpt)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          a[cpt][i] = *reinterpret_cast<const v4i*>(A0 + cpt * planeA + offA[i] + 64 * kk);
-          b[cpt][i] = *reinterpret_cast<const v4i*>(B0 + cpt * planeB + offB[i] + 64 * kk);
-        }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          p1[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[0][i], b[0][j], p1[i][j], 0, 0, 0);
-          p2[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[1][i], b[1][j], p2[i][j], 0, 0, 0);
-          p3[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[2][i], b[2][j], p3[i][j], 0, 0, 0);
-        }
+      for (int q = 0; q < 4; ++q) {
+        const double tr = (double)(prr[i][j][q] - pii[i][j][q]) * y;
+        const double ti = (double)(pri[i][j][q] + pir[i][j][q]) * y;
+        // symmetric residues, |v| <= m / 2 <= 128 (the 128 of m = 256 wraps
+        // to -128, the same residue)
+        const int ir = (int)fma(-rint(tr * im), m, tr), ii = (int)fma(-rint(ti * im), m, ti);
+        br |= (unsigned)(ir & 0xFF) << (8 * q);
+        bi |= (unsigned)(ii & 0xFF) << (8 * q);
+      }
+      vr[2 * i + j] = (int)br;
+      vi[2 * i + j] = (int)bi;
     }
-    // residues of Re C', Im C' modulo m_l, weighted by y_l / m_l
-    const double m = g.c.m[l], im = g.c.inv_m[l], y = g.c.y[l];
+  int8_t* V = g.V + ((et * g.c.nmod + l) * 2) * 4096;
+  reinterpret_cast<v4i*>(V)[threadIdx.x] = vr;
+  reinterpret_cast<v4i*>(V + 4096)[threadIdx.x] = vi;
+}
+
+// Chinese remainder reconstruction and epilogue: one workgroup per 64 x 64
+// output tile, each thread owning the 16 elements it held in oz2_mod_kernel.
+__global__ __launch_bounds__(256) void oz2_crt_kernel(Oz2Args g) {
+  const int tiles = g.tilesM * g.tilesN;
+  const int64_t et = oz2_xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t e = et / tiles;
+  const int t = (int)(et - e * tiles);
+  const int m0 = (t / g.tilesN) * 64, n0 = (t % g.tilesN) * 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  double sr[16], si[16];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+  for (int q = 0; q < 16; ++q) sr[q] = si[q] = 0.0;
+  const int8_t* V = g.V + et * g.c.nmod * 2 * 4096;
+  for (int l = 0; l < g.c.nmod; ++l) {
+    const v4i vr = reinterpret_cast<const v4i*>(V + (2 * l) * 4096)[threadIdx.x];
+    const v4i vi = reinterpret_cast<const v4i*>(V + (2 * l + 1) * 4096)[threadIdx.x];
+    const double im = g.c.inv_m[l];
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int xr = p1[i][j][q] - p2[i][j][q];
-          const int xi = p3[i][j][q] - p1[i][j][q] - p2[i][j][q];
-          const double tr = (double)xr * y, ti = (double)xi * y;
-          const double vr = fma(-rint(tr * im), m, tr), vi = fma(-rint(ti * im), m, ti);
-          sr[i][j][q] = fma(vr, im, sr[i][j][q]);
-          si[i][j][q] = fma(vi, im, si[i][j][q]);
-        }
+    for (int q = 0; q < 16; ++q) {
+      const int br = (int)(signed char)((unsigned)vr[q >> 2] >> (8 * (q & 3)));
+      const int bi = (int)(signed char)((unsigned)vi[q >> 2] >> (8 * (q & 3)));
+      sr[q] = fma((double)br, im, sr[q]);
+      si[q] = fma((double)bi, im, si[q]);
+    }
     if ((l & 3) == 3) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            sr[i][j][q] -= rint(sr[i][j][q]);
-            si[i][j][q] -= rint(si[i][j][q]);
-          }
+      for (int q = 0; q < 16; ++q) {
+        sr[q] -= rint(sr[q]);
+        si[q] -= rint(si[q]);
+      }
     }
   }
-
   const int* ea = g.ea + e * g.m;
   const int* eb = g.eb + e * g.n;
   z2* C = g.C + e * g.sC;
@@ -291,9 +325,10 @@ __global__ __launch_bounds__(256) void oz2_gemm_kernel(Oz2Args g) {
       for (int q = 0; q < 4; ++q) {
         const int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + q;
         const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+        const int x = 4 * (2 * i + j) + q;
         if (row < g.m && col < g.n) {
-          const double fr = sr[i][j][q] - rint(sr[i][j][q]);
-          const double fi = si[i][j][q] - rint(si[i][j][q]);
+          const double fr = sr[x] - rint(sr[x]);
+          const double fi = si[x] - rint(si[x]);
           const int ex = ea[row] + eb[col] - 2 * g.c.T;
           const double vr = ldexp(fr * g.c.M, ex), vi = ldexp(fi * g.c.M, ex);
           double outr = fma(g.ar, vr, -g.ai * vi), outi = fma(g.ar, vi, g.ai * vr);
@@ -351,8 +386,10 @@ OzConsts make_consts(int nmod, int k) {
 
 size_t oz2_workspace_bytes(int64_t batch, int m, int n, int k, int nmod) {
   const int64_t kp = pad64(k);
-  return (size_t)(batch * 3 * nmod * (int64_t)(m + n) * kp) +
-         (size_t)(batch * (m + n)) * sizeof(int) + 512;
+  const int64_t tiles = (int64_t)((m + 63) / 64) * ((n + 63) / 64);
+  return (size_t)(batch * 2 * nmod * (int64_t)(m + n) * kp) +
+         (size_t)(batch * (m + n)) * sizeof(int) + 256 +
+         (size_t)(batch * tiles * nmod * 2 * 4096) + 256;
 }
 
 void zgemm_oz2(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int opB, Cplx alpha,
@@ -382,16 +419,19 @@ void zgemm_oz2(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int o
   const OzConsts c = make_consts(nmod, k);
   const int k_pad = (int)pad64(k);
   int8_t* Ra = static_cast<int8_t*>(ws);
-  const int64_t sRa = (int64_t)3 * nmod * m * k_pad, sRb = (int64_t)3 * nmod * n * k_pad;
+  const int64_t sRa = (int64_t)2 * nmod * m * k_pad, sRb = (int64_t)2 * nmod * n * k_pad;
   int8_t* Rb = Ra + batch * sRa;
   int* ea = reinterpret_cast<int*>(Rb + batch * sRb);
   int* eb = ea + batch * m;
+  // residues of C', 256-byte aligned after the exponents
+  const uintptr_t vbase = (reinterpret_cast<uintptr_t>(eb + batch * n) + 255) & ~uintptr_t(255);
+  int8_t* V = reinterpret_cast<int8_t*>(vbase);
   KernelTimer* timer = active_timer;
   hipEvent_t ev0 = nullptr;
   if (timer) timer->begin(s, &ev0);
   // line i of op(A): row i of A, or conj of column i of A for A^H;
   // line j of op(B): column j of B, or conj of row j of B for B^H
-  const int64_t ga = batch * ((m + 63) / 64), gb = batch * ((n + 63) / 64);
+  const int64_t ga = batch * ((m + 7) / 8), gb = batch * ((n + 7) / 8);
   QX_REQUIRE(ga < (int64_t(1) << 31) && gb < (int64_t(1) << 31), "zgemm_oz2: grid too large");
   hipLaunchKernelGGL(oz2_split_kernel, dim3((unsigned)ga), dim3(256), 0, s, x.A.p, x.A.stride,
                      x.A.ld, opA == QX_OP_C ? 1 : 0, opA == QX_OP_C ? 1 : 0, m, k, k_pad, c, Ra,
@@ -429,10 +469,12 @@ void zgemm_oz2(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int o
   g.tilesM = (m + 63) / 64;
   g.tilesN = (n + 63) / 64;
   g.beta_zero = beta_zero ? 1 : 0;
+  g.V = V;
   g.c = c;
   const int64_t blocks = batch * g.tilesM * g.tilesN;
-  QX_REQUIRE(blocks < (int64_t(1) << 31), "zgemm_oz2: grid too large");
-  hipLaunchKernelGGL(oz2_gemm_kernel, dim3((unsigned)blocks), dim3(256), 0, s, g);
+  QX_REQUIRE(blocks * nmod < (int64_t(1) << 31), "zgemm_oz2: grid too large");
+  hipLaunchKernelGGL(oz2_mod_kernel, dim3((unsigned)(blocks * nmod)), dim3(256), 0, s, g);
+  hipLaunchKernelGGL(oz2_crt_kernel, dim3((unsigned)blocks), dim3(256), 0, s, g);
   QX_HIP_CHECK(hipGetLastError());
   if (timer) timer->end(s, ev0, KernelTimer::GEMM, 8.0 * m * n * (double)k * batch);
 }
