@@ -192,12 +192,14 @@ __device__ inline int oz2_xcd_remap(int bid, int nblocks) {
 // Residues of the product modulo one modulus per workgroup: a 64 x 64
 // output tile of one matrix, 4 waves (2 x 2, 32 x 32 each: 2 x 2 MFMA tiles),
 // grid (matrix, tile, modulus).  Operand fragments are read straight from
-// global memory (L2); four waves per SIMD and the many short workgroups of
-// the grid hide the load latency.  The
+// global memory (L2), up to four 64-wide k-steps ahead of the products that
+// use them; the many short workgroups of the grid hide the rest of the
+// latency.  NK = k_pad / 64 is a template parameter (1..8).  The
 // residues v_l = (x_l y_l) mod m_l of Re C' and Im C' leave as int8 in the
 // MFMA result order: per (matrix, tile, modulus, re / im) 256 threads x 16
 // values, one 16-byte store per thread.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void oz2_mod_kernel(Oz2Args g) {
+template <int NK>
+__global__ __launch_bounds__(256) void oz2_mod_kernel(Oz2Args g) {
   const int tiles = g.tilesM * g.tilesN;
   const int64_t id = oz2_xcd_remap(blockIdx.x, gridDim.x);
   const int l = (int)(id % g.c.nmod);
@@ -209,25 +211,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void o
   const int wm = w >> 1, wn = w & 1;
   const int kg = 16 * (lane >> 4);
   const int64_t planeA = (int64_t)g.m * g.k_pad, planeB = (int64_t)g.n * g.k_pad;
-  const int8_t* A0 = g.Ra + e * g.sRa + (int64_t)(2 * l) * planeA;
-  const int8_t* B0 = g.Rb + e * g.sRb + (int64_t)(2 * l) * planeB;
+  // wave-uniform plane bases and 32-bit per-lane offsets (one address
+  // register per fragment); planes hold < 2^31 bytes (k_pad <= 512)
+  const int8_t* Ar = g.Ra + e * g.sRa + (int64_t)(2 * l) * planeA;
+  const int8_t* Ai = Ar + planeA;
+  const int8_t* Br = g.Rb + e * g.sRb + (int64_t)(2 * l) * planeB;
+  const int8_t* Bi = Br + planeB;
   // fragment rows / columns, clamped into the matrix (ragged edges)
-  int64_t offA[2], offB[2];
+  int offA[2], offB[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int r = min(m0 + wm * 32 + i * 16 + (lane & 15), g.m - 1);
     const int cc = min(n0 + wn * 32 + i * 16 + (lane & 15), g.n - 1);
-    offA[i] = (int64_t)r * g.k_pad + kg;
-    offB[i] = (int64_t)cc * g.k_pad + kg;
+    offA[i] = r * g.k_pad + kg;
+    offB[i] = cc * g.k_pad + kg;
   }
   auto fetch = [&](int kk, v4i (&a)[2][2], v4i (&b)[2][2]) {
 #pragma unroll
-    for (int cpt = 0; cpt < 2; ++cpt)
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        a[cpt][i] = *reinterpret_cast<const v4i*>(A0 + cpt * planeA + offA[i] + 64 * kk);
-        b[cpt][i] = *reinterpret_cast<const v4i*>(B0 + cpt * planeB + offB[i] + 64 * kk);
-      }
+    for (int i = 0; i < 2; ++i) {
+      a[0][i] = *reinterpret_cast<const v4i*>(Ar + offA[i] + 64 * kk);
+      a[1][i] = *reinterpret_cast<const v4i*>(Ai + offA[i] + 64 * kk);
+      b[0][i] = *reinterpret_cast<const v4i*>(Br + offB[i] + 64 * kk);
+      b[1][i] = *reinterpret_cast<const v4i*>(Bi + offB[i] + 64 * kk);
+    }
   };
   v4i prr[2][2], pii[2][2], pri[2][2], pir[2][2];
 #pragma unroll
@@ -245,13 +251,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void o
         pir[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[1][i], b[0][j], pir[i][j], 0, 0, 0);
       }
   };
-  // one fragment buffer: four waves per SIMD (128 registers) hide the load
-  // latency of one another
-  const int nk = g.k_pad / 64;
-  for (int kk = 0; kk < nk; ++kk) {
-    v4i fa[2][2], fb[2][2];
-    fetch(kk, fa, fb);
-    step(fa, fb);
+  // NK = k_pad / 64 k-steps, in groups of up to four whose loads are all
+  // issued before the first of their products (straight-line code, so each
+  // product waits for exactly its own fragments)
+#pragma unroll
+  for (int k0 = 0; k0 < NK; k0 += 4) {
+    constexpr int G = NK < 4 ? NK : 4;
+    v4i fa[G][2][2], fb[G][2][2];
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+      if (k0 + q < NK) fetch(k0 + q, fa[q], fb[q]);
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+      if (k0 + q < NK) step(fa[q], fb[q]);
   }
   const double m = g.c.m[l], im = g.c.inv_m[l], y = g.c.y[l];
   v4i vr, vi;
@@ -413,7 +425,7 @@ void zgemm_oz2(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int o
     if (!x.Ct.zero()) ew_neg_h(s, batch, m, n, x.C, x.Ct);
     return;
   }
-  QX_REQUIRE(k <= 4096, "zgemm_oz2: k must be <= 4096");
+  QX_REQUIRE(k <= 512, "zgemm_oz2: k must be <= 512");
   QX_REQUIRE(ws_bytes >= oz2_workspace_bytes(batch, m, n, k, nmod),
              "zgemm_oz2: workspace too small");
   const OzConsts c = make_consts(nmod, k);
@@ -473,7 +485,24 @@ void zgemm_oz2(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int o
   g.c = c;
   const int64_t blocks = batch * g.tilesM * g.tilesN;
   QX_REQUIRE(blocks * nmod < (int64_t(1) << 31), "zgemm_oz2: grid too large");
-  hipLaunchKernelGGL(oz2_mod_kernel, dim3((unsigned)(blocks * nmod)), dim3(256), 0, s, g);
+  const dim3 mgrid((unsigned)(blocks * nmod));
+  switch (k_pad / 64) {
+#define QX_OZ_MOD(NK)                                                   \
+  case NK:                                                              \
+    hipLaunchKernelGGL(oz2_mod_kernel<NK>, mgrid, dim3(256), 0, s, g);  \
+    break;
+    QX_OZ_MOD(1)
+    QX_OZ_MOD(2)
+    QX_OZ_MOD(3)
+    QX_OZ_MOD(4)
+    QX_OZ_MOD(5)
+    QX_OZ_MOD(6)
+    QX_OZ_MOD(7)
+    QX_OZ_MOD(8)
+#undef QX_OZ_MOD
+    default:
+      throw Error{QX_EINVAL, "zgemm_oz2: k must be <= 512"};
+  }
   hipLaunchKernelGGL(oz2_crt_kernel, dim3((unsigned)blocks), dim3(256), 0, s, g);
   QX_HIP_CHECK(hipGetLastError());
   if (timer) timer->end(s, ev0, KernelTimer::GEMM, 8.0 * m * n * (double)k * batch);
