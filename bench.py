@@ -347,9 +347,10 @@ def obc_workload(args):
     Lead principal layer of b x b (H00 Hermitian, H01 ~ CN / sqrt(b)),
     M = (E + i eta) I - H over ``--energies`` energies.  Times the surface
     Green's function by Sancho-Rubio (FP64, tolerance 1e-12) and by the
-    Spectral OBC on Beyn's contour method (annulus r_i = 1 / r_o, 64 nodes,
-    32 probe vectors), compares the two, and checks Sancho-Rubio against
-    the NumPy oracle on a few energies.
+    Spectral OBC on Beyn's contour method (annuli r_i = 1 / r_o for each
+    radius of ``--beyn-radius``, 64 nodes, m_0 probe vectors from the largest
+    in-annulus eigenvalue count), compares the two, and checks Sancho-Rubio
+    against the NumPy oracle on a few energies.
     """
     import numpy as np
     import torch
@@ -381,7 +382,6 @@ def obc_workload(args):
         x_sr, t_sr = timed(lambda: sr(a_ii, a_ij, a_ji))
     g = timer.stats["gemm"]
     log(f"sancho-rubio: {t_sr * 1e3:.1f} ms per batch, {sr.iterations} iterations")
-    r_o = args.beyn_radius
     from quatrex_amd.ops import inv, mm
 
     res_sr = torch.linalg.norm(x_sr - inv(a_ii - mm(a_ji, x_sr, a_ij)), dim=(-2, -1))
@@ -399,16 +399,28 @@ def obc_workload(args):
                                              "max": float(f"{diff.max().item():.3e}")}}
 
     # Beyn's two-moment projection resolves at most b eigenvalues inside the
-    # annulus; this random dense lead has ~90 % of its 2b eigenvalues there,
-    # so the Spectral OBC on Beyn cannot converge for it (its throughput is
-    # still measured).  The Spectral OBC on the full companion eigenproblem
-    # shows the mode machinery agrees with Sancho-Rubio at this size.
-    legs = [spectral_leg(Beyn(r_o=r_o, r_i=1.0 / r_o, m_0=32, num_quad_points=64), ne,
-                         f"beyn(r_o={r_o}, m_0=32, 64 nodes)"),
-            spectral_leg(Full(), min(ne, 8), "full")]
-    w_full, _ = Full()((a_ji[:1], a_ii[:1], a_ij[:1]))
+    # annulus r_i < |w| < r_o.  For each outer radius the probe count m_0 is
+    # the largest in-annulus count over a sample of energies (companion
+    # eigenvalues, Full) plus a margin, capped at b; the Spectral OBC on Beyn
+    # is then compared with Sancho-Rubio on a subset of energies (its small
+    # SVD / eig run on the host per energy).  The Spectral OBC on the full
+    # companion eigenproblem shows the mode machinery at this size.
+    n_cnt = min(ne, 8)
+    w_full, _ = Full()((a_ji[:n_cnt], a_ii[:n_cnt], a_ij[:n_cnt]))
     mag = w_full.abs()
-    inside = int(((mag > 1.0 / r_o) & (mag < r_o)).sum().item())
+    radii = [float(r) for r in str(args.beyn_radius).split(",")]
+    legs, annulus = [], []
+    n_beyn = min(ne, args.beyn_energies)
+    for r_o in radii:
+        inside = ((mag > 1.0 / r_o) & (mag < r_o)).sum(-1)
+        cnt = int(inside.max().item())
+        m0 = min(cnt + 8, b)
+        annulus.append({"r_o": r_o, "r_i": round(1.0 / r_o, 6), "in_annulus_max": cnt,
+                        "in_annulus_min": int(inside.min().item()), "of": 2 * b, "m_0": m0,
+                        "resolvable": cnt <= b})
+        legs.append(spectral_leg(Beyn(r_o=r_o, r_i=1.0 / r_o, m_0=m0, num_quad_points=64),
+                                 n_beyn, f"beyn(r_o={r_o}, m_0={m0}, 64 nodes)"))
+    legs.append(spectral_leg(Full(), min(ne, 8), "full"))
     oracle = None
     if args.cpu_budget > 0:
         import oracle.obc as oobc
@@ -432,14 +444,14 @@ def obc_workload(args):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"lead surface Green's function, {ne} energies, layer {b}",
                    "energies": ne, "block_size": b, "sancho_rubio_tol": 1e-12,
-                   "beyn": {"r_o": r_o, "r_i": 1.0 / r_o, "quad_points": 64, "m_0": 32}},
+                   "beyn": {"radii": radii, "quad_points": 64,
+                            "m_0": "in-annulus count + 8, at most the layer size"}},
         "sancho_rubio": {"energies_per_s": round(ne / t_sr, 3), "iterations": sr.iterations,
                          "max_fixed_point_residual": float(f"{res_sr:.3e}"),
                          "algorithmic_tflops": round(flops_sr / t_sr / 1e12, 3),
                          "gemm_tflops": round(g["flops"] / max(g["busy_ms"], 1e-9) / 1e9, 3)},
         "spectral": legs,
-        "lead_eigenvalues_in_annulus": {"count": inside, "of": 2 * b,
-                                        "beyn_resolvable_at_most": b},
+        "lead_eigenvalues_in_annulus": annulus,
         "oracle": oracle,
     }
     print(json.dumps(result), flush=True)
@@ -667,8 +679,10 @@ def main():
     ap.add_argument("--rates", default="1,2,4", help="serve: arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=16, help="serve: requests per rate")
     ap.add_argument("--hbm-budget", type=int, default=200, help="serve: HBM budget in GiB")
-    ap.add_argument("--beyn-radius", type=float, default=2.0,
-                    help="outer contour radius of the Beyn leg (inner = 1 / radius)")
+    ap.add_argument("--beyn-radius", default="1.5,2,3",
+                    help="outer contour radii of the Beyn legs (inner = 1 / radius)")
+    ap.add_argument("--beyn-energies", type=int, default=32,
+                    help="obc: energies of each Spectral-on-Beyn leg")
     args = ap.parse_args()
 
     if args.gpus != 1:
