@@ -13,9 +13,10 @@ the inputs.
 The primary arithmetic (``--precision``, default complex128 with the 4M
 GEMM split) gives ``value``; every mode of ``--compare`` (default:
 c128-3m = complex128 with the 3M split, mixed = complex64 products and
-inverses + one FP64 Newton-Schulz step per block inverse, and oz2-12 =
+inverses + one FP64 Newton-Schulz step per block inverse, oz2-12 =
 block products on int8 matrix cores by modular arithmetic with 12 moduli
-and FP64 inverses) is timed the
+and FP64 inverses, bf16x7 = block products on bf16 matrix cores from 7
+digit slices and FP64 inverses) is timed the
 same way and reported beside it with its speed-up and its error against the
 complex128 GPU result (per block, electron density diag G^<, transmission
 T(E) = Tr[Gamma_L G^R_{0,N-1} Gamma_R G^A_{N-1,0}]) on the first
@@ -47,6 +48,7 @@ PEAK_TFLOPS = {"c128": 78.6, "c128-3m": 78.6, "c64": 157.3, "mixed": 157.3}
 # modular-arithmetic products on int8 cores: rated against the FP64 matrix
 # peak of the arithmetic they replace (complex flops 8 m n k per product)
 PEAK_TFLOPS.update({f"oz2-{n}": 78.6 for n in range(8, 21)})
+PEAK_TFLOPS.update({f"bf16x{s}": 78.6 for s in range(2, 8)})
 HBM_PEAK_GBS = 8000.0
 
 
@@ -350,7 +352,9 @@ def obc_workload(args):
     Spectral OBC on Beyn's contour method (annuli r_i = 1 / r_o for each
     radius of ``--beyn-radius``, 64 nodes, m_0 probe vectors from the largest
     in-annulus eigenvalue count), compares the two, and checks Sancho-Rubio
-    against the NumPy oracle on a few energies.
+    against the NumPy oracle on a few energies.  The Beyn legs are repeated
+    on a second lead whose onsite spread is ``--evanescent-scale`` times
+    wider (fewer modes near the unit circle).
     """
     import numpy as np
     import torch
@@ -381,17 +385,19 @@ def obc_workload(args):
     with timer:
         x_sr, t_sr = timed(lambda: sr(a_ii, a_ij, a_ji))
     g = timer.stats["gemm"]
-    log(f"sancho-rubio: {t_sr * 1e3:.1f} ms per batch, {sr.iterations} iterations")
+    sr_iterations = sr.iterations
+    log(f"sancho-rubio: {t_sr * 1e3:.1f} ms per batch, {sr_iterations} iterations")
     from quatrex_amd.ops import inv, mm
 
     res_sr = torch.linalg.norm(x_sr - inv(a_ii - mm(a_ji, x_sr, a_ij)), dim=(-2, -1))
     res_sr = (res_sr / torch.linalg.norm(x_sr, dim=(-2, -1))).max().item()
 
-    def spectral_leg(nevp, n, label):
+    def spectral_leg(nevp, n, label, lead, x_ref):
         spec = Spectral(nevp)
-        x_sp, t_sp = timed(lambda: spec(a_ii[:n], a_ij[:n], a_ji[:n]))
-        diff = torch.linalg.norm(x_sp - x_sr[:n], dim=(-2, -1)) / torch.linalg.norm(
-            x_sr[:n], dim=(-2, -1))
+        l_ii, l_ij, l_ji = lead
+        x_sp, t_sp = timed(lambda: spec(l_ii[:n], l_ij[:n], l_ji[:n]))
+        diff = torch.linalg.norm(x_sp - x_ref[:n], dim=(-2, -1)) / torch.linalg.norm(
+            x_ref[:n], dim=(-2, -1))
         log(f"spectral/{label} on {n} energies: {t_sp * 1e3:.1f} ms per batch, median "
             f"diff vs SR {diff.median().item():.2e}")
         return {"nevp": label, "energies": n, "energies_per_s": round(n / t_sp, 3),
@@ -402,25 +408,46 @@ def obc_workload(args):
     # annulus r_i < |w| < r_o.  For each outer radius the probe count m_0 is
     # the largest in-annulus count over a sample of energies (companion
     # eigenvalues, Full) plus a margin, capped at b; the Spectral OBC on Beyn
-    # is then compared with Sancho-Rubio on a subset of energies (its small
-    # SVD / eig run on the host per energy).  The Spectral OBC on the full
-    # companion eigenproblem shows the mode machinery at this size.
-    n_cnt = min(ne, 8)
-    w_full, _ = Full()((a_ji[:n_cnt], a_ii[:n_cnt], a_ij[:n_cnt]))
-    mag = w_full.abs()
+    # is then compared with Sancho-Rubio on a subset of energies.
     radii = [float(r) for r in str(args.beyn_radius).split(",")]
-    legs, annulus = [], []
     n_beyn = min(ne, args.beyn_energies)
-    for r_o in radii:
-        inside = ((mag > 1.0 / r_o) & (mag < r_o)).sum(-1)
-        cnt = int(inside.max().item())
-        m0 = min(cnt + 8, b)
-        annulus.append({"r_o": r_o, "r_i": round(1.0 / r_o, 6), "in_annulus_max": cnt,
-                        "in_annulus_min": int(inside.min().item()), "of": 2 * b, "m_0": m0,
-                        "resolvable": cnt <= b})
-        legs.append(spectral_leg(Beyn(r_o=r_o, r_i=1.0 / r_o, m_0=m0, num_quad_points=64),
-                                 n_beyn, f"beyn(r_o={r_o}, m_0={m0}, 64 nodes)"))
-    legs.append(spectral_leg(Full(), min(ne, 8), "full"))
+
+    def beyn_legs(lead, x_ref):
+        l_ii, l_ij, l_ji = lead
+        n_cnt = min(ne, 8)
+        w_full, _ = Full()((l_ji[:n_cnt], l_ii[:n_cnt], l_ij[:n_cnt]))
+        mag = w_full.abs()
+        legs, annulus = [], []
+        for r_o in radii:
+            inside = ((mag > 1.0 / r_o) & (mag < r_o)).sum(-1)
+            cnt = int(inside.max().item())
+            m0 = min(cnt + 8, b)
+            annulus.append({"r_o": r_o, "r_i": round(1.0 / r_o, 6), "in_annulus_max": cnt,
+                            "in_annulus_min": int(inside.min().item()), "of": 2 * b,
+                            "m_0": m0, "resolvable": cnt <= b})
+            legs.append(spectral_leg(Beyn(r_o=r_o, r_i=1.0 / r_o, m_0=m0, num_quad_points=64),
+                                     n_beyn, f"beyn(r_o={r_o}, m_0={m0}, 64 nodes)", lead,
+                                     x_ref))
+        return legs, annulus
+
+    lead = (a_ii, a_ij, a_ji)
+    legs, annulus = beyn_legs(lead, x_sr)
+    # the Spectral OBC on the full companion eigenproblem shows the mode
+    # machinery at this size
+    legs.append(spectral_leg(Full(), min(ne, 8), "full", lead, x_sr))
+    # A lead whose onsite spread is --evanescent-scale times wider has most
+    # of its modes far from the unit circle, few enough inside the annuli
+    # for the projection to resolve them: Beyn against Sancho-Rubio there.
+    evanescent = None
+    if args.evanescent_scale > 0:
+        e_ji, e_ii, e_ij, _ = lead_blocks(seed=args.seed, n_energies=n_beyn, block_size=b,
+                                          onsite_scale=args.evanescent_scale)
+        lead_e = tuple(torch.as_tensor(x, device=dev) for x in (e_ii, e_ij, e_ji))
+        x_e = sr(*lead_e)
+        legs_e, annulus_e = beyn_legs(lead_e, x_e)
+        evanescent = {"onsite_scale": args.evanescent_scale, "energies": n_beyn,
+                      "sancho_rubio_iterations": sr.iterations, "spectral": legs_e,
+                      "lead_eigenvalues_in_annulus": annulus_e}
     oracle = None
     if args.cpu_budget > 0:
         import oracle.obc as oobc
@@ -433,7 +460,7 @@ def obc_workload(args):
         oracle = {"energies": k, "sancho_rubio_rel_err": float(
             np.linalg.norm(x_sr[:k].cpu().numpy() - x_o) / np.linalg.norm(x_o)),
             "cpu_s_per_energy": round(t_o / k, 3), "cores": host_cores()}
-    flops_sr = sr.iterations * 7 * 8.0 * b ** 3 * ne
+    flops_sr = sr_iterations * 7 * 8.0 * b ** 3 * ne
     result = {
         "metric": "OBC lead solves: energies/s at 1024 energies x 256 (Sancho-Rubio, Beyn)",
         "value": round(ne / t_sr, 3),
@@ -446,12 +473,13 @@ def obc_workload(args):
                    "energies": ne, "block_size": b, "sancho_rubio_tol": 1e-12,
                    "beyn": {"radii": radii, "quad_points": 64,
                             "m_0": "in-annulus count + 8, at most the layer size"}},
-        "sancho_rubio": {"energies_per_s": round(ne / t_sr, 3), "iterations": sr.iterations,
+        "sancho_rubio": {"energies_per_s": round(ne / t_sr, 3), "iterations": sr_iterations,
                          "max_fixed_point_residual": float(f"{res_sr:.3e}"),
                          "algorithmic_tflops": round(flops_sr / t_sr / 1e12, 3),
                          "gemm_tflops": round(g["flops"] / max(g["busy_ms"], 1e-9) / 1e9, 3)},
         "spectral": legs,
         "lead_eigenvalues_in_annulus": annulus,
+        "evanescent_lead": evanescent,
         "oracle": oracle,
     }
     print(json.dumps(result), flush=True)
@@ -465,7 +493,9 @@ def uneven_workload(args):
     (the full mix is ~1.8e16 flops, minutes of GPU time).  Each device keeps
     H, Sigma and its results resident; lanes solve energy chunks side by
     side; each lane's schedule is captured into a HIP graph and the lanes'
-    graphs are replayed concurrently.
+    graphs are replayed concurrently, unless a lane's schedule exceeds the
+    capture limit (``engine.CAPTURE_BLOCK_STEPS_MAX``), as at full energy
+    scale: then the eager launches are timed.
     Tile shape and chunk per block size come from the autotuner.  complex128
     gives ``value``; the mixed mode is timed the same way and compared.
     """
@@ -497,23 +527,33 @@ def uneven_workload(args):
         torch.cuda.synchronize()
         t_eager = time.perf_counter() - t0
         log(f"{prec}: eager {t_eager:.2f} s")
-        t0 = time.perf_counter()
-        ub.capture()
-        torch.cuda.synchronize()
-        log(f"{prec}: captured ({time.perf_counter() - t0:.1f} s)")
-        ub.replay()
-        ub.check()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
+        graphed = ub.can_capture()
+        if graphed:
+            t0 = time.perf_counter()
+            ub.capture()
+            torch.cuda.synchronize()
+            log(f"{prec}: captured ({time.perf_counter() - t0:.1f} s)")
             ub.replay()
-        torch.cuda.synchronize()
-        t_graph = (time.perf_counter() - t0) / args.steps
-        ub.check()
-        log(f"{prec}: graph replay {t_graph:.2f} s")
+            ub.check()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                ub.replay()
+            torch.cuda.synchronize()
+            t_graph = (time.perf_counter() - t0) / args.steps
+            ub.check()
+            log(f"{prec}: graph replay {t_graph:.2f} s")
+        else:
+            # schedules above the capture limit run eagerly only
+            log(f"{prec}: {max(ub.lane_block_steps())} block steps per lane, above the "
+                f"capture limit; timing eager launches")
+            t_graph = t_eager
         results[prec] = {
             "energies_per_s": round(ub.total_energies / t_graph, 3),
             "eager_energies_per_s": round(ub.total_energies / t_eager, 3),
-            "graph_s": round(t_graph, 3), "eager_s": round(t_eager, 3),
+            "launch": "graph" if graphed else "eager",
+            "lane_block_steps": ub.lane_block_steps(),
+            "step_s": round(t_graph, 3),
+            "graph_s": round(t_graph, 3) if graphed else None, "eager_s": round(t_eager, 3),
             "required_tflops": round(ub.total_flops / t_graph / 1e12, 3),
             "lane_load_balance": round(min(ub.lane_flops) / max(ub.lane_flops), 3),
             "items": sum(len(l.items) for l in ub.lanes),
@@ -537,10 +577,11 @@ def uneven_workload(args):
             rec["speedup_vs_c128"] = round(r["energies_per_s"] / prim["energies_per_s"], 3)
         out_modes[prec] = rec
     result = {
-        "metric": "uneven batched RGF: energies/s over a 24-device size mix, HIP-graph replay",
+        "metric": "uneven batched RGF: energies/s over a 24-device size mix, "
+                  + ("HIP-graph replay" if prim["launch"] == "graph" else "eager launches"),
         "value": prim["energies_per_s"], "unit": "energies/s", "n_gpus": 1,
         "steps": args.steps, "warmup": 1,
-        "ms_per_step": round(prim["graph_s"] * 1e3, 2), "higher_is_better": True,
+        "ms_per_step": round(prim["step_s"] * 1e3, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "uneven device mix", "devices": [list(m) for m in mix],
                    "energy_scale": args.energy_scale, "lanes": args.lanes,
@@ -650,8 +691,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="c128", choices=list(PEAK_TFLOPS))
-    ap.add_argument("--compare", default="c128-3m,mixed,oz2-12",
+    ap.add_argument("--compare", default="c128-3m,mixed,oz2-12,bf16x7",
                     help="comma-separated modes timed beside the primary one ('' for none)")
+    ap.add_argument("--compare-steps", type=int, default=3,
+                    help="timed steps of each compared mode (at most --steps; 1 warm-up)")
     ap.add_argument("--energies", type=int, default=512)
     ap.add_argument("--blocks", type=int, default=64)
     ap.add_argument("--block-size", type=int, default=256)
@@ -683,6 +726,8 @@ def main():
                     help="outer contour radii of the Beyn legs (inner = 1 / radius)")
     ap.add_argument("--beyn-energies", type=int, default=32,
                     help="obc: energies of each Spectral-on-Beyn leg")
+    ap.add_argument("--evanescent-scale", type=float, default=8.0,
+                    help="obc: onsite spread of the second lead of the Beyn legs (0: none)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -734,9 +779,15 @@ def main():
         peak = PEAK_TFLOPS[m]
         # the RGF splits a call into 4 concurrent sub-batches (QX_RGF_STREAMS)
         dominant = 8.0 * b ** 3 * ((chunk + 3) // 4)
-        legs[m] = timed_leg(rgf, views, (xl, xg, xr), args.steps, args.warmup, dev,
+        # compared modes: a bounded number of steps, so that the default run
+        # stays within minutes
+        primary = m == args.precision
+        steps = args.steps if primary else max(1, min(args.steps, args.compare_steps))
+        warm = args.warmup if primary else min(args.warmup, 1)
+        legs[m] = timed_leg(rgf, views, (xl, xg, xr), steps, warm, dev,
                             not args.no_timer, peak, dominant, E * required_flops(N, b))
-        legs[m]["value"] = E * args.steps / legs[m]["elapsed"]
+        legs[m]["value"] = E * steps / legs[m]["elapsed"]
+        legs[m]["steps"] = steps
         log(f"{m}: {legs[m]['value']:.1f} energies/s")
     del xl, xg, xr
 
@@ -770,6 +821,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": ("f64+i8" if args.precision.startswith("oz2-") else
+                  "f64+bf16" if args.precision.startswith("bf16x") else
                   {"c128": "f64", "c128-3m": "f64", "c64": "f32",
                    "mixed": "f32+f64"}[args.precision]),
         "data": "synthetic",
@@ -784,7 +836,7 @@ def main():
     }
     modes_out = {}
     for m in modes:
-        r = {"energies_per_s": round(legs[m]["value"], 3),
+        r = {"energies_per_s": round(legs[m]["value"], 3), "steps": legs[m]["steps"],
              "speedup_vs_primary": round(legs[m]["value"] / prim["value"], 3),
              "latency_ms": legs[m]["latency_ms"]}
         if "roofline" in legs[m]:

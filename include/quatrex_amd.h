@@ -54,6 +54,11 @@ extern "C" {
  * product is ~2^-T of the line maxima.  Accepted by qx_zgemm_batched and the
  * RGF calls. */
 #define QX_PREC_OZAKI2(n) (16 + (n))
+/* complex128 storage and inverses; block products on bf16 matrix cores from
+ * s 7-bit digit slices of every operand line (2 <= s <= 7, the arithmetic of
+ * qx_zgemm_ozaki): relative error per product ~2^(-7 s).  Accepted by
+ * qx_zgemm_batched and the RGF calls. */
+#define QX_PREC_OZAKI_BF16(s) (40 + (s))
 
 /* GEMM operand ops */
 #define QX_OP_N 0
@@ -204,6 +209,18 @@ int qx_beyn_moments(qx_handle* handle, void* stream, int64_t n, int b,
                     int n_quad, double r_o, double r_i, const void* a_ji,
                     const void* a_ii, const void* a_ij, void* q0, void* q1,
                     int precision);
+
+/* Batched thin SVD a = u diag(s) v^H of (n, rows, cols) matrices, cols <=
+ * rows, cols <= 512, by one-sided Jacobi rotations: u (n, rows, cols) and
+ * v (n, cols, cols) contiguous, s (n, cols) doubles in descending order.
+ * Replaces the device branch of linalg.svd in Beyn's projection
+ * (src/qttools/nevp/beyn.py:96-145 with project_compute_location other than
+ * "numpy").  Stops after a sweep in which every column pair is orthogonal to
+ * |g_p^H g_q| <= tol |g_p| |g_q|, or after max_sweeps; sweeps (nullable,
+ * device, n ints) receives the sweeps used per matrix. */
+int qx_svd_batched(qx_handle* handle, void* stream, int64_t n, int rows, int cols,
+                   const void* a, int lda, int64_t stride_a, void* u, double* s, void* v,
+                   int max_sweeps, double tol, int* sweeps);
 
 /* Contact (boundary) self-energies of one lead.  Replaces the assembly of
  * src/quatrex/electron/solver.py:324-337 (left) and :370-386 (right):

@@ -42,6 +42,15 @@ def PREC_OZAKI2(n: int) -> int:
 PRECISIONS = {"c128": PREC_C128, "complex128": PREC_C128, "c64": PREC_C64, "complex64": PREC_C64,
               "c128-3m": PREC_C128_3M}
 PRECISIONS.update({f"oz2-{n}": PREC_OZAKI2(n) for n in range(8, 21)})
+
+
+def PREC_OZAKI_BF16(s: int) -> int:
+    """QX_PREC_OZAKI_BF16(s): complex128 data, block products on bf16 matrix
+    cores from s 7-bit digit slices per operand line (2..7)."""
+    return 40 + s
+
+
+PRECISIONS.update({f"bf16x{s}": PREC_OZAKI_BF16(s) for s in range(2, 8)})
 # RGF also accepts the mixed mode (complex64 products, FP64 Newton-Schulz
 # step on every block inverse).
 RGF_PRECISIONS = dict(PRECISIONS, mixed=PREC_MIXED)
@@ -63,7 +72,7 @@ EXPORTED_SYMBOLS = (
     "qx_sync_check", "qx_rgf_selected_inv", "qx_rgf_selected_solve", "qx_zgemm_batched",
     "qx_inv_batched", "qx_sancho_rubio", "qx_beyn_moments", "qx_timer_start", "qx_timer_stop",
     "qx_contact_self_energy", "qx_assemble_system_matrix", "qx_set_tuning", "qx_zgemm_ozaki",
-    "qx_inv_batched_noalloc",
+    "qx_inv_batched_noalloc", "qx_svd_batched",
 )
 
 
@@ -132,6 +141,8 @@ def load() -> ctypes.CDLL:
                                        i64, i32]
         lib.qx_inv_batched_noalloc.argtypes = [vp, vp, i64, i32, vp, i32, i64, vp, i32, i64,
                                                vp, i32, i64, vp, i32]
+        lib.qx_svd_batched.argtypes = [vp, vp, i64, i32, i32, vp, i32, i64, vp, vp, vp, i32, f64,
+                                       vp]
         lib.qx_sancho_rubio.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, i32, f64,
                                         ctypes.POINTER(ctypes.c_int)]
         lib.qx_beyn_moments.argtypes = [vp, vp, i64, i32, i32, f64, f64, vp, vp, vp, vp, vp, i32]

@@ -489,9 +489,8 @@ int qx_zgemm_batched(qx_handle* h, void* stream, int64_t batch, int m, int n, in
     struct OzScope {
       ~OzScope() { qx::active_oz_ws = qx::OzWorkspace{}; }
     } oz_scope;
-    if (const int nmod = qx::oz2_moduli(precision)) {
+    if (const size_t bytes = qx::split_workspace_bytes(precision, batch, m, n, k)) {
       QX_REQUIRE(h, "null handle");
-      const size_t bytes = qx::oz2_workspace_bytes(batch, m, n, k, nmod);
       qx::active_oz_ws = qx::OzWorkspace{h->ensure_ws(bytes), h->ws_bytes};
     }
     qx::zgemm(static_cast<hipStream_t>(stream), batch, m, n, k, op_a, op_b,
@@ -525,6 +524,23 @@ int qx_sancho_rubio(qx_handle* h, void* stream, int64_t n, int b, const void* a_
     QX_REQUIRE(h, "null handle");
     return qx::sancho_rubio(h, static_cast<hipStream_t>(stream), n, b, a_ii, a_ij, a_ji, x_ii,
                             max_iterations, tol, iterations);
+  });
+}
+
+int qx_svd_batched(qx_handle* h, void* stream, int64_t n, int rows, int cols, const void* a,
+                   int lda, int64_t stride_a, void* u, double* s, void* v, int max_sweeps,
+                   double tol, int* sweeps) {
+  return guarded(h, [&]() {
+    QX_REQUIRE(h, "null handle");
+    QX_REQUIRE(n >= 0 && rows >= 1 && cols >= 1, "svd: bad sizes");
+    if (n == 0) return QX_OK;
+    const size_t bytes = qx::svd_workspace_bytes(n, rows, cols);
+    void* ws = h->ensure_ws(bytes);
+    qx::svd_tall(static_cast<hipStream_t>(stream), n, rows, cols, qx::Mat(a, stride_a, lda),
+                 qx::Mat(u, (int64_t)rows * cols, cols), s, cols,
+                 qx::Mat(v, (int64_t)cols * cols, cols), max_sweeps, tol, ws, h->ws_bytes,
+                 sweeps);
+    return QX_OK;
   });
 }
 

@@ -766,9 +766,10 @@ void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int 
   QX_REQUIRE(opA == QX_OP_N || opA == QX_OP_C, "zgemm: bad op_a");
   QX_REQUIRE(opB == QX_OP_N || opB == QX_OP_C, "zgemm: bad op_b");
   QX_REQUIRE(precision == QX_PREC_C128 || precision == QX_PREC_C64 ||
-                 precision == QX_PREC_C128_3M || oz2_moduli(precision) > 0,
-             "zgemm: precision must be QX_PREC_C128, QX_PREC_C64, QX_PREC_C128_3M or "
-             "QX_PREC_OZAKI2(8..20)");
+                 precision == QX_PREC_C128_3M || oz2_moduli(precision) > 0 ||
+                 ozbf_slices(precision) > 0,
+             "zgemm: precision must be QX_PREC_C128, QX_PREC_C64, QX_PREC_C128_3M, "
+             "QX_PREC_OZAKI2(8..20) or QX_PREC_OZAKI_BF16(2..7)");
   if (batch <= 0 || m <= 0 || n <= 0) return;
   if (const int nmod = oz2_moduli(precision)) {
     // one launch sequence per product, in order, on the calling thread's
@@ -776,6 +777,27 @@ void zgemm_impl(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int 
     for (int a = 0; a < ng; ++a)
       zgemm_oz2(s, batch, m, n, k, opA, opB, alpha, beta, *gs[a], nmod, active_oz_ws.p,
                 active_oz_ws.bytes);
+    return;
+  }
+  if (const int sl = ozbf_slices(precision)) {
+    // bf16 digit products; the D + E - E^H beta term is formed in C first
+    // (C aliases neither D nor E) and -C^H written after
+    for (int a = 0; a < ng; ++a) {
+      const GemmOperands& o = *gs[a];
+      const bool bz = (beta.re == 0.0 && beta.im == 0.0) || o.Cin.zero();
+      Mat cin = o.Cin;
+      if (!o.E.zero() && !bz) {
+        ew_herm_add(s, batch, n, o.Cin, 1.0, o.E, o.C);
+        cin = o.C;
+      }
+      if (o.A.zero() || o.B.zero() || k <= 0)
+        ew_axpby(s, batch, m, n, bz ? Cplx{0, 0} : beta, bz ? Mat() : cin, Cplx{0, 0}, Mat(),
+                 o.C);
+      else
+        zgemm_ozaki(s, batch, m, n, k, opA, opB, alpha, o.A, o.B, beta, bz ? Mat() : cin, o.C,
+                    sl, active_oz_ws.p, active_oz_ws.bytes);
+      if (!o.Ct.zero()) ew_neg_h(s, batch, m, n, o.C, o.Ct);
+    }
     return;
   }
   if (ng > 1) {

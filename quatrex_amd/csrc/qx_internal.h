@@ -157,6 +157,19 @@ void zgemm_oz2(hipStream_t s, int64_t batch, int m, int n, int k, int opA, int o
 inline int oz2_moduli(int precision) {
   return (precision >= QX_PREC_OZAKI2(8) && precision <= QX_PREC_OZAKI2(20)) ? precision - 16 : 0;
 }
+// Number of digit slices of a bf16 Ozaki precision code, or 0.
+inline int ozbf_slices(int precision) {
+  return (precision >= QX_PREC_OZAKI_BF16(2) && precision <= QX_PREC_OZAKI_BF16(7))
+             ? precision - 40
+             : 0;
+}
+// Workspace of the split products (Ozaki II residues or bf16 digits) of one
+// launch of batch products of at most m x n x k, or 0 for other precisions.
+inline size_t split_workspace_bytes(int precision, int64_t batch, int m, int n, int k) {
+  if (const int nmod = oz2_moduli(precision)) return oz2_workspace_bytes(batch, m, n, k, nmod);
+  if (const int sl = ozbf_slices(precision)) return ozaki_workspace_bytes(batch, m, n, k, sl);
+  return 0;
+}
 // Residue workspace of the calling thread for Ozaki-II products (set by the
 // RGF drivers per sub-batch, by qx_zgemm_batched per call).
 struct OzWorkspace {
@@ -164,6 +177,14 @@ struct OzWorkspace {
   size_t bytes = 0;
 };
 extern thread_local OzWorkspace active_oz_ws;
+
+// Batched thin SVD A = U diag(S) V^H of d x m matrices (m <= d, m <= 512) by
+// one-sided Jacobi rotations; U (d x m), V (m x m) contiguous row-major, S
+// descending with stride sS; ws holds the rotated columns.
+size_t svd_workspace_bytes(int64_t n, int d, int m);
+void svd_tall(hipStream_t s, int64_t n, int d, int m, const Mat& A, const Mat& U, double* S,
+              int64_t sS, const Mat& V, int max_sweeps, double tol, void* ws, size_t ws_bytes,
+              int* sweeps);
 
 // X = inv(A - S) with partial pivoting (S may be zero).  info[e] is set to a
 // nonzero value for an exactly singular matrix.  n <= kInvMaxN.  The scratch

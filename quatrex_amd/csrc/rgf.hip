@@ -86,19 +86,20 @@ void fold(hipStream_t st, int64_t S, int m, int n, const Mat& a, double sign,
 }
 
 // Arithmetic of the block products: QX_PREC_MIXED runs them in complex64,
-// QX_PREC_OZAKI2(n) on int8 matrix cores (the block inverses stay FP64).
+// QX_PREC_OZAKI2(n) on int8 and QX_PREC_OZAKI_BF16(s) on bf16 matrix cores
+// (the block inverses stay FP64).
 int gemm_precision(int prec) {
   QX_REQUIRE(prec == QX_PREC_C128 || prec == QX_PREC_C64 || prec == QX_PREC_MIXED ||
-                 prec == QX_PREC_C128_3M || oz2_moduli(prec) > 0,
+                 prec == QX_PREC_C128_3M || oz2_moduli(prec) > 0 || ozbf_slices(prec) > 0,
              "unknown precision selector");
   return prec == QX_PREC_MIXED ? QX_PREC_C64 : prec;
 }
 
-// Residue workspace of one matrix for Ozaki-II products of blocks <= bmax.
+// Split-product workspace (Ozaki II residues or bf16 digits) of one matrix
+// for products of blocks <= bmax.
 size_t oz2_bytes_per_matrix(int prec, int bmax) {
-  const int nmod = oz2_moduli(prec);
-  if (!nmod) return 0;
-  return (oz2_workspace_bytes(1, bmax, bmax, bmax, nmod) + 255) / 256 * 256;
+  const size_t b = split_workspace_bytes(prec, 1, bmax, bmax, bmax);
+  return (b + 255) / 256 * 256;
 }
 
 // Sets the calling thread's Ozaki-II residue workspace to the part of
@@ -173,7 +174,8 @@ size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve, int prec) {
   for (int i = 0; i < la->n_blocks; ++i) bmax = std::max(bmax, (int)la->block_sizes[i]);
   size_t bytes = rgf_slot_bytes(la, solve, bmax);
   // Ozaki-II products: residues of one product per matrix of the call
-  bytes += (size_t)la->n_stack * oz2_bytes_per_matrix(prec, bmax) + (oz2_moduli(prec) ? 512 : 0);
+  const size_t per = oz2_bytes_per_matrix(prec, bmax);
+  bytes += (size_t)la->n_stack * per + (per ? 512 : 0);
   // the fused small-block kernels keep 6 N + 16 (solve) or 2 N + 3 (inverse)
   // slots of 32 x 32 per energy
   if (la->n_blocks <= 16 && bmax <= 32) {

@@ -19,7 +19,8 @@ Execution model (MI355X-first):
 - one item = system-matrix assembly kernel (A = E I - H - Sigma_B on the
   nnz pattern) -> RGF selected solve with fused density and corner-block
   outputs -> transmission;
-- ``UnevenBatch.capture()`` records each lane's whole schedule into one HIP
+- ``UnevenBatch.capture()`` records each lane's whole schedule (up to
+  ``CAPTURE_BLOCK_STEPS_MAX`` block steps) into one HIP
   graph; ``replay()`` launches the lanes' graphs on their streams (one
   launch per lane), so the thousands of small kernel launches of small
   devices cost no host time;
@@ -325,6 +326,13 @@ def plan_items(devices, n_lanes: int):
     return assign, load
 
 
+# Largest per-lane schedule (block steps) recorded into one HIP graph: ~800
+# block steps are ~10^5 kernel nodes per graph.  A schedule of ~2100 block
+# steps per lane ended the process with a host segmentation fault while it
+# was recorded, cause not established, so larger schedules are not captured.
+CAPTURE_BLOCK_STEPS_MAX = 800
+
+
 @dataclass
 class UnevenBatch:
     """Devices of mixed shapes solved together on ``n_lanes`` lanes."""
@@ -388,6 +396,15 @@ class UnevenBatch:
         for lane in self.lanes:
             lane.sync_check("UnevenBatch")
 
+    def lane_block_steps(self) -> list[int]:
+        """Block steps (sum of the block counts of its items) per lane: the
+        number of kernel nodes a lane's graph records grows with it."""
+        return [sum(self.devices[it.device].n_blocks for it in lane.items)
+                for lane in self.lanes]
+
+    def can_capture(self) -> bool:
+        return max(self.lane_block_steps()) <= CAPTURE_BLOCK_STEPS_MAX
+
     def capture(self) -> None:
         """Record each lane's schedule into its own HIP graph.
 
@@ -395,8 +412,15 @@ class UnevenBatch:
         nothing is allocated while recording).  One graph per lane, replayed
         on the lane's stream: graph launches on different streams run
         concurrently, where the branches of a single multi-stream graph are
-        not guaranteed to.
+        not guaranteed to.  Refused (RuntimeError) when a lane's schedule
+        holds more than ``CAPTURE_BLOCK_STEPS_MAX`` block steps; run such a
+        schedule with ``run_eager``.
         """
+        steps = max(self.lane_block_steps())
+        if steps > CAPTURE_BLOCK_STEPS_MAX:
+            raise RuntimeError(
+                f"UnevenBatch.capture: a lane holds {steps} block steps, above the "
+                f"capture limit of {CAPTURE_BLOCK_STEPS_MAX}; use run_eager()")
         self.run_eager()
         torch.cuda.synchronize(self.lanes[0].device)
         self.graph = []

@@ -190,7 +190,7 @@ class RGF(GFSolver):
     max_batch_size : int
         Largest number of first-axis stack entries per batch (as the
         reference; batches may exceed it, see ``get_batches``).
-    precision : {"c128", "c128-3m", "c64", "mixed", "oz2-8" ... "oz2-20"}
+    precision : {"c128", "c128-3m", "c64", "mixed", "oz2-8" ... "oz2-20", "bf16x2" ... "bf16x7"}
         Arithmetic of the block products and inverses: complex128 on FP64
         matrix cores (4M complex split; "c128-3m": 3M split, a quarter less
         matrix-core work, rounding error a small factor larger); complex64
@@ -198,8 +198,10 @@ class RGF(GFSolver):
         with one FP64 Newton-Schulz step on every block inverse; or
         "oz2-n", block products on int8 matrix cores by modular arithmetic
         with n moduli (relative error ~2^-T per product of the line maxima,
-        T = 42 for n = 12 at blocks <= 256) and FP64 block inverses.  Inputs
-        and outputs stay complex128 in every mode.
+        T = 42 for n = 12 at blocks <= 256) and FP64 block inverses; or
+        "bf16xs", block products on bf16 matrix cores from s 7-bit digit
+        slices per operand line (relative error ~2^(-7 s)) and FP64 block
+        inverses.  Inputs and outputs stay complex128 in every mode.
     retarded_offdiag : bool
         Also write the off-diagonal retarded blocks in ``selected_solve``.
     """

@@ -6,12 +6,15 @@ Drop-ins for src/qttools/nevp (``NEVP`` nevp.py:8-36, ``Beyn`` beyn.py:14-379,
 
 ``Beyn`` computes the contour moments q0, q1 on the GPU (qx_beyn_moments:
 T(z) assembled per node and inverted by the batched Gauss-Jordan kernel,
-weights accumulated in place), forms the probed moments P = q Y with the MFMA
-GEMM, and solves the small projected problems (SVD or QR, then eig) on the
-host in float64 -- the reference's default placement
-(``project_compute_location = eig_compute_location = "numpy"``).  The probe
-matrix Y comes from the same persistent ``numpy.random.default_rng(42)``
-stream as the reference, so runs are reproducible call by call.
+weights accumulated in place) and forms the probed moments P = q Y with the
+MFMA GEMM.  The projection follows ``project_compute_location`` as in the
+reference: "numpy" (its default) runs the SVD or QR on the host; any other
+value runs the SVD on the device (one-sided Jacobi, ``qx_svd_batched``) and
+forms the projected matrices U^H P1 V S^-1 there, so only those m_0 x m_0
+matrices reach the host for the small eigenproblems
+(``eig_compute_location = "numpy"``).  The probe matrix Y comes from the
+same persistent ``numpy.random.default_rng(42)`` stream as the reference,
+so runs are reproducible call by call.
 """
 
 from __future__ import annotations
@@ -126,6 +129,40 @@ class Beyn(NEVP):
             a.append(ui.conj().T @ p1[i] @ vhi.conj().T / si)
         return a, u_out
 
+    def _project_svd_device(self, p0, p1):
+        """beyn.py:96-145 with the SVD and the projection on the device:
+        returns the host projected matrices (each cut to the singular values
+        above s_max m_0 eps) and the device projector U."""
+        from quatrex_amd.ops import svd
+
+        d = p0.shape[-1]
+        u, s, v = svd(p0)
+        eps = s[:, :1] * d * np.finfo(np.float64).eps
+        keep = (s > eps).sum(-1).cpu().numpy()
+        # U^H P1 V with column j divided by s_j
+        a_dev = gemm(gemm(u, p1, op_a="C"), v) / s[:, None, :].to(torch.complex128)
+        a_host = a_dev.cpu().numpy()
+        return [a_host[i, :k, :k] for i, k in enumerate(keep)], u
+
+    def _solve_reduced_device(self, a, y, u):
+        """beyn.py:237-287 with the small eigenproblems on the host and the
+        back-projection V = U v on the device."""
+        n, d, m0 = y.shape
+        ws = np.zeros((n, m0), dtype=np.complex128)
+        vsmall = np.zeros((n, m0, m0), dtype=np.complex128)
+        found = np.zeros(n, dtype=np.int64)
+        for i in range(n):
+            w, v = np.linalg.eig(a[i])
+            k = len(w)
+            ws[i, :k] = w
+            vsmall[i, :k, :k] = v
+            found[i] = k
+        vs = gemm(u, torch.as_tensor(vsmall, device=u.device)).cpu().numpy()
+        # slots without an eigenpair keep the probe vectors, as the reference
+        cols = np.arange(m0)[None, None, :] >= found[:, None, None]
+        vs = np.where(cols, y, vs)
+        return ws, vs
+
     def _project_qr(self, p0, p1):
         """beyn.py:147-175 (host)."""
         q, r = np.linalg.qr(p0)
@@ -161,13 +198,17 @@ class Beyn(NEVP):
         y = self.rng.random((n, d, self.m_0)) + 1j * self.rng.random((n, d, self.m_0))
         q0, q1 = self._contour_integrate(a_xx)
         yd = as_device(y, q0.device)
-        p0 = gemm(q0, yd).cpu().numpy()
-        p1 = gemm(q1, yd).cpu().numpy()
-        if self.use_qr:
-            a, p_back = self._project_qr(p0, p1)
+        p0d, p1d = gemm(q0, yd), gemm(q1, yd)
+        if self.project_compute_location != "numpy" and not self.use_qr:
+            a, u = self._project_svd_device(p0d, p1d)
+            ws, vs = self._solve_reduced_device(a, y, u)
         else:
-            a, p_back = self._project_svd(p0, p1)
-        ws, vs = self._solve_reduced_system(a, y, p_back)
+            p0, p1 = p0d.cpu().numpy(), p1d.cpu().numpy()
+            if self.use_qr:
+                a, p_back = self._project_qr(p0, p1)
+            else:
+                a, p_back = self._project_svd(p0, p1)
+            ws, vs = self._solve_reduced_system(a, y, p_back)
         ws = ws.reshape(*batch_shape, self.m_0)
         vs = vs.reshape(*batch_shape, d, self.m_0)
         if old is not None:

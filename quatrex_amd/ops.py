@@ -123,5 +123,25 @@ def inv(a, precision: str = "c128", check: bool = True) -> torch.Tensor:
     return out
 
 
+def svd(a, max_sweeps: int = 30, tol: float = 1e-15):
+    """Thin SVD a = u diag(s) v^H of (..., rows, cols) stacks, cols <= rows
+    (one-sided Jacobi on the device); s descending.  Returns (u, s, v)."""
+    a = as_device(a)
+    rows, cols = a.shape[-2], a.shape[-1]
+    if cols > rows:
+        raise ValueError("svd needs cols <= rows")
+    a3 = _flat(a)
+    n = a3.shape[0]
+    u = torch.empty_like(a)
+    v = torch.empty(a.shape[:-2] + (cols, cols), dtype=torch.complex128, device=a.device)
+    s = torch.empty(a.shape[:-2] + (cols,), dtype=torch.float64, device=a.device)
+    h, st = _lib.handle(a.device), _lib.stream_ptr(a.device)
+    rc = _lib.load().qx_svd_batched(h, st, n, rows, cols, a3.data_ptr(), cols, rows * cols,
+                                    u.data_ptr(), s.data_ptr(), v.data_ptr(), int(max_sweeps),
+                                    float(tol), None)
+    _lib.check(rc, h, "svd")
+    return u, s, v
+
+
 def dagger(x: torch.Tensor) -> torch.Tensor:
     return x.conj().transpose(-2, -1).resolve_conj().contiguous()

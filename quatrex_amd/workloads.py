@@ -127,17 +127,21 @@ def make_workload(seed: int = 0, n_energies: int = 16, n_blocks: int = 8, block_
     return Workload(energies, h_diag, h_upper, sbl, sbr, sl, sg)
 
 
-def lead_blocks(seed: int = 0, n_energies: int = 1024, block_size: int = 256, eta: float = 1e-3):
+def lead_blocks(seed: int = 0, n_energies: int = 1024, block_size: int = 256, eta: float = 1e-3,
+                onsite_scale: float = 1.0):
     """Semi-infinite lead of one principal layer: (m_ji, m_ii, m_ij) stacks.
 
-    H00 Hermitian, H01 ~ CN / sqrt(b), H10 = H01^H; M = (E + i eta) I - H,
-    E ~ U[-1, 1].  Returned as numpy arrays (n_e, b, b).
+    H00 Hermitian (its spread multiplied by ``onsite_scale``),
+    H01 ~ CN / sqrt(b), H10 = H01^H; M = (E + i eta) I - H, E ~ U[-1, 1].
+    Returned as numpy arrays (n_e, b, b).  At layer 256 about 280 of the 512
+    eigenvalues of the lead's quadratic eigenproblem lie in the annulus
+    2/3 < |w| < 3/2 at onsite_scale 1, about 30 at onsite_scale 8.
     """
     rng = np.random.default_rng(seed)
     b = block_size
     e = rng.uniform(-1.0, 1.0, n_energies) + 1j * eta
     x = cn(rng, (b, b)) / np.sqrt(b)
-    h00 = (x + x.conj().T) / 2
+    h00 = onsite_scale * (x + x.conj().T) / 2
     h01 = cn(rng, (b, b)) / np.sqrt(b)
     m_ii = e[:, None, None] * np.eye(b) - h00[None]
     m_ij = np.broadcast_to(-h01, (n_energies, b, b)).copy()

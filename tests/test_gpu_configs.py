@@ -28,6 +28,9 @@ Bars (each written in its test):
   3.4e-10.  The same bound holds for the GPU result's fixed-point residual.
 - Beyn moments: 1e-10 (the oracle's own spread under a BLAS thread change is
   5e-13 at these radii).
+- oz2-12 (int8 modular products) and bf16x7 (bf16 digit products, relative
+  product error ~1e-13): 1e-6 per block, 1e-8 density and transmission (the
+  precision target).
 """
 
 import numpy as np
@@ -51,7 +54,8 @@ pytestmark = pytest.mark.gpu
 N_PROD, B_PROD = 64, 256
 # (per-block, density, transmission) bars per arithmetic
 BARS = {"c128": (1e-10, 1e-10, 1e-10), "c128-3m": (1e-10, 1e-10, 1e-10),
-        "mixed": (5e-2, 2e-3, 2e-4), "oz2-12": (1e-6, 1e-8, 1e-8)}
+        "mixed": (5e-2, 2e-3, 2e-4), "oz2-12": (1e-6, 1e-8, 1e-8),
+        "bf16x7": (1e-6, 1e-8, 1e-8)}
 
 
 def _gammas(w, ne):

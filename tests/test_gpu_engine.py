@@ -82,3 +82,16 @@ def test_autotune_table(device, tmp_path, monkeypatch):
         assert v["gemm_tile"] in (64, 128) and v["chunk"] in (2, 4)
     again = autotune([16, 32], device, n_blocks=3)
     assert again == table
+
+
+def test_capture_refused_above_the_limit(device):
+    # 30 items of a 32-block device on one lane: 960 block steps
+    from quatrex_amd.engine import CAPTURE_BLOCK_STEPS_MAX
+
+    w = make_workload(seed=3, n_energies=30, n_blocks=32, block_size=8)
+    ub = UnevenBatch([Device(w, device, "long")], n_lanes=1, default_chunk=1)
+    assert ub.lane_block_steps() == [960] and 960 > CAPTURE_BLOCK_STEPS_MAX
+    assert not ub.can_capture()
+    with pytest.raises(RuntimeError, match="capture limit"):
+        ub.capture()
+    assert ub.graph is None

@@ -183,3 +183,57 @@ def test_spectral_block_sections_matches_oracle(device, block_sections):
     got = Spectral(Full(), block_sections=block_sections, residual_tolerance=1e-6)(
         m_ii, -a_ij, -a_ji)
     assert rel_err(np_(got), ref) < 1e-9
+
+
+@pytest.mark.parametrize("shape", [(5, 256, 32), (3, 64, 40), (4, 33, 33), (2, 50, 7)],
+                         ids=["256x32", "64x40", "33x33", "50x7"])
+def test_device_svd_against_numpy(device, shape):
+    """One-sided Jacobi SVD (Beyn's projection on the device): singular
+    values against LAPACK's, reconstruction, orthonormal V and U columns,
+    descending order; one column repeated makes the matrix rank-deficient."""
+    from quatrex_amd.ops import svd
+
+    n, d, m = shape
+    rng = np.random.default_rng(d + m)
+    a = rng.standard_normal((n, d, m)) + 1j * rng.standard_normal((n, d, m))
+    a[:, :, min(3, m - 1)] = 2 * a[:, :, 0]
+    u, s, v = (np_(t) for t in svd(torch.as_tensor(a, device=device)))
+    s_ref = np.linalg.svd(a, compute_uv=False)
+    assert np.max(np.abs(s - s_ref)) < 1e-13 * s_ref.max()
+    assert np.all(np.diff(s, axis=-1) <= 0)
+    rec = (u * s[:, None, :]) @ np.conj(np.swapaxes(v, -1, -2))
+    assert rel_err(rec, a) < 1e-13
+    eye = np.eye(m)
+    assert np.abs(np.conj(np.swapaxes(v, -1, -2)) @ v - eye).max() < 1e-12
+    keep = s[0] > 1e-10 * s[0, 0]
+    uk = u[0][:, keep]
+    assert np.abs(uk.conj().T @ uk - np.eye(keep.sum())).max() < 1e-12
+
+
+def test_beyn_device_projection_matches_host(device):
+    """The device projection (Jacobi SVD, U^H P1 V S^-1 on the device) gives
+    the eigenpairs of the host projection: the projected matrices differ by a
+    unitary similarity, so the eigenvalues agree and the eigenvectors V = U v
+    are parallel."""
+    a_xx = lead(b=24, rank=12, seed=4)
+    kw = dict(r_o=3.0, r_i=1 / 3.0, m_0=24, num_quad_points=64)
+    wh, vh = Beyn(**kw)(a_xx)
+    wd, vd = Beyn(project_compute_location="device", **kw)(a_xx)
+    wh, vh, wd, vd = np_(wh), np_(vh), np_(wd), np_(vd)
+    n_checked = 0
+    for e in range(wh.shape[0]):
+        assert np.sum(np.abs(wh[e]) > 0) == np.sum(np.abs(wd[e]) > 0)
+        a_ji, a_ii, a_ij = (x[e] for x in a_xx)
+        for k, lam in enumerate(wh[e]):
+            if lam == 0 or not (1.05 / 3.0 < abs(lam) < 3.0 / 1.05):
+                continue
+            v = vh[e][:, k]
+            if np.linalg.norm((a_ji / lam + a_ii + a_ij * lam) @ v) > 1e-8 * np.linalg.norm(v):
+                continue
+            j = int(np.argmin(np.abs(wd[e] - lam)))
+            assert abs(wd[e][j] - lam) < 1e-9 * max(1.0, abs(lam))
+            w2 = vd[e][:, j]
+            cos = abs(np.vdot(v, w2)) / (np.linalg.norm(v) * np.linalg.norm(w2))
+            assert cos > 1 - 1e-9
+            n_checked += 1
+    assert n_checked > 0
