@@ -32,6 +32,7 @@ it) and the CPU baseline.
 """
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -412,7 +413,7 @@ def obc_workload(args):
     radii = [float(r) for r in str(args.beyn_radius).split(",")]
     n_beyn = min(ne, args.beyn_energies)
 
-    def beyn_legs(lead, x_ref):
+    def beyn_legs(lead, x_ref, check=None):
         l_ii, l_ij, l_ji = lead
         n_cnt = min(ne, 8)
         w_full, _ = Full()((l_ji[:n_cnt], l_ii[:n_cnt], l_ij[:n_cnt]))
@@ -428,6 +429,22 @@ def obc_workload(args):
             legs.append(spectral_leg(Beyn(r_o=r_o, r_i=1.0 / r_o, m_0=m0, num_quad_points=64),
                                      n_beyn, f"beyn(r_o={r_o}, m_0={m0}, 64 nodes)", lead,
                                      x_ref))
+            if check is not None:
+                # the same Spectral OBC on Beyn in the NumPy oracle, 2 energies;
+                # both Beyn objects fresh, so both draw the same probe block
+                import oracle.nevp as onevp
+                import oracle.obc as oobc
+
+                t0 = time.perf_counter()
+                x_o = oobc.Spectral(onevp.Beyn(r_o=r_o, r_i=1.0 / r_o, m_0=m0,
+                                               num_quad_points=64))(*(x[:2] for x in check))
+                got = Spectral(Beyn(r_o=r_o, r_i=1.0 / r_o, m_0=m0, num_quad_points=64))(
+                    *(x[:2] for x in lead)).cpu().numpy()
+                legs[-1]["oracle"] = {
+                    "energies": 2, "cpu_s": round(time.perf_counter() - t0, 1),
+                    "rel_diff": float(f"{np.linalg.norm(got - x_o) / np.linalg.norm(x_o):.3e}")}
+                log(f"  oracle Spectral/Beyn on 2 energies: rel diff "
+                    f"{legs[-1]['oracle']['rel_diff']:.2e}")
         return legs, annulus
 
     lead = (a_ii, a_ij, a_ji)
@@ -444,7 +461,8 @@ def obc_workload(args):
                                           onsite_scale=args.evanescent_scale)
         lead_e = tuple(torch.as_tensor(x, device=dev) for x in (e_ii, e_ij, e_ji))
         x_e = sr(*lead_e)
-        legs_e, annulus_e = beyn_legs(lead_e, x_e)
+        legs_e, annulus_e = beyn_legs(lead_e, x_e,
+                                      (e_ii, e_ij, e_ji) if args.cpu_budget > 0 else None)
         evanescent = {"onsite_scale": args.evanescent_scale, "energies": n_beyn,
                       "sancho_rubio_iterations": sr.iterations, "spectral": legs_e,
                       "lead_eigenvalues_in_annulus": annulus_e}
@@ -513,6 +531,13 @@ def uneven_workload(args):
     torch.cuda.synchronize()
     log(f"uneven: {len(devices)} devices, {sum(d.n_energies for d in devices)} energies "
         f"resident ({time.perf_counter() - t0:.1f} s)")
+
+    def mem(where):
+        free, total = torch.cuda.mem_get_info(dev)
+        log(f"memory {where}: torch {torch.cuda.memory_allocated(dev) / 2**30:.1f} GiB "
+            f"allocated, {(total - free) / 2**30:.1f} of {total / 2**30:.0f} GiB in use")
+
+    mem("with the devices resident")
     results = {}
     tables = {}
     for prec in ["c128"] + [m for m in args.compare.split(",") if m == "mixed"]:
@@ -520,6 +545,7 @@ def uneven_workload(args):
         tables[prec] = autotune([d.b for d in devices], dev, precision=prec,
                                 budget_bytes=args.lane_budget << 30, verbose=log)
         log(f"{prec}: autotune {time.perf_counter() - t0:.1f} s: {tables[prec]}")
+        mem(f"after the {prec} autotune")
         ub = UnevenBatch(devices, n_lanes=args.lanes, precision=prec, tuning=tables[prec],
                          lane_budget_bytes=args.lane_budget << 30)
         t0 = time.perf_counter()
@@ -527,6 +553,7 @@ def uneven_workload(args):
         torch.cuda.synchronize()
         t_eager = time.perf_counter() - t0
         log(f"{prec}: eager {t_eager:.2f} s")
+        mem(f"after the {prec} eager run")
         graphed = ub.can_capture()
         if graphed:
             t0 = time.perf_counter()
@@ -560,8 +587,11 @@ def uneven_workload(args):
             "density": [d.density.clone() for d in devices],
             "transmission": [d.transmission.clone() for d in devices],
         }
+        ub.close()
         del ub
+        gc.collect()
         torch.cuda.empty_cache()
+        mem(f"after the {prec} leg")
     prim = results["c128"]
     out_modes = {}
     for prec, r in results.items():

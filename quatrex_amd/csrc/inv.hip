@@ -449,6 +449,236 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
   panel_gathers<NT4>(perm_l, n, k0, nbw, tid, perm, sig, idx);
 }
 
+// Lane l's value v, read by every lane of the wave (l uniform).
+__device__ inline double lane_read(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ inline float lane_read(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Panel kernel with column groups (n <= 64 RPL, 32-column panels): wave w of
+// the 32 / CW waves holds columns CW w .. CW w + CW - 1 of the panel for all
+// rows, lane l the rows l + 64 j (j < RPL).  Rows stay in their registers;
+// the row interchanges of the factorisation are tracked as positions (pos_j:
+// where row l + 64 j currently stands), so the candidates of a column are the
+// rows whose position is not below it, and every row is written to its
+// final position at the end.  Per column, with one barrier:
+//   - the wave owning the column publishes the column (all rows) and the
+//     pivot key (wave max of the candidates' keys) to LDS, double-buffered
+//     by column parity so no second barrier guards the reuse;
+//   - after the barrier every wave reads the pivot row p and the multipliers
+//     of its rows from the published column, takes its own part of row p
+//     from lane p % 64 (readlane into scalars: no LDS broadcast), computes
+//     1 / x_pk and the scaled pivot row itself, and eliminates; lane p % 64
+//     then holds the scaled row in slot p / 64, and the position record
+//     follows the interchange (k <-> pos_p);
+//   - the owner of the next column computes its pivot key.
+// Pivot choice (largest |re| + |im| through the 32-bit key, ties to the lower
+// row), arithmetic and outputs are those of the other panel kernels; ties
+// are broken by the row's index at the start of the panel.
+template <typename T, int RPL, int CW>
+__global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
+  constexpr int NB = 32, NW = NB / CW, NTH = 64 * NW, NR = 64 * RPL;
+  __shared__ T col_r[2][NR], col_i[2][NR];
+  __shared__ unsigned key_b[2];
+  __shared__ int perm_l[NB];
+  const int64_t e = blockIdx.x;
+  if (g.mask && !g.mask[e]) return;  // uniform over the workgroup
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int n = g.n, k0 = g.k0, nbw = g.nbw;
+  const z2* Xc = g.Xc + e * g.sXc;
+  z2* Xn = g.Xn + e * g.sXn;
+  int* perm = g.ints + e * g.sI;
+  int* sig = perm + n;
+  int* idx = sig + n;
+
+  T xr[RPL][CW], xi[RPL][CW];
+  int pos[RPL];
+#pragma unroll
+  for (int j = 0; j < RPL; ++j) {
+    const int r = lane + 64 * j;
+    pos[j] = r;
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+      const int cc = w * CW + c;
+      z2 v = make_double2(0.0, 0.0);
+      if (r < n && cc < nbw) v = Xc[(int64_t)r * g.ldc + k0 + cc];
+      xr[j][c] = (T)v.x;
+      xi[j][c] = (T)v.y;
+    }
+  }
+  // column 0: published by wave 0
+  if (w == 0) {
+    unsigned key = 0;
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) {
+      const int r = lane + 64 * j;
+      col_r[0][r] = xr[j][0];
+      col_i[0][r] = xi[j][0];
+      if (r >= k0 && r < n) {
+        const unsigned kj = pivot_key((double)xr[j][0], (double)xi[j][0], r);
+        key = kj > key ? kj : key;
+      }
+    }
+    key = wave_max_dpp(key);
+    if (lane == 0) key_b[0] = key;
+  }
+  __syncthreads();
+
+  // The column groups run as a loop and only the CW columns of a group are
+  // unrolled (register indices stay compile-time constants).
+#pragma unroll 1
+  for (int qa = 0; qa < NW; ++qa) {
+#pragma unroll
+    for (int ca = 0; ca < CW; ++ca) {
+      const int kk = qa * CW + ca;
+      if (kk < nbw) {
+        const int k = k0 + kk;
+        const int buf = ca & 1;  // kk & 1 (CW is even)
+        const unsigned kmax = __builtin_amdgcn_readfirstlane(key_b[buf]);
+        // a candidate always exists (n - k rows), so p < n; min() keeps every
+        // index below inside the panel whatever the key holds
+        const int p = min(511 - (int)(kmax & 0x1FFu), n - 1);
+        const int lp = p & 63, jp = p >> 6;
+        const double pvr = (double)col_r[buf][p], pvi = (double)col_i[buf][p];
+        T fr[RPL], fi[RPL];
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+          fr[j] = col_r[buf][lane + 64 * j];
+          fi[j] = col_i[buf][lane + 64 * j];
+        }
+        // this wave's part of the pivot row, and its position
+        T pr[CW], pi[CW];
+#pragma unroll
+        for (int c = 0; c < CW; ++c) pr[c] = pi[c] = (T)0;
+        int q = k;
+        switch (jp) {
+#define QX_CG_CASE(J)                                  \
+  case J:                                              \
+    if (J < RPL) {                                     \
+      _Pragma("unroll") for (int c = 0; c < CW; ++c) { \
+        pr[c] = lane_read(xr[J < RPL ? J : 0][c], lp); \
+        pi[c] = lane_read(xi[J < RPL ? J : 0][c], lp); \
+      }                                                \
+      q = __builtin_amdgcn_readlane(pos[J < RPL ? J : 0], lp); \
+    }                                                  \
+    break;
+          QX_CG_CASE(0)
+          QX_CG_CASE(1)
+          QX_CG_CASE(2)
+          QX_CG_CASE(3)
+          QX_CG_CASE(4)
+          QX_CG_CASE(5)
+          QX_CG_CASE(6)
+          QX_CG_CASE(7)
+#undef QX_CG_CASE
+          default: break;
+        }
+        double vr, vi;
+        complex_recip(pvr, pvi, vr, vi);
+        // scaled pivot row; the active column (owner wave only) becomes 1 / x_pk
+        T sr[CW], si[CW];
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+          const double a = (double)pr[c], b = (double)pi[c];
+          sr[c] = (T)(a * vr - b * vi);
+          si[c] = (T)(a * vi + b * vr);
+        }
+        const bool owner = (w == qa);
+        if (owner) {
+          sr[ca] = (T)vr;
+          si[ca] = (T)vi;
+#pragma unroll
+          for (int j = 0; j < RPL; ++j) {
+            xr[j][ca] = (T)0;
+            xi[j][ca] = (T)0;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+#pragma unroll
+          for (int c = 0; c < CW; ++c) {
+            xr[j][c] -= fr[j] * sr[c] - fi[j] * si[c];
+            xi[j][c] -= fr[j] * si[c] + fi[j] * sr[c];
+          }
+        }
+        // the pivot row becomes the scaled row
+        if (lane == lp) {
+          switch (jp) {
+#define QX_CG_SET(J)                                   \
+  case J:                                              \
+    if (J < RPL) {                                     \
+      _Pragma("unroll") for (int c = 0; c < CW; ++c) { \
+        xr[J < RPL ? J : 0][c] = sr[c];                \
+        xi[J < RPL ? J : 0][c] = si[c];                \
+      }                                                \
+    }                                                  \
+    break;
+            QX_CG_SET(0)
+            QX_CG_SET(1)
+            QX_CG_SET(2)
+            QX_CG_SET(3)
+            QX_CG_SET(4)
+            QX_CG_SET(5)
+            QX_CG_SET(6)
+            QX_CG_SET(7)
+#undef QX_CG_SET
+            default: break;
+          }
+        }
+        // interchange k <-> q: row p moves to position k, the row at k to q
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+          const int r = lane + 64 * j;
+          pos[j] = (r == p) ? k : ((pos[j] == k) ? q : pos[j]);
+        }
+        if (tid == 0) {
+          perm_l[kk] = q;
+          if ((kmax >> 9) == 0u) g.info[e] = 1;  // no nonzero pivot: singular
+        }
+        // the next column: published by its owner
+        if (kk + 1 < nbw) {
+          const int cn = (ca + 1) % CW;  // constant once unrolled
+          const int wn = qa + (ca + 1) / CW;
+          if (w == wn) {
+            unsigned key = 0;
+#pragma unroll
+            for (int j = 0; j < RPL; ++j) {
+              const int r = lane + 64 * j;
+              col_r[buf ^ 1][r] = xr[j][cn];
+              col_i[buf ^ 1][r] = xi[j][cn];
+              if (pos[j] > k && r < n) {
+                const unsigned kj = pivot_key((double)xr[j][cn], (double)xi[j][cn], r);
+                key = kj > key ? kj : key;
+              }
+            }
+            key = wave_max_dpp(key);
+            if (lane == 0) key_b[buf ^ 1] = key;
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+  __syncthreads();  // perm_l complete
+
+#pragma unroll
+  for (int j = 0; j < RPL; ++j) {
+    const int r = lane + 64 * j;
+    if (r < n) {
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        const int cc = w * CW + c;
+        if (cc < nbw)
+          Xn[(int64_t)pos[j] * g.ldn + k0 + cc] = make_double2((double)xr[j][c], (double)xi[j][c]);
+      }
+    }
+  }
+  panel_gathers<NTH>(perm_l, n, k0, nbw, tid, perm, sig, idx);
+}
+
 struct UpdArgs {
   int n, k0, nbw, tiles;
   const z2* Xc;
@@ -598,19 +828,37 @@ __global__ __launch_bounds__(NT) void inv_masked_start(int n, const z2* A, int64
   }
 }
 
-// Four threads per row for 32-column panels of n <= 256 (QX_INV_PANEL=1
-// selects the one-thread-per-row kernel).
-bool use_panel4() {
-  static const bool on = []() {
-    const char* e = std::getenv("QX_INV_PANEL");
-    return !(e && std::atoi(e) == 1);
-  }();
-  return on;
+// Panel kernel of the 32-column panels (read per call): 0 (default) column
+// groups (inv_panel_cg, any n <= 512), 4 four threads per row (n <= 256),
+// 1 one thread per row.  QX_INV_CW = 4 (default: eight waves) or 8 (four
+// waves) sets the column-group width.
+int inv_panel_kind() {
+  const char* e = std::getenv("QX_INV_PANEL");
+  const int v = e ? std::atoi(e) : 0;
+  return (v == 1 || v == 4) ? v : 0;
+}
+
+int inv_group_width() {
+  const char* e = std::getenv("QX_INV_CW");
+  return (e && std::atoi(e) == 8) ? 8 : 4;
+}
+
+template <typename T>
+void launch_panel_cg(hipStream_t s, int64_t batch, int n, const PanelArgs& pa) {
+  const bool narrow = inv_group_width() == 4;
+  if (n <= 256) {
+    if (narrow) hipLaunchKernelGGL((inv_panel_cg<T, 4, 4>), dim3((unsigned)batch), dim3(512), 0, s, pa);
+    else hipLaunchKernelGGL((inv_panel_cg<T, 4, 8>), dim3((unsigned)batch), dim3(256), 0, s, pa);
+  } else {
+    if (narrow) hipLaunchKernelGGL((inv_panel_cg<T, 8, 4>), dim3((unsigned)batch), dim3(512), 0, s, pa);
+    else hipLaunchKernelGGL((inv_panel_cg<T, 8, 8>), dim3((unsigned)batch), dim3(256), 0, s, pa);
+  }
 }
 
 template <typename T, int NB, int RPT>
 void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScratch& scr,
                  int s0, int* info, const int* mask) {
+  const int kind = inv_panel_kind();
   const int npan = (n + NB - 1) / NB;
   const Mat B[2] = {X, Mat(scr.buf, scr.buf_stride, n)};
   int cur = s0;
@@ -619,7 +867,9 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
     const int k0 = p * NB, nbw = min(NB, n - k0), nxt = 1 - cur;
     PanelArgs pa{n, k0, nbw, B[cur].p, B[cur].stride, B[cur].ld,
                  B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, info, mask};
-    if (NB == 32 && RPT == 1 && use_panel4())
+    if (NB == 32 && kind == 0)
+      launch_panel_cg<T>(s, batch, n, pa);
+    else if (NB == 32 && RPT == 1 && kind == 4)
       hipLaunchKernelGGL((inv_panel4<T>), dim3((unsigned)batch), dim3(NT4), 0, s, pa);
     else
       hipLaunchKernelGGL((inv_panel<T, NB, RPT>), dim3((unsigned)batch), dim3(NT), 0, s, pa);
@@ -643,15 +893,17 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
 
 }  // namespace
 
-// Panel width: 32 columns up to n = 256 (16 above, where two rows per
-// thread double the per-column work); QX_INV_NB=16|32 overrides for n <= 256.
+// Panel width: 32 columns; with the older panel kernels 16 above n = 256
+// (where two rows per thread double their per-column work).  QX_INV_NB=16|32
+// overrides for n <= 256 (16 runs the one-thread-per-row kernel).
 int inv_panel_width(int n) {
   static const int forced_nb = []() {
     const char* e = std::getenv("QX_INV_NB");
     const int v = e ? std::atoi(e) : 0;
     return (v == 16 || v == 32) ? v : 0;
   }();
-  return n > 256 ? 16 : (forced_nb ? forced_nb : 32);
+  if (n > 256) return inv_panel_kind() == 0 ? 32 : 16;
+  return forced_nb ? forced_nb : 32;
 }
 
 Mat zinv_input(int n, const Mat& X, const InvScratch& scr) {
@@ -685,8 +937,8 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const
     return;
   }
 
-  const bool big = n > 256;
   const int NB = inv_panel_width(n);
+  const bool big = n > 256 && NB == 16;
   const int npan = (n + NB - 1) / NB;
   // start in the buffer that makes the last panel write the scratch buffer
   const int s0 = 1 ^ (npan % 2);

@@ -33,6 +33,7 @@ The per-device computation is the reference's RGF.selected_solve
 
 from __future__ import annotations
 
+import gc
 import json
 import os
 import threading
@@ -182,6 +183,15 @@ class Lane:
         _lib.load().qx_destroy(self.h)
         self.h = _lib.new_handle(self.device)
         _lib.set_tuning(self.h, streams=1, gemm_tile=0)
+
+    def close(self) -> None:
+        """Free the pool and destroy the handle (with its workspace)."""
+        self.stream.synchronize()
+        self._views.clear()
+        self._pool = torch.empty(0, dtype=torch.complex128, device=self.device)
+        if self.h is not None:
+            _lib.load().qx_destroy(self.h)
+            self.h = None
 
     def views(self, d: Device, c: int):
         """(A, X^<, X^>, corner) containers for a chunk of c energies of d."""
@@ -396,6 +406,13 @@ class UnevenBatch:
         for lane in self.lanes:
             lane.sync_check("UnevenBatch")
 
+    def close(self) -> None:
+        """Free the lanes' pools, graphs and library handles (with their RGF
+        workspaces); the batch cannot run afterwards."""
+        self.graph = None
+        for lane in self.lanes:
+            lane.close()
+
     def lane_block_steps(self) -> list[int]:
         """Block steps (sum of the block counts of its items) per lane: the
         number of kernel nodes a lane's graph records grows with it."""
@@ -542,6 +559,9 @@ def autotune(block_sizes, device, precision: str = "c128", n_blocks: int = 6,
             del d
         out[b] = dict(best[1], ms_per_energy_probe=round(best[0] * 1e3, 4))
         known[str(b)] = out[b]
+    if lane is not None:
+        lane.close()
+        gc.collect()  # the probe devices' containers hold reference cycles
     if cache:
         table[key] = known
         try:

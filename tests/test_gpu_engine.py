@@ -95,3 +95,5 @@ def test_capture_refused_above_the_limit(device):
     with pytest.raises(RuntimeError, match="capture limit"):
         ub.capture()
     assert ub.graph is None
+    ub.close()
+    assert all(lane.h is None for lane in ub.lanes)

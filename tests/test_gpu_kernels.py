@@ -215,6 +215,47 @@ def test_inverse_extreme_scales(device, n, scale):
         assert rel_err(got[e], ref) < 1e-15 * np.linalg.cond(a[e]) * n, (e, rel_err(got[e], ref))
 
 
+@pytest.mark.parametrize("n", [33, 64, 100, 256, 300, 416, 512])
+@pytest.mark.parametrize("prec", [0, 1], ids=["c128", "c64"])
+def test_panel_kernels_agree(device, monkeypatch, n, prec):
+    """The column-group panel kernel (default; 4 or 8 columns per wave)
+    against the four-threads-per-row kernel (n <= 256) and the
+    one-thread-per-row kernel (all n): the same Gauss-Jordan elimination with
+    partial pivoting, so equal to rounding (near-ties between pivot
+    candidates may be broken differently)."""
+    rng = np.random.default_rng(3 * n + prec)
+    batch = 3
+    a = _cplx(rng, (batch, n, n))
+    a[:, 0, 0] = 0.0
+    A = torch.as_tensor(a, device=device)
+    variants = [("0", "4"), ("0", "8"), ("1", "8")] + ([("4", "8")] if n <= 256 else [])
+    outs = {}
+    for kind, cw in variants:
+        monkeypatch.setenv("QX_INV_PANEL", kind)
+        monkeypatch.setenv("QX_INV_CW", cw)
+        X = torch.empty_like(A)
+        _inv(device, A, X, prec)
+        outs[(kind, cw)] = X.cpu().numpy()
+    cond = max(np.linalg.cond(a[e]) for e in range(batch))
+    tol = (1e-15 if prec == 0 else 1e-7) * cond * n
+    for e in range(batch):
+        ref = np.linalg.inv(a[e])
+        for v, o in outs.items():
+            assert rel_err(o[e], ref) < tol, (v, e, rel_err(o[e], ref))
+            assert rel_err(o[e], outs[("1", "8")][e]) < tol, (v, e)
+
+
+@pytest.mark.parametrize("n", [64, 300])
+def test_singular_panel_raises(device, n):
+    a = np.zeros((2, n, n), complex)
+    a[0] = np.eye(n)
+    a[1, :, : n // 2] = 1.0  # rank 1
+    A = torch.as_tensor(a, device=device)
+    X = torch.empty_like(A)
+    with pytest.raises(np.linalg.LinAlgError):
+        _inv(device, A, X, 0)
+
+
 def test_inverse_in_place(device):
     rng = np.random.default_rng(5)
     a = _cplx(rng, (3, 40, 40))
