@@ -449,13 +449,17 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
   panel_gathers<NT4>(perm_l, n, k0, nbw, tid, perm, sig, idx);
 }
 
-// Lane l's value v, read by every lane of the wave (l uniform).
-__device__ inline double lane_read(double v, int l) {
-  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
-                          __builtin_amdgcn_readlane(__double2loint(v), l));
-}
-__device__ inline float lane_read(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+// Wave-wide unsigned max with DPP only: quad and row butterflies give every
+// lane its row's maximum, row_bcast:15 / row_bcast:31 carry the row maxima
+// up to lane 63, which is read once.  Every lane of the wave must be active.
+__device__ inline unsigned wave_max_bcast(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xf, 0xf, false));   // quad xor 1
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xf, 0xf, false));   // quad xor 2
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xf, 0xf, false));  // row_ror 4
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xf, 0xf, false));  // row_ror 8
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // Panel kernel with column groups (n <= 64 RPL, 32-column panels): wave w of
@@ -464,25 +468,36 @@ __device__ inline float lane_read(float v, int l) {
 // the row interchanges of the factorisation are tracked as positions (pos_j:
 // where row l + 64 j currently stands), so the candidates of a column are the
 // rows whose position is not below it, and every row is written to its
-// final position at the end.  Per column, with one barrier:
-//   - the wave owning the column publishes the column (all rows) and the
-//     pivot key (wave max of the candidates' keys) to LDS, double-buffered
-//     by column parity so no second barrier guards the reuse;
-//   - after the barrier every wave reads the pivot row p and the multipliers
-//     of its rows from the published column, takes its own part of row p
-//     from lane p % 64 (readlane into scalars: no LDS broadcast), computes
-//     1 / x_pk and the scaled pivot row itself, and eliminates; lane p % 64
-//     then holds the scaled row in slot p / 64, and the position record
-//     follows the interchange (k <-> pos_p);
-//   - the owner of the next column computes its pivot key.
+// final position at the end.
+//
+// The scaling of a pivot row by 1 / x_pk is deferred to the end of the panel:
+// the pivot row keeps its values (its multiplier is taken as zero, its entry
+// in the active column becomes 1), and every later elimination step acts on
+// it linearly, so scaling it once at the end gives the same row.  Nothing
+// then has to be written into the register slot of a runtime row.
+//
+// Per column, with one barrier:
+//   - the wave owning the column has published the column (all rows), the
+//     pivot key (wave max of the candidates' keys) and the pivot value to
+//     LDS, double-buffered by column parity so no second barrier guards the
+//     reuse;
+//   - after the barrier every wave reads the pivot key and value and the
+//     multipliers of its rows; the lane holding row p writes this wave's
+//     part of it (and its position) to a per-wave LDS slot, which the wave
+//     reads back; 1 / x_pk, the scaled pivot row and the elimination follow;
+//   - the owner of the next column computes and publishes its pivot data.
 // Pivot choice (largest |re| + |im| through the 32-bit key, ties to the lower
-// row), arithmetic and outputs are those of the other panel kernels; ties
-// are broken by the row's index at the start of the panel.
+// row), arithmetic and outputs are those of the other panel kernels up to
+// rounding; ties are broken by the row's index at the start of the panel.
 template <typename T, int RPL, int CW>
 __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
   constexpr int NB = 32, NW = NB / CW, NTH = 64 * NW, NR = 64 * RPL;
   __shared__ T col_r[2][NR], col_i[2][NR];
+  __shared__ T piv_r[2], piv_i[2];
   __shared__ unsigned key_b[2];
+  __shared__ T prow_r[NW][CW], prow_i[NW][CW];
+  __shared__ int prow_pos[NW];
+  __shared__ double rcp_r[NB], rcp_i[NB];
   __shared__ int perm_l[NB];
   const int64_t e = blockIdx.x;
   if (g.mask && !g.mask[e]) return;  // uniform over the workgroup
@@ -495,36 +510,50 @@ __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
   int* idx = sig + n;
 
   T xr[RPL][CW], xi[RPL][CW];
-  int pos[RPL];
+  int pos[RPL], step[RPL];  // position; panel column where the row was pivot (-1: none)
 #pragma unroll
   for (int j = 0; j < RPL; ++j) {
     const int r = lane + 64 * j;
     pos[j] = r;
+    step[j] = -1;
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
+      // unconditional loads from a clamped address, so that all of them are
+      // in flight together (a load under a branch is waited for in the branch)
       const int cc = w * CW + c;
-      z2 v = make_double2(0.0, 0.0);
-      if (r < n && cc < nbw) v = Xc[(int64_t)r * g.ldc + k0 + cc];
-      xr[j][c] = (T)v.x;
-      xi[j][c] = (T)v.y;
+      const bool ok = r < n && cc < nbw;
+      const z2 v = Xc[ok ? (int64_t)r * g.ldc + k0 + cc : 0];
+      xr[j][c] = ok ? (T)v.x : (T)0;
+      xi[j][c] = ok ? (T)v.y : (T)0;
     }
   }
-  // column 0: published by wave 0
-  if (w == 0) {
+  // Pivot data of panel column cn (register cn of the owner wave) into
+  // buffer b: the column, the key of the candidates (rows at position > kb)
+  // and the pivot value.
+  auto publish = [&](int cn, int b, int kb) {
     unsigned key = 0;
 #pragma unroll
     for (int j = 0; j < RPL; ++j) {
       const int r = lane + 64 * j;
-      col_r[0][r] = xr[j][0];
-      col_i[0][r] = xi[j][0];
-      if (r >= k0 && r < n) {
-        const unsigned kj = pivot_key((double)xr[j][0], (double)xi[j][0], r);
+      col_r[b][r] = xr[j][cn];
+      col_i[b][r] = xi[j][cn];
+      if (pos[j] > kb && r < n) {
+        const unsigned kj = pivot_key((double)xr[j][cn], (double)xi[j][cn], r);
         key = kj > key ? kj : key;
       }
     }
-    key = wave_max_dpp(key);
-    if (lane == 0) key_b[0] = key;
-  }
+    key = wave_max_bcast(key);
+    const int pn = min(511 - (int)(key & 0x1FFu), n - 1);
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) {
+      if (lane + 64 * j == pn) {
+        piv_r[b] = xr[j][cn];
+        piv_i[b] = xi[j][cn];
+      }
+    }
+    if (lane == 0) key_b[b] = key;
+  };
+  if (w == 0) publish(0, 0, k0 - 1);
   __syncthreads();
 
   // The column groups run as a loop and only the CW columns of a group are
@@ -539,60 +568,52 @@ __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
         const int buf = ca & 1;  // kk & 1 (CW is even)
         const unsigned kmax = __builtin_amdgcn_readfirstlane(key_b[buf]);
         // a candidate always exists (n - k rows), so p < n; min() keeps every
-        // index below inside the panel whatever the key holds
+        // index inside the panel whatever the key holds
         const int p = min(511 - (int)(kmax & 0x1FFu), n - 1);
-        const int lp = p & 63, jp = p >> 6;
-        const double pvr = (double)col_r[buf][p], pvi = (double)col_i[buf][p];
+        const double pvr = (double)piv_r[buf], pvi = (double)piv_i[buf];
         T fr[RPL], fi[RPL];
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
-          fr[j] = col_r[buf][lane + 64 * j];
-          fi[j] = col_i[buf][lane + 64 * j];
+          const bool piv = (lane + 64 * j == p);
+          fr[j] = piv ? (T)0 : col_r[buf][lane + 64 * j];
+          fi[j] = piv ? (T)0 : col_i[buf][lane + 64 * j];
         }
-        // this wave's part of the pivot row, and its position
-        T pr[CW], pi[CW];
+        // this wave's part of row p and its position, through LDS
 #pragma unroll
-        for (int c = 0; c < CW; ++c) pr[c] = pi[c] = (T)0;
-        int q = k;
-        switch (jp) {
-#define QX_CG_CASE(J)                                  \
-  case J:                                              \
-    if (J < RPL) {                                     \
-      _Pragma("unroll") for (int c = 0; c < CW; ++c) { \
-        pr[c] = lane_read(xr[J < RPL ? J : 0][c], lp); \
-        pi[c] = lane_read(xi[J < RPL ? J : 0][c], lp); \
-      }                                                \
-      q = __builtin_amdgcn_readlane(pos[J < RPL ? J : 0], lp); \
-    }                                                  \
-    break;
-          QX_CG_CASE(0)
-          QX_CG_CASE(1)
-          QX_CG_CASE(2)
-          QX_CG_CASE(3)
-          QX_CG_CASE(4)
-          QX_CG_CASE(5)
-          QX_CG_CASE(6)
-          QX_CG_CASE(7)
-#undef QX_CG_CASE
-          default: break;
+        for (int j = 0; j < RPL; ++j) {
+          if (lane + 64 * j == p) {
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+              prow_r[w][c] = xr[j][c];
+              prow_i[w][c] = xi[j][c];
+            }
+            prow_pos[w] = pos[j];
+          }
         }
+        // The other lanes read what one lane of this wave wrote: without the
+        // fences the compiler may forward the writing lane's registers and
+        // place the other lanes' reads ahead of the write.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         double vr, vi;
         complex_recip(pvr, pvi, vr, vi);
-        // scaled pivot row; the active column (owner wave only) becomes 1 / x_pk
         T sr[CW], si[CW];
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
-          const double a = (double)pr[c], b = (double)pi[c];
+          const double a = (double)prow_r[w][c], b = (double)prow_i[w][c];
           sr[c] = (T)(a * vr - b * vi);
           si[c] = (T)(a * vi + b * vr);
         }
-        const bool owner = (w == qa);
-        if (owner) {
+        const int q = prow_pos[w];
+        if (w == qa) {
+          // active column: -f / x_pk for the other rows, 1 (scaled at the
+          // end) for the pivot row
           sr[ca] = (T)vr;
           si[ca] = (T)vi;
 #pragma unroll
           for (int j = 0; j < RPL; ++j) {
-            xr[j][ca] = (T)0;
+            xr[j][ca] = (lane + 64 * j == p) ? (T)1 : (T)0;
             xi[j][ca] = (T)0;
           }
         }
@@ -600,79 +621,53 @@ __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
         for (int j = 0; j < RPL; ++j) {
 #pragma unroll
           for (int c = 0; c < CW; ++c) {
-            xr[j][c] -= fr[j] * sr[c] - fi[j] * si[c];
-            xi[j][c] -= fr[j] * si[c] + fi[j] * sr[c];
-          }
-        }
-        // the pivot row becomes the scaled row
-        if (lane == lp) {
-          switch (jp) {
-#define QX_CG_SET(J)                                   \
-  case J:                                              \
-    if (J < RPL) {                                     \
-      _Pragma("unroll") for (int c = 0; c < CW; ++c) { \
-        xr[J < RPL ? J : 0][c] = sr[c];                \
-        xi[J < RPL ? J : 0][c] = si[c];                \
-      }                                                \
-    }                                                  \
-    break;
-            QX_CG_SET(0)
-            QX_CG_SET(1)
-            QX_CG_SET(2)
-            QX_CG_SET(3)
-            QX_CG_SET(4)
-            QX_CG_SET(5)
-            QX_CG_SET(6)
-            QX_CG_SET(7)
-#undef QX_CG_SET
-            default: break;
+            xr[j][c] = fma(-fr[j], sr[c], fma(fi[j], si[c], xr[j][c]));
+            xi[j][c] = fma(-fr[j], si[c], fma(-fi[j], sr[c], xi[j][c]));
           }
         }
         // interchange k <-> q: row p moves to position k, the row at k to q
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
           const int r = lane + 64 * j;
+          step[j] = (r == p) ? kk : step[j];
           pos[j] = (r == p) ? k : ((pos[j] == k) ? q : pos[j]);
         }
         if (tid == 0) {
           perm_l[kk] = q;
+          rcp_r[kk] = vr;
+          rcp_i[kk] = vi;
           if ((kmax >> 9) == 0u) g.info[e] = 1;  // no nonzero pivot: singular
         }
-        // the next column: published by its owner
         if (kk + 1 < nbw) {
-          const int cn = (ca + 1) % CW;  // constant once unrolled
-          const int wn = qa + (ca + 1) / CW;
-          if (w == wn) {
-            unsigned key = 0;
-#pragma unroll
-            for (int j = 0; j < RPL; ++j) {
-              const int r = lane + 64 * j;
-              col_r[buf ^ 1][r] = xr[j][cn];
-              col_i[buf ^ 1][r] = xi[j][cn];
-              if (pos[j] > k && r < n) {
-                const unsigned kj = pivot_key((double)xr[j][cn], (double)xi[j][cn], r);
-                key = kj > key ? kj : key;
-              }
-            }
-            key = wave_max_dpp(key);
-            if (lane == 0) key_b[buf ^ 1] = key;
-          }
+          if (w == qa + (ca + 1) / CW) publish((ca + 1) % CW, buf ^ 1, k);
           __syncthreads();
         }
       }
     }
   }
-  __syncthreads();  // perm_l complete
+  __syncthreads();  // perm_l and the pivot reciprocals complete
 
 #pragma unroll
   for (int j = 0; j < RPL; ++j) {
     const int r = lane + 64 * j;
     if (r < n) {
+      double ar = 1.0, ai = 0.0;
+      if (step[j] >= 0) {
+        ar = rcp_r[step[j]];
+        ai = rcp_i[step[j]];
+      }
 #pragma unroll
       for (int c = 0; c < CW; ++c) {
         const int cc = w * CW + c;
-        if (cc < nbw)
-          Xn[(int64_t)pos[j] * g.ldn + k0 + cc] = make_double2((double)xr[j][c], (double)xi[j][c]);
+        if (cc < nbw) {
+          const double a = (double)xr[j][c], b = (double)xi[j][c];
+          double yr = a, yi = b;
+          if (step[j] >= 0) {
+            yr = (double)(T)(a * ar - b * ai);
+            yi = (double)(T)(a * ai + b * ar);
+          }
+          Xn[(int64_t)pos[j] * g.ldn + k0 + cc] = make_double2(yr, yi);
+        }
       }
     }
   }
@@ -731,24 +726,37 @@ __global__ __launch_bounds__(NT) void inv_update(UpdArgs g) {
         const int rl = wm * 32 + i * 16 + MM::row(lane, q);
         const int r = r0 + rl;
         const int c = c0 + wn * 32 + j * 16 + (lane & 15);
-        z2 v = make_double2(0.0, 0.0);
-        if (r < n && c < n && !(r >= k0 && r < k0 + nbw)) v = Xc[(int64_t)sig_r[rl] * g.ldc + c];
-        aR[i][j][q] = (T)v.x;
-        aI[i][j][q] = (T)v.y;
+        // unconditional loads from clamped addresses: all in flight together
+        const bool ok = r < n && c < n && !(r >= k0 && r < k0 + nbw);
+        const z2 v = Xc[ok ? (int64_t)sig_r[rl] * g.ldc + c : 0];
+        aR[i][j][q] = ok ? (T)v.x : (T)0;
+        aI[i][j][q] = ok ? (T)v.y : (T)0;
       }
-  // A = reduced panel rows of this tile, B = gathered pivot rows
+  // A = reduced panel rows of this tile, B = gathered pivot rows: all loads
+  // first (unconditional, clamped addresses), then the LDS writes
+  constexpr int NS = 64 * NB / NT;
+  z2 va[NS], vb[NS];
 #pragma unroll
-  for (int q = tid; q < 64 * NB; q += NT) {
+  for (int it = 0; it < NS; ++it) {
+    const int q = tid + it * NT;
     const int r = q / NB, k = q - r * NB;
-    z2 v = make_double2(0.0, 0.0);
-    if (r0 + r < n && k < nbw) v = Xn[(int64_t)(r0 + r) * g.ldn + k0 + k];
-    As[0][k][r] = (T)v.x;
-    As[1][k][r] = (T)v.y;
+    const bool oka = r0 + r < n && k < nbw;
+    va[it] = Xn[oka ? (int64_t)(r0 + r) * g.ldn + k0 + k : 0];
+    if (!oka) va[it] = make_double2(0.0, 0.0);
     const int k2 = q >> 6, c = q & 63;
-    z2 u = make_double2(0.0, 0.0);
-    if (k2 < nbw && c0 + c < n) u = Xc[(int64_t)sig_k[k2] * g.ldc + c0 + c];
-    Bs[0][k2][c] = (T)u.x;
-    Bs[1][k2][c] = (T)u.y;
+    const bool okb = k2 < nbw && c0 + c < n;
+    vb[it] = Xc[okb ? (int64_t)sig_k[k2] * g.ldc + c0 + c : 0];
+    if (!okb) vb[it] = make_double2(0.0, 0.0);
+  }
+#pragma unroll
+  for (int it = 0; it < NS; ++it) {
+    const int q = tid + it * NT;
+    const int r = q / NB, k = q - r * NB;
+    As[0][k][r] = (T)va[it].x;
+    As[1][k][r] = (T)va[it].y;
+    const int k2 = q >> 6, c = q & 63;
+    Bs[0][k2][c] = (T)vb[it].x;
+    Bs[1][k2][c] = (T)vb[it].y;
   }
   __syncthreads();
 #pragma unroll
