@@ -222,6 +222,31 @@ int qx_svd_batched(qx_handle* handle, void* stream, int64_t n, int rows, int col
                    const void* a, int lda, int64_t stride_a, void* u, double* s, void* v,
                    int max_sweeps, double tol, int* sweeps);
 
+/* Reflected-mode classification of the Spectral OBC, fused into one pass
+ * over the eigenpairs.  Replaces the elementwise part of
+ * src/qttools/boundary_conditions/obc/spectral.py:198-320
+ * (_find_reflected_modes, no injection vectors) and :160-196
+ * (_compute_dE_dk): for every eigenpair (ws[e, j], column j of vs[e]), given
+ * the products av_ji = a_ji vs, av_ii = a_ii vs, av_ij = a_ij vs,
+ *   residual = || av_ji w^-1 + av_ii + av_ij w || / || v ||  (/ |w| when
+ *              residual_normalization),
+ *   de_dk    = i (v^H av_ji) w^-1 - i (v^H av_ij) w,  k = -i log w,
+ * and mask = 1 for a propagating mode (|Im k| < min_decay, Re dE/dk < 0,
+ * |Re dE/dk| / (|Im dE/dk| + eps) > min_propagation) or a decaying one
+ * (Im k < -min_decay, or < -eta_decay when the ratio is not above
+ * min_propagation; Im k > -max_decay) whose residual is below
+ * residual_tolerance; NaN and infinities in dE/dk and k count as zero.
+ * ws (n, m), vs and the products (n, b, m) contiguous complex128; mask
+ * (n, m) bytes; residual (n, m) doubles and de_dk (n, m) complex128 are
+ * nullable. */
+int qx_spectral_modes(qx_handle* handle, void* stream, int64_t n, int b, int m,
+                      const void* ws, const void* vs, const void* av_ji,
+                      const void* av_ii, const void* av_ij, double min_decay,
+                      double max_decay, double min_propagation,
+                      double residual_tolerance, double eta_decay,
+                      int residual_normalization, unsigned char* mask,
+                      double* residual, void* de_dk);
+
 /* Contact (boundary) self-energies of one lead.  Replaces the assembly of
  * src/quatrex/electron/solver.py:324-337 (left) and :370-386 (right):
  *   sigma_r = m10 g m01,  Gamma = i (sigma_r - sigma_r^H),

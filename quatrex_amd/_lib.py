@@ -72,7 +72,7 @@ EXPORTED_SYMBOLS = (
     "qx_sync_check", "qx_rgf_selected_inv", "qx_rgf_selected_solve", "qx_zgemm_batched",
     "qx_inv_batched", "qx_sancho_rubio", "qx_beyn_moments", "qx_timer_start", "qx_timer_stop",
     "qx_contact_self_energy", "qx_assemble_system_matrix", "qx_set_tuning", "qx_zgemm_ozaki",
-    "qx_inv_batched_noalloc", "qx_svd_batched",
+    "qx_inv_batched_noalloc", "qx_svd_batched", "qx_spectral_modes",
 )
 
 
@@ -143,6 +143,8 @@ def load() -> ctypes.CDLL:
                                                vp, i32, i64, vp, i32]
         lib.qx_svd_batched.argtypes = [vp, vp, i64, i32, i32, vp, i32, i64, vp, vp, vp, i32, f64,
                                        vp]
+        lib.qx_spectral_modes.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp, vp, f64, f64,
+                                          f64, f64, f64, i32, vp, vp, vp]
         lib.qx_sancho_rubio.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, i32, f64,
                                         ctypes.POINTER(ctypes.c_int)]
         lib.qx_beyn_moments.argtypes = [vp, vp, i64, i32, i32, f64, f64, vp, vp, vp, vp, vp, i32]

@@ -30,7 +30,7 @@ import numpy as np
 import torch
 
 from quatrex_amd import _lib
-from quatrex_amd.ops import as_device, dagger, gemm, inv, mm
+from quatrex_amd.ops import as_device, dagger, gemm, inv, mm, spectral_modes
 
 
 class OBCSolver(ABC):
@@ -263,19 +263,6 @@ class Spectral(OBCSolver):
         sel = blocks[n:-n] if n > 0 else blocks
         return tuple(np.ascontiguousarray(b) for b in sel)
 
-    @staticmethod
-    def _compute_dE_dk(ws, vs, a_vs):
-        """spectral.py:160-196: dE/dk_m = -sum_n i n (v_m^H a_n v_m) w_m^n, with
-        the products a_n v precomputed (device tensors)."""
-        b = len(a_vs) // 2
-        out = torch.zeros_like(ws)
-        for av, n in zip(a_vs, range(-b, b + 1)):
-            if n == 0:
-                continue
-            quad = (vs.conj() * av).sum(-2)  # diag(V^H a_n V)
-            out = out - (1j * n) * quad * ws ** n
-        return out
-
     def _upscale_eigenmodes(self, ws, vs):
         """spectral.py:322-377 on device tensors."""
         if self.block_sections == 1:
@@ -292,24 +279,21 @@ class Spectral(OBCSolver):
         """spectral.py:198-320 (no injection vectors) on device tensors.
 
         The products a_n V run on the MFMA GEMM; residuals, group velocities
-        and the decay classification are elementwise on the device.
+        (spectral.py:160-196) and the decay classification are one fused
+        kernel over the eigenpairs (``ops.spectral_modes``).
         """
-        nb = len(a_xx) // 2
-        a_vs = [gemm(a.contiguous(), vs.contiguous()) for a in a_xx]
-        prod = sum(av * ws[..., None, :] ** (i - nb) for i, av in enumerate(a_vs))
-        res = torch.linalg.norm(prod, dim=-2) / torch.linalg.norm(vs, dim=-2)
-        if self.residual_normalization:
-            res = res / ws.abs()
-        dk = torch.nan_to_num(self._compute_dE_dk(ws, vs, a_vs), nan=0.0, posinf=0.0,
-                              neginf=0.0)
-        ks = torch.nan_to_num(-1j * torch.log(ws), nan=0.0, posinf=0.0, neginf=0.0)
-        eta = torch.finfo(torch.float64).eps
-        ratio = dk.real.abs() / (dk.imag.abs() + eta)
-        prop = (ks.imag.abs() < self.min_decay) & (self.min_propagation < ratio) & (dk.real < 0)
-        dec = ks.imag < -self.min_decay
-        dec = dec | ((self.min_propagation >= ratio) & (ks.imag < -self.eta_decay))
-        dec = dec & (ks.imag > -self.max_decay)
-        return (prop | dec) & (res < self.residual_tolerance)
+        if len(a_xx) != 3:
+            raise ValueError("the Spectral OBC takes the three blocks (a_ji, a_ii, a_ij)")
+        n = vs.shape[:-2]
+        vs3 = vs.reshape(-1, *vs.shape[-2:]).contiguous()
+        ws2 = ws.reshape(-1, ws.shape[-1]).contiguous()
+        a_vs = [gemm(a.contiguous(), vs.contiguous()).reshape(vs3.shape) for a in a_xx]
+        mask = spectral_modes(ws2, vs3, *a_vs, min_decay=self.min_decay,
+                              max_decay=self.max_decay, min_propagation=self.min_propagation,
+                              residual_tolerance=self.residual_tolerance,
+                              eta_decay=self.eta_decay,
+                              residual_normalization=self.residual_normalization)
+        return mask.reshape(*n, ws.shape[-1])
 
     @staticmethod
     def _x_ii_a_ij(ws, vs, mask):

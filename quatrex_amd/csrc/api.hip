@@ -544,6 +544,23 @@ int qx_svd_batched(qx_handle* h, void* stream, int64_t n, int rows, int cols, co
   });
 }
 
+int qx_spectral_modes(qx_handle* h, void* stream, int64_t n, int b, int m, const void* ws,
+                      const void* vs, const void* av_ji, const void* av_ii, const void* av_ij,
+                      double min_decay, double max_decay, double min_propagation,
+                      double residual_tolerance, double eta_decay, int residual_normalization,
+                      unsigned char* mask, double* residual, void* de_dk) {
+  return guarded(h, [&]() {
+    QX_REQUIRE(h, "null handle");
+    qx::spectral_modes(static_cast<hipStream_t>(stream), n, b, m,
+                       static_cast<const qx::z2*>(ws), static_cast<const qx::z2*>(vs),
+                       static_cast<const qx::z2*>(av_ji), static_cast<const qx::z2*>(av_ii),
+                       static_cast<const qx::z2*>(av_ij), min_decay, max_decay, min_propagation,
+                       residual_tolerance, eta_decay, residual_normalization != 0, mask, residual,
+                       static_cast<qx::z2*>(de_dk));
+    return QX_OK;
+  });
+}
+
 int qx_beyn_moments(qx_handle* h, void* stream, int64_t n, int b, int n_quad, double r_o,
                     double r_i, const void* a_ji, const void* a_ii, const void* a_ij, void* q0,
                     void* q1, int precision) {

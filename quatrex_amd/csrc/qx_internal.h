@@ -185,6 +185,14 @@ size_t svd_workspace_bytes(int64_t n, int d, int m);
 void svd_tall(hipStream_t s, int64_t n, int d, int m, const Mat& A, const Mat& U, double* S,
               int64_t sS, const Mat& V, int max_sweeps, double tol, void* ws, size_t ws_bytes,
               int* sweeps);
+// Spectral OBC mode classification (csrc/spectral.hip): mask[e, j] = 1 for
+// the reflected modes; res and dk (nullable) receive the residuals and
+// dE/dk.  ws (n, m); vs and the products a_ji V, a_ii V, a_ij V (n, b, m).
+void spectral_modes(hipStream_t s, int64_t n, int b, int m, const z2* ws, const z2* vs,
+                    const z2* am, const z2* a0, const z2* ap, double min_decay,
+                    double max_decay, double min_propagation, double residual_tolerance,
+                    double eta_decay, bool residual_normalization, unsigned char* mask,
+                    double* res, z2* dk);
 
 // X = inv(A - S) with partial pivoting (S may be zero).  info[e] is set to a
 // nonzero value for an exactly singular matrix.  n <= kInvMaxN.  The scratch

@@ -143,5 +143,31 @@ def svd(a, max_sweeps: int = 30, tol: float = 1e-15):
     return u, s, v
 
 
+def spectral_modes(ws, vs, av_ji, av_ii, av_ij, *, min_decay: float, max_decay: float,
+                   min_propagation: float, residual_tolerance: float, eta_decay: float,
+                   residual_normalization: bool = True, details: bool = False):
+    """Reflected-mode mask of the Spectral OBC from eigenpairs (ws (n, m), vs
+    (n, b, m)) and the products a_ji V, a_ii V, a_ij V, in one fused kernel
+    (spectral.py:160-320).  Returns the boolean mask (n, m); with
+    ``details`` also the residuals and dE/dk."""
+    ws, vs = as_device(ws).contiguous(), as_device(vs).contiguous()
+    avs = [as_device(a).contiguous() for a in (av_ji, av_ii, av_ij)]
+    n, b, m = vs.shape
+    if ws.shape != (n, m) or any(a.shape != vs.shape for a in avs):
+        raise ValueError("spectral_modes: ws must be (n, m), vs and the products (n, b, m)")
+    mask = torch.empty((n, m), dtype=torch.uint8, device=vs.device)
+    res = torch.empty((n, m), dtype=torch.float64, device=vs.device) if details else None
+    dk = torch.empty((n, m), dtype=torch.complex128, device=vs.device) if details else None
+    h, st = _lib.handle(vs.device), _lib.stream_ptr(vs.device)
+    rc = _lib.load().qx_spectral_modes(
+        h, st, n, b, m, ws.data_ptr(), vs.data_ptr(), avs[0].data_ptr(), avs[1].data_ptr(),
+        avs[2].data_ptr(), float(min_decay), float(max_decay), float(min_propagation),
+        float(residual_tolerance), float(eta_decay), int(bool(residual_normalization)),
+        mask.data_ptr(), res.data_ptr() if details else None, dk.data_ptr() if details else None)
+    _lib.check(rc, h, "spectral_modes")
+    mask = mask.bool()
+    return (mask, res, dk) if details else mask
+
+
 def dagger(x: torch.Tensor) -> torch.Tensor:
     return x.conj().transpose(-2, -1).resolve_conj().contiguous()
