@@ -660,7 +660,8 @@ def serve_workload(args):
         reqs = poisson_requests(list(works), rate, args.requests, (lo, hi), seed=args.seed + k,
                                 devices=works)
         st = srv.run(reqs, realtime=True)
-        log(f"serve rate {rate}/s: {st['energies_per_s']:.1f} energies/s, p50 "
+        log(f"serve rate {rate}/s: {st['energies_per_s']:.1f} energies/s "
+            f"({st['required_tflops']:.1f} TF/s required), p50 "
             f"{st['latency_ms']['p50']:.0f} ms, p99 {st['latency_ms']['p99']:.0f} ms")
         per_rate.append(dict(rate_per_s=rate, **{k2: (round(v, 3) if isinstance(v, float) else v)
                                                  for k2, v in st.items()}))

@@ -24,7 +24,8 @@ Execution model (MI355X-first):
   graph; ``replay()`` launches the lanes' graphs on their streams (one
   launch per lane), so the thousands of small kernel launches of small
   devices cost no host time;
-- ``autotune`` picks, per block size, the GEMM tile shape (64 or 128) and
+- ``autotune`` picks, per block size, the GEMM kernel (the register-direct
+  default or an LDS-staged 64 x 64 or 128 x 128 tile) and
   the energies per call by timing probe solves; its table is cached.
 
 The per-device computation is the reference's RGF.selected_solve
@@ -487,7 +488,10 @@ class UnevenBatch:
 # --------------------------------------------------------------------------
 # autotuner
 
-TILES = (64, 128)  # the first is the default
+# GEMM kernel choices (qx_set_tuning codes): 0 = the library default
+# (register-direct, tile shape by product), 64 / 128 = the LDS-staged
+# 64 x 64 / 128 x 128 tiles; the first is the default
+TILES = (0, 64, 128)
 CHUNKS = (8, 16, 32, 64, 128, 256)
 # A tile other than the default must beat it by this fraction in the probe:
 # the probe runs one lane alone, the batch runs several side by side, and a
@@ -522,7 +526,7 @@ def autotune(block_sizes, device, precision: str = "c128", n_blocks: int = 6,
             table = json.loads(path.read_text())
         except (OSError, ValueError):
             table = {}
-    key = f"{gpu}|{precision}"
+    key = f"{gpu}|{precision}|tiles={','.join(map(str, TILES))}"
     known = table.get(key, {})
     out = {}
     lane = None

@@ -226,8 +226,12 @@ class Server:
         makespan = max(r.t_done for r in requests)
         lat = np.array([r.latency for r in requests]) * 1e3
         n_e = sum(len(r.energies) for r in requests)
+        # request costs span orders of magnitude (b = 64 to 416, 16 to 128
+        # blocks), so the required flops per second measure throughput too
+        flops = sum(len(r.energies) * self.devices[r.key].flops_per_energy() for r in requests)
         return {"requests": len(requests), "energies": n_e,
                 "energies_per_s": n_e / makespan, "makespan_s": makespan,
+                "required_tflops": flops / makespan / 1e12,
                 "latency_ms": {"p50": float(np.percentile(lat, 50)),
                                "p99": float(np.percentile(lat, 99)),
                                "max": float(lat.max())}}
