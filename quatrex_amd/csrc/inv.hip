@@ -510,12 +510,11 @@ __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
   int* idx = sig + n;
 
   T xr[RPL][CW], xi[RPL][CW];
-  int pos[RPL], step[RPL];  // position; panel column where the row was pivot (-1: none)
+  int pos[RPL];  // where row lane + 64 j currently stands
 #pragma unroll
   for (int j = 0; j < RPL; ++j) {
     const int r = lane + 64 * j;
     pos[j] = r;
-    step[j] = -1;
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
       // unconditional loads from a clamped address, so that all of them are
@@ -529,7 +528,9 @@ __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
   }
   // Pivot data of panel column cn (register cn of the owner wave) into
   // buffer b: the column, the key of the candidates (rows at position > kb)
-  // and the pivot value.
+  // and the pivot value; the pivot row's entry of the published column (its
+  // multiplier in the elimination) is then zeroed, as the deferred scaling
+  // requires.
   auto publish = [&](int cn, int b, int kb) {
     unsigned key = 0;
 #pragma unroll
@@ -549,6 +550,8 @@ __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
       if (lane + 64 * j == pn) {
         piv_r[b] = xr[j][cn];
         piv_i[b] = xi[j][cn];
+        col_r[b][pn] = (T)0;
+        col_i[b][pn] = (T)0;
       }
     }
     if (lane == 0) key_b[b] = key;
@@ -574,9 +577,8 @@ __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
         T fr[RPL], fi[RPL];
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
-          const bool piv = (lane + 64 * j == p);
-          fr[j] = piv ? (T)0 : col_r[buf][lane + 64 * j];
-          fi[j] = piv ? (T)0 : col_i[buf][lane + 64 * j];
+          fr[j] = col_r[buf][lane + 64 * j];  // zero for the pivot row
+          fi[j] = col_i[buf][lane + 64 * j];
         }
         // this wave's part of row p and its position, through LDS
 #pragma unroll
@@ -629,7 +631,6 @@ __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
           const int r = lane + 64 * j;
-          step[j] = (r == p) ? kk : step[j];
           pos[j] = (r == p) ? k : ((pos[j] == k) ? q : pos[j]);
         }
         if (tid == 0) {
@@ -651,10 +652,14 @@ __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
   for (int j = 0; j < RPL; ++j) {
     const int r = lane + 64 * j;
     if (r < n) {
+      // the pivot of panel column kk ends at position k0 + kk, and those
+      // positions hold exactly this panel's pivots
+      const int st = pos[j] - k0;
+      const bool pivot = st >= 0 && st < nbw;
       double ar = 1.0, ai = 0.0;
-      if (step[j] >= 0) {
-        ar = rcp_r[step[j]];
-        ai = rcp_i[step[j]];
+      if (pivot) {
+        ar = rcp_r[st];
+        ai = rcp_i[st];
       }
 #pragma unroll
       for (int c = 0; c < CW; ++c) {
@@ -662,7 +667,7 @@ __global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
         if (cc < nbw) {
           const double a = (double)xr[j][c], b = (double)xi[j][c];
           double yr = a, yi = b;
-          if (step[j] >= 0) {
+          if (pivot) {
             yr = (double)(T)(a * ar - b * ai);
             yi = (double)(T)(a * ai + b * ar);
           }

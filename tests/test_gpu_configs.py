@@ -139,6 +139,27 @@ def test_beyn_moments_lead_layer_256(device, lead256, r_o):
     assert rel_err(q1.cpu().numpy(), q1r) < 1e-10
 
 
+@pytest.mark.parametrize("r_o", [1.5, 3.0])
+def test_spectral_beyn_lead_layer_256(device, r_o):
+    """The Spectral OBC on Beyn at lead layer 256 (device moments, device SVD
+    projection, fused mode classification) against the oracle's Spectral
+    OBC on Beyn, both with fresh probe streams, on a lead whose in-annulus
+    eigenvalue count (~20 at r_o = 1.5, ~110 at 3) the projection resolves;
+    m_0 = count + 8 as in the benchmark.  Measured 3e-13 and 1.4e-13."""
+    from quatrex_amd.boundary_conditions import Spectral
+
+    m_ji, m_ii, m_ij, _ = lead_blocks(seed=1, n_energies=2, block_size=256, onsite_scale=8.0)
+    w, _ = onevp.Full()((m_ji, m_ii, m_ij))
+    mag = np.abs(w)
+    m0 = int(((mag > 1 / r_o) & (mag < r_o)).sum(-1).max()) + 8
+    want = oobc.Spectral(onevp.Beyn(r_o=r_o, r_i=1 / r_o, m_0=m0, num_quad_points=64))(
+        m_ii, m_ij, m_ji)
+    T = lambda x: torch.as_tensor(x, device=device)
+    got = Spectral(Beyn(r_o=r_o, r_i=1 / r_o, m_0=m0, num_quad_points=64))(
+        T(m_ii), T(m_ij), T(m_ji)).cpu().numpy()
+    assert rel_err(got, want) < 1e-10
+
+
 # one device per block size of the uneven mix: (b, N, energies, seed)
 MIX = [(64, 4, 2, 31), (128, 3, 2, 32), (256, 3, 2, 33), (416, 3, 2, 34)]
 
