@@ -41,8 +41,9 @@ extern "C" {
 /* arithmetic selector */
 #define QX_PREC_C128 0       /* complex128 on FP64 MFMA                      */
 #define QX_PREC_C64 1        /* complex64 on FP32 MFMA, FP64 in/out          */
-#define QX_PREC_MIXED 2      /* RGF only: complex64 products and inverses,   */
-                             /* one FP64 Newton-Schulz step per block inverse */
+#define QX_PREC_MIXED 2      /* RGF only: complex64 products, FP64 block     */
+                             /* inverses (env QX_MIXED_INVERSE=c64: complex64 */
+                             /* inverses with one FP64 Newton-Schulz step)    */
 #define QX_PREC_C128_3M 3    /* complex128 on FP64 MFMA with the 3M product   */
                              /* split (3/4 of the matrix-core work, rounding  */
                              /* error a small factor above the 4M split)      */
@@ -110,9 +111,10 @@ int qx_timer_stop(qx_handle* handle, void* stream, double* stats);
 
 /* Wait for `stream`, then report (and clear) device-side errors recorded
  * since the last check: QX_ESINGULAR if an inversion met a zero pivot, else
- * QX_FP64_FALLBACK if a QX_PREC_MIXED call replaced a complex64 block inverse
- * whose Newton-Schulz residual ||I - M X0||_F was not below the tolerance
- * (env QX_MIXED_TOL, default 1e-2) by the FP64 pivoted inverse. */
+ * QX_FP64_FALLBACK if a QX_PREC_MIXED call with complex64 block inverses
+ * (QX_MIXED_INVERSE=c64) replaced one whose Newton-Schulz residual
+ * ||I - M X0||_F was not below the tolerance (env QX_MIXED_TOL, default 1e-2)
+ * by the FP64 pivoted inverse. */
 int qx_sync_check(qx_handle* handle, void* stream);
 
 /* RGF selected inversion.  Replaces RGF.selected_inv

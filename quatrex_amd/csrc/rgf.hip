@@ -129,9 +129,20 @@ struct OzSubBatch {
 // rounding) is inverted again by the FP64 pivoted inverse, selected per
 // matrix on the device through `mask`, and flagged with info bit 1
 // (reported by qx_sync_check as QX_FP64_FALLBACK).
+// Block inverses of the mixed arithmetic: the FP64 inverse (default), or,
+// with QX_MIXED_INVERSE=c64, the complex64 inverse refined by one FP64
+// Newton-Schulz step with the FP64 re-inversion of flagged blocks.  The
+// inverse kernels are bound by their pivot chains, not by arithmetic, so the
+// complex64 inverse is barely faster than the FP64 one (523 vs 584 us for
+// 32 x 256^2) and the refinement costs more than it saves.
+bool mixed_inverse_c64() {
+  const char* e = std::getenv("QX_MIXED_INVERSE");
+  return e && std::string(e) == "c64";
+}
+
 void block_inv(hipStream_t st, int64_t S, int n, const Mat& Ain, const Mat& X, int* info,
                int prec, const InvScratch& scr, const Mat& Mc, const Mat& R, int* mask) {
-  if (prec != QX_PREC_MIXED) {
+  if (prec != QX_PREC_MIXED || !mixed_inverse_c64()) {
     zinv(st, S, n, Ain, Mat(), X, info, prec == QX_PREC_C64 ? QX_PREC_C64 : QX_PREC_C128, scr);
     return;
   }

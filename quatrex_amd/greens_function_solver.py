@@ -194,8 +194,9 @@ class RGF(GFSolver):
         Arithmetic of the block products and inverses: complex128 on FP64
         matrix cores (4M complex split; "c128-3m": 3M split, a quarter less
         matrix-core work, rounding error a small factor larger); complex64
-        on FP32 matrix cores; "mixed", complex64 products and inverses
-        with one FP64 Newton-Schulz step on every block inverse; or
+        on FP32 matrix cores; "mixed", complex64 products and FP64 block
+        inverses (with QX_MIXED_INVERSE=c64: complex64 inverses refined by
+        one FP64 Newton-Schulz step, FP64 re-inversion of failed blocks); or
         "oz2-n", block products on int8 matrix cores by modular arithmetic
         with n moduli (relative error ~2^-T per product of the line maxima,
         T = 42 for n = 12 at blocks <= 256) and FP64 block inverses; or

@@ -1,5 +1,6 @@
-"""FP64 safety net of the mixed arithmetic (complex64 block inverses with one
-Newton-Schulz step).
+"""FP64 safety net of the mixed arithmetic's optional complex64 block
+inverses (QX_MIXED_INVERSE=c64: complex64 inverse with one FP64
+Newton-Schulz step; by default the mixed mode inverts in FP64).
 
 I - M X1 = (I - M X0)^2 for the Newton-Schulz iterate, so a complex64
 inverse X0 whose residual ||I - M X0||_F is not below the tolerance (1e-2)
@@ -33,6 +34,11 @@ from quatrex_amd.greens_function_solver import RGF
 from tests.conftest import rel_err
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def c64_inverses(monkeypatch):
+    monkeypatch.setenv("QX_MIXED_INVERSE", "c64")
 
 
 def _ill(rng, n, cond):

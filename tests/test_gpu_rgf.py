@@ -335,10 +335,14 @@ def test_workload_mixed_accuracy(device, workload):
     assert rel_err(dens, dens_ref) < 1e-5
 
 
-def test_mixed_newton_schulz_single_block(device):
-    """With one block the mixed mode is one complex64 inverse plus one FP64
-    Newton-Schulz step: the result must be FP64-accurate, where plain complex64
+@pytest.mark.parametrize("inverse", ["fp64", "c64"])
+def test_mixed_newton_schulz_single_block(device, monkeypatch, inverse):
+    """With one block the mixed mode is one block inverse: in FP64 by
+    default, or (QX_MIXED_INVERSE=c64) one complex64 inverse plus one FP64
+    Newton-Schulz step.  Either must be FP64-accurate, where plain complex64
     stays at FP32 accuracy."""
+    if inverse == "c64":
+        monkeypatch.setenv("QX_MIXED_INVERSE", "c64")
     rng = np.random.default_rng(3)
     n, ne = 96, 6
     m = (rng.standard_normal((ne, n, n)) + 1j * rng.standard_normal((ne, n, n))) / np.sqrt(n)
