@@ -6,16 +6,17 @@
 // Algorithm: blocked in-place Gauss-Jordan elimination split into kernels so
 // that the O(n^3) part runs on every CU of the chip while the sequential
 // pivoting chain stays short:
-//   for each panel of NB columns (NB = 32 for n <= 256, 16 up to n = 512)
-//     panel kernel  (one workgroup per matrix): for n <= 256 four threads
-//        hold one row of the 32-column panel (8 columns each, inv_panel4);
-//        above, each thread holds one or two rows (inv_panel); per column the pivot (first
-//        row of maximal |re| + |im| among the unreduced rows, compared through
-//        a 32-bit key: float magnitude with 9 low mantissa bits replaced by the
-//        row index) is found with wave shuffles plus one LDS exchange, the
-//        pivot row is scaled and broadcast through LDS, and every other row
-//        eliminates the pivot column -- two barriers per column.  The reduced
-//        panel, the pivot rows and the composed row gather sig are written out.
+//   for each panel of NB = 32 columns
+//     panel kernel  (one workgroup per matrix): by default inv_panel_cg, in
+//        which each wave holds a group of the panel's columns for all rows
+//        and the rows never move (one barrier per column; see the kernel);
+//        the earlier row layouts stay selectable (QX_INV_PANEL=4: four
+//        threads per row for n <= 256, inv_panel4; QX_INV_PANEL=1: one or two
+//        rows per thread, 16-column panels above n = 256, inv_panel).  The
+//        pivot of a column is the first row of maximal |re| + |im| among the
+//        unreduced rows, compared through a 32-bit key (float magnitude with
+//        9 low mantissa bits replaced by the row index).  The reduced panel,
+//        the pivot rows and the composed row gather sig are written out.
 //     update kernel (64 x 64 tiles of every matrix, MFMA): the other columns
 //        receive the row interchanges and the rank-NB update in one pass,
 //          Xn[r, C] = (r in pivot rows ? 0 : Xc[sig r, C]) + P[r, :] Xc[sig K, C],
