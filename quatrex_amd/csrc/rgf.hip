@@ -187,10 +187,10 @@ size_t rgf_workspace_bytes(const qx_btd_layout* la, bool solve, int prec) {
   // Ozaki-II products: residues of one product per matrix of the call
   const size_t per = oz2_bytes_per_matrix(prec, bmax);
   bytes += (size_t)la->n_stack * per + (per ? 512 : 0);
-  // the fused small-block kernels keep 6 N + 16 (solve) or 2 N + 3 (inverse)
+  // the fused small-block kernels keep 6 N + 23 (solve) or 2 N + 3 (inverse)
   // slots of 32 x 32 per energy
   if (la->n_blocks <= 16 && bmax <= 32) {
-    const int64_t fs = solve ? 6 * (int64_t)la->n_blocks + 16 : 2 * (int64_t)la->n_blocks + 3;
+    const int64_t fs = solve ? 6 * (int64_t)la->n_blocks + 23 : 2 * (int64_t)la->n_blocks + 3;
     bytes = std::max(bytes, (size_t)(fs * la->n_stack * 1024) * sizeof(z2));
   }
   return bytes;

@@ -200,6 +200,70 @@ def test_deep_recursion_with_obc_blocks(device):
     assert _per_block_worst(w, xi, orgf.RGF(2).selected_inv(oa, obc_blocks=oobc)) < 1e-10
 
 
+@pytest.mark.parametrize("precision", ["c128", "c128-3m"])
+def test_small_blocks_every_output(device, precision):
+    """Blocks of mixed sizes up to 32 (the fused kernel's range, on the fused
+    path), Σ^< with off-diagonal blocks and Σ^> block-diagonal (only one of
+    the two chains has the Σ_ij terms), boundary blocks that differ between
+    the chains (retarded at both ends, lesser at the first block, greater at
+    the last), the current, the off-diagonal G^R blocks, the corner
+    G^R_{0,N-1} and the density, against the oracle."""
+    bs = np.array([8, 32, 16, 24, 32, 12, 32])
+    n, ne = len(bs), 5
+    off = np.hstack(([0], np.cumsum(bs)))
+    rng = np.random.default_rng(17)
+    oa = ODSDBCOO.from_sparray(sps.coo_matrix(bt_dense(bs, seed=21)), bs, (ne,))
+    oa.data = oa.data * (1.0 + 0.05 * np.arange(ne))[:, None]
+    s = bt_dense(bs, seed=22)
+    d = bt_dense(bs, seed=23)
+    mask = np.zeros(d.shape, dtype=bool)
+    for i in range(n):
+        mask[off[i]:off[i + 1], off[i]:off[i + 1]] = True
+    d = np.where(mask, d, 0)
+    osl = ODSDBCOO.from_sparray(sps.coo_matrix(s - s.conj().T), bs, (ne,))
+    osg = ODSDBCOO.from_sparray(sps.coo_matrix(d - d.conj().T), bs, (ne,))
+    assert osg.data.shape[-1] < osl.data.shape[-1]  # no off-diagonal Σ^> blocks
+
+    def block(b, scale=0.05):
+        return scale * (rng.standard_normal((ne, b, b)) + 1j * rng.standard_normal((ne, b, b)))
+    g0, g1 = block(bs[0]), block(bs[-1])
+    obc = {"retarded": {0: block(bs[0]), n - 1: block(bs[-1])},
+           "lesser": {0: 1j * (g0 @ g0.conj().transpose(0, 2, 1))},
+           "greater": {n - 1: -1j * (g1 @ g1.conj().transpose(0, 2, 1))}}
+    oobc, gobc = orgf.OBCBlocks(n), OBCBlocks(n)
+    for kind, blocks in obc.items():
+        for i, v in blocks.items():
+            getattr(oobc, kind)[i] = v
+            getattr(gobc, kind)[i] = torch.as_tensor(v, device=device)
+    ocorner = np.zeros((ne, bs[0], bs[-1]), dtype=np.complex128)
+    rl, rg, rr, rcur = orgf.RGF(ne).selected_solve(
+        oa, osl, osg, obc_blocks=oobc, return_retarded=True, return_current=True,
+        retarded_offdiag=True, corner=ocorner)
+    corner = torch.zeros((ne, bs[0], bs[-1]), dtype=torch.complex128, device=device)
+    dens = torch.zeros((ne, int(off[-1])), dtype=torch.float64, device=device)
+    a, sl, sg = (to_gpu(o, device) for o in (oa, osl, osg))
+    xl, xg, xr, cur = RGF(ne, precision=precision, retarded_offdiag=True).selected_solve(
+        a, sl, sg, obc_blocks=gobc, return_retarded=True, return_current=True, corner=corner,
+        density=dens)
+    tol = 1e-10 if precision == "c128" else 1e-9
+    for g, o in ((xl, rl), (xg, rg), (xr, rr)):
+        for i in range(n):
+            for j in (i - 1, i, i + 1):
+                if 0 <= j < n:
+                    assert rel_err(g.blocks[i, j].cpu().numpy(), o.blocks[i, j]) < tol, (i, j)
+    assert rel_err(cur.cpu().numpy(), rcur) < tol
+    assert rel_err(corner.cpu().numpy(), ocorner) < tol
+    from oracle.observables import density as o_density
+    assert rel_err(dens.cpu().numpy(), o_density(rl)) < tol
+    # the retarded inverse with the same boundary blocks
+    xi = RGF(ne, precision=precision).selected_inv(a, obc_blocks=gobc)
+    ri = orgf.RGF(ne).selected_inv(oa, obc_blocks=oobc)
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                assert rel_err(xi.blocks[i, j].cpu().numpy(), ri.blocks[i, j]) < tol, (i, j)
+
+
 def test_sub_batch_offset_changes_only_timing(device, monkeypatch):
     """The odd sub-batches start after sub-batch 0's first block inverse
     (QX_RGF_STAGGER); the energies are independent, so the outputs are
