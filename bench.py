@@ -175,11 +175,12 @@ def measured_traffic(flops_per_launch: float):
 
 
 def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak, dominant_flops,
-              required_step_flops):
+              required_step_flops, fused=False):
     """Warm-up + timed steps of one arithmetic mode; returns its record.
     dominant_flops: flops of one launch of the dominant GEMM shape (one
     sub-batch of b x b x b products), the shape the traffic was measured at;
-    required_step_flops: flops one step requires (all energies)."""
+    required_step_flops: flops one step requires (all energies); fused: the
+    calls run as one fused small-block kernel launch each."""
     import numpy as np
     import torch
 
@@ -248,7 +249,8 @@ def timed_leg(rgf, views, out, steps, warmup, dev, timer_on, peak, dominant_flop
             "required_flops_per_step": required_step_flops,
             "traffic_source": step_traffic["source"] if step_traffic else None,
             "dominant_kernel": {
-                "name": "zgemm_reg_kernel (batched complex GEMM, register-direct)",
+                "name": ("rgf_small_kernel (fused small-block solve, one workgroup per energy)"
+                         if fused else "zgemm_reg_kernel (batched complex GEMM, register-direct)"),
                 "launches": g["launches"], "flops_per_launch": per_launch,
                 "avg_launch_ms": round(avg_ms, 4),
                 "per_launch_tflops": round(pl_tflops, 3),
@@ -815,8 +817,11 @@ def main():
         primary = m == args.precision
         steps = args.steps if primary else max(1, min(args.steps, args.compare_steps))
         warm = args.warmup if primary else min(args.warmup, 1)
+        # blocks <= 32, <= 16 of them: one fused kernel launch per call
+        fused = (N <= 16 and b <= 32 and m in ("c128", "c128-3m")
+                 and os.environ.get("QX_RGF_FUSED", "1") != "0")
         legs[m] = timed_leg(rgf, views, (xl, xg, xr), steps, warm, dev,
-                            not args.no_timer, peak, dominant, E * required_flops(N, b))
+                            not args.no_timer, peak, dominant, E * required_flops(N, b), fused)
         legs[m]["value"] = E * steps / legs[m]["elapsed"]
         legs[m]["steps"] = steps
         log(f"{m}: {legs[m]['value']:.1f} energies/s")

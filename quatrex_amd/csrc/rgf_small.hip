@@ -1124,12 +1124,20 @@ bool rgf_small_solve(qx_handle* h, hipStream_t st, const qx_btd_layout* la, cons
     hipLaunchKernelGGL(rgf_small_kernel<4>, dim3((unsigned)S), dim3(kSolveThreads), 0, st, g);
   QX_HIP_CHECK(hipGetLastError());
   if (timer) {
-    // algorithmic GEMM flops of the sweep (8 b^3 per product, products with
-    // stored operands only): counted as GEMM work
-    double fl = 0.0;
+    // matrix flops of the products and inverses the kernel runs when no
+    // optional term is present (no current, corner, Σ off-diagonal blocks or
+    // off-diagonal G^R): 8 m n k per product, 8 n^3 per inverse; all counted
+    // as GEMM work of this one launch
+    auto cube = [](double m, double n, double k) { return 8.0 * m * n * k; };
+    const double b0 = la->block_sizes[0];
+    double fl = 4.0 * cube(b0, b0, b0) + cube(b0, b0, b0);
     for (int i = 0; i + 1 < N; ++i) {
-      const double b = la->block_sizes[i + 1];
-      fl += 23.0 * 8.0 * b * b * b;
+      const double bi = la->block_sizes[i], bj = la->block_sizes[i + 1];
+      fl += 3.0 * cube(bj, bi, bi) + 3.0 * cube(bj, bj, bi) + 4.0 * cube(bj, bj, bj)  // forward
+            + cube(bj, bj, bj)                                                          // inverse
+            + cube(bi, bj, bi) + cube(bj, bi, bj) + cube(bi, bj, bj) + cube(bi, bi, bj)  // b1..b4
+            + 3.0 * cube(bi, bi, bi) + 2.0 * cube(bi, bj, bj) + 2.0 * cube(bi, bi, bj) +
+            2.0 * cube(bi, bj, bi);
     }
     timer->end(st, ev0, KernelTimer::GEMM, fl * (double)S);
   }
@@ -1176,10 +1184,16 @@ bool rgf_small_inv(qx_handle* h, hipStream_t st, const qx_btd_layout* la, const 
     hipLaunchKernelGGL(rgf_small_inv_kernel<4>, dim3((unsigned)S), dim3(kSolveThreads), 0, st, g);
   QX_HIP_CHECK(hipGetLastError());
   if (timer) {
-    double fl = 0.0;
+    // matrix flops of its products and inverses (8 m n k, 8 n^3), counted as
+    // GEMM work of this one launch
+    auto cube = [](double m, double n, double k) { return 8.0 * m * n * k; };
+    const double b0 = la->block_sizes[0];
+    double fl = cube(b0, b0, b0);
     for (int i = 0; i + 1 < N; ++i) {
-      const double b = la->block_sizes[i + 1];
-      fl += 7.0 * 8.0 * b * b * b;
+      const double bi = la->block_sizes[i], bj = la->block_sizes[i + 1];
+      fl += cube(bj, bi, bi) + cube(bj, bj, bi) + cube(bj, bj, bj)  // forward + inverse
+            + cube(bj, bi, bj) + cube(bj, bi, bi) + cube(bi, bj, bi) + cube(bi, bj, bj) +
+            cube(bi, bi, bj);                                          // backward
     }
     timer->end(st, ev0, KernelTimer::GEMM, fl * (double)S);
   }
