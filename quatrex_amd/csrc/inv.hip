@@ -463,8 +463,8 @@ __device__ inline unsigned wave_max_bcast(unsigned v) {
   return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// Panel kernel with column groups (n <= 64 RPL, 32-column panels): wave w of
-// the 32 / CW waves holds columns CW w .. CW w + CW - 1 of the panel for all
+// Panel kernel with column groups (n <= 64 RPL, NB-column panels): wave w of
+// the NB / CW waves holds columns CW w .. CW w + CW - 1 of the panel for all
 // rows, lane l the rows l + 64 j (j < RPL).  Rows stay in their registers;
 // the row interchanges of the factorisation are tracked as positions (pos_j:
 // where row l + 64 j currently stands), so the candidates of a column are the
@@ -490,9 +490,9 @@ __device__ inline unsigned wave_max_bcast(unsigned v) {
 // Pivot choice (largest |re| + |im| through the 32-bit key, ties to the lower
 // row), arithmetic and outputs are those of the other panel kernels up to
 // rounding; ties are broken by the row's index at the start of the panel.
-template <typename T, int RPL, int CW>
-__global__ __launch_bounds__(64 * (32 / CW)) void inv_panel_cg(PanelArgs g) {
-  constexpr int NB = 32, NW = NB / CW, NTH = 64 * NW, NR = 64 * RPL;
+template <typename T, int RPL, int CW, int NB>
+__global__ __launch_bounds__(64 * (NB / CW)) void inv_panel_cg(PanelArgs g) {
+  constexpr int NW = NB / CW, NTH = 64 * NW, NR = 64 * RPL;
   __shared__ T col_r[2][NR], col_i[2][NR];
   __shared__ T piv_r[2], piv_i[2];
   __shared__ unsigned key_b[2];
@@ -695,12 +695,17 @@ struct UpdArgs {
 
 // Xn[r, c] = (r in K ? 0 : Xc[sig r, c]) + sum_k Xn[r, k0+k] Xc[sig(k0+k), c]
 // for the columns c outside the panel; 64 x 64 tiles, 4 waves of 32 x 32.
+// The rank-NB sum runs over k-chunks of KC = 32 staged through LDS one after
+// another (NB = 64: two), so a wider panel costs no more LDS per workgroup.
 template <typename T, int NB>
 __global__ __launch_bounds__(NT) void inv_update(UpdArgs g) {
   typedef InvMma<T> MM;
   typedef typename MM::acc_t acc_t;
-  __shared__ T As[2][NB][64 + 1];
-  __shared__ T Bs[2][NB][64 + 1];
+  constexpr int KC = 32;
+  static_assert(NB % KC == 0 || NB < KC, "panel width: a multiple of 32, or below 32");
+  constexpr int KW = NB < KC ? NB : KC;  // staged k per chunk
+  __shared__ T As[2][KW][64 + 1];
+  __shared__ T Bs[2][KW][64 + 1];
   __shared__ int sig_r[64], sig_k[NB];
   const int tiles = g.tiles;  // per matrix, row-major over (row tile, col tile)
   const int tn = (g.n + 63) / 64;
@@ -738,59 +743,63 @@ __global__ __launch_bounds__(NT) void inv_update(UpdArgs g) {
         aR[i][j][q] = ok ? (T)v.x : (T)0;
         aI[i][j][q] = ok ? (T)v.y : (T)0;
       }
-  // A = reduced panel rows of this tile, B = gathered pivot rows: all loads
-  // first (unconditional, clamped addresses), then the LDS writes
-  constexpr int NS = 64 * NB / NT;
-  z2 va[NS], vb[NS];
+  // per chunk: A = reduced panel rows of this tile, B = gathered pivot rows;
+  // all loads first (unconditional, clamped addresses), then the LDS writes
+  constexpr int NS = 64 * KW / NT;
 #pragma unroll
-  for (int it = 0; it < NS; ++it) {
-    const int q = tid + it * NT;
-    const int r = q / NB, k = q - r * NB;
-    const bool oka = r0 + r < n && k < nbw;
-    va[it] = Xn[oka ? (int64_t)(r0 + r) * g.ldn + k0 + k : 0];
-    if (!oka) va[it] = make_double2(0.0, 0.0);
-    const int k2 = q >> 6, c = q & 63;
-    const bool okb = k2 < nbw && c0 + c < n;
-    vb[it] = Xc[okb ? (int64_t)sig_k[k2] * g.ldc + c0 + c : 0];
-    if (!okb) vb[it] = make_double2(0.0, 0.0);
-  }
+  for (int kc = 0; kc < NB; kc += KW) {
+    if (kc > 0) __syncthreads();  // the previous chunk's LDS reads are done
+    z2 va[NS], vb[NS];
 #pragma unroll
-  for (int it = 0; it < NS; ++it) {
-    const int q = tid + it * NT;
-    const int r = q / NB, k = q - r * NB;
-    As[0][k][r] = (T)va[it].x;
-    As[1][k][r] = (T)va[it].y;
-    const int k2 = q >> 6, c = q & 63;
-    Bs[0][k2][c] = (T)vb[it].x;
-    Bs[1][k2][c] = (T)vb[it].y;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int kq = 0; kq < NB / 4; ++kq) {
-    const int kr = kq * 4 + (lane >> 4);
-    T a_r[2], a_i[2], na_i[2], b_r[2], b_i[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      a_r[i] = As[0][kr][wm * 32 + i * 16 + (lane & 15)];
-      a_i[i] = As[1][kr][wm * 32 + i * 16 + (lane & 15)];
-      na_i[i] = -a_i[i];
-      b_r[i] = Bs[0][kr][wn * 32 + i * 16 + (lane & 15)];
-      b_i[i] = Bs[1][kr][wn * 32 + i * 16 + (lane & 15)];
+    for (int it = 0; it < NS; ++it) {
+      const int q = tid + it * NT;
+      const int r = q / KW, k = q - r * KW;
+      const bool oka = r0 + r < n && kc + k < nbw;
+      va[it] = Xn[oka ? (int64_t)(r0 + r) * g.ldn + k0 + kc + k : 0];
+      if (!oka) va[it] = make_double2(0.0, 0.0);
+      const int k2 = q >> 6, c = q & 63;
+      const bool okb = kc + k2 < nbw && c0 + c < n;
+      vb[it] = Xc[okb ? (int64_t)sig_k[kc + k2] * g.ldc + c0 + c : 0];
+      if (!okb) vb[it] = make_double2(0.0, 0.0);
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int it = 0; it < NS; ++it) {
+      const int q = tid + it * NT;
+      const int r = q / KW, k = q - r * KW;
+      As[0][k][r] = (T)va[it].x;
+      As[1][k][r] = (T)va[it].y;
+      const int k2 = q >> 6, c = q & 63;
+      Bs[0][k2][c] = (T)vb[it].x;
+      Bs[1][k2][c] = (T)vb[it].y;
+    }
+    __syncthreads();
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        aR[i][j] = MM::mma(a_r[i], b_r[j], aR[i][j]);
-        aI[i][j] = MM::mma(a_r[i], b_i[j], aI[i][j]);
+    for (int kq = 0; kq < KW / 4; ++kq) {
+      const int kr = kq * 4 + (lane >> 4);
+      T a_r[2], a_i[2], na_i[2], b_r[2], b_i[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a_r[i] = As[0][kr][wm * 32 + i * 16 + (lane & 15)];
+        a_i[i] = As[1][kr][wm * 32 + i * 16 + (lane & 15)];
+        na_i[i] = -a_i[i];
+        b_r[i] = Bs[0][kr][wn * 32 + i * 16 + (lane & 15)];
+        b_i[i] = Bs[1][kr][wn * 32 + i * 16 + (lane & 15)];
       }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        aR[i][j] = MM::mma(na_i[i], b_i[j], aR[i][j]);
-        aI[i][j] = MM::mma(a_i[i], b_r[j], aI[i][j]);
-      }
+        for (int j = 0; j < 2; ++j) {
+          aR[i][j] = MM::mma(a_r[i], b_r[j], aR[i][j]);
+          aI[i][j] = MM::mma(a_r[i], b_i[j], aI[i][j]);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          aR[i][j] = MM::mma(na_i[i], b_i[j], aR[i][j]);
+          aI[i][j] = MM::mma(a_i[i], b_r[j], aI[i][j]);
+        }
+    }
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -857,15 +866,21 @@ int inv_group_width() {
   return (e && std::atoi(e) == 8) ? 8 : 4;
 }
 
-template <typename T>
+template <typename T, int NB>
 void launch_panel_cg(hipStream_t s, int64_t batch, int n, const PanelArgs& pa) {
-  const bool narrow = inv_group_width() == 4;
-  if (n <= 256) {
-    if (narrow) hipLaunchKernelGGL((inv_panel_cg<T, 4, 4>), dim3((unsigned)batch), dim3(512), 0, s, pa);
-    else hipLaunchKernelGGL((inv_panel_cg<T, 4, 8>), dim3((unsigned)batch), dim3(256), 0, s, pa);
+  if constexpr (NB == 64) {
+    // eight waves of eight columns, four rows per lane (n <= 256)
+    hipLaunchKernelGGL((inv_panel_cg<T, 4, 8, 64>), dim3((unsigned)batch), dim3(512), 0, s, pa);
+    return;
   } else {
-    if (narrow) hipLaunchKernelGGL((inv_panel_cg<T, 8, 4>), dim3((unsigned)batch), dim3(512), 0, s, pa);
-    else hipLaunchKernelGGL((inv_panel_cg<T, 8, 8>), dim3((unsigned)batch), dim3(256), 0, s, pa);
+    const bool narrow = inv_group_width() == 4;
+    if (n <= 256) {
+      if (narrow) hipLaunchKernelGGL((inv_panel_cg<T, 4, 4, 32>), dim3((unsigned)batch), dim3(512), 0, s, pa);
+      else hipLaunchKernelGGL((inv_panel_cg<T, 4, 8, 32>), dim3((unsigned)batch), dim3(256), 0, s, pa);
+    } else {
+      if (narrow) hipLaunchKernelGGL((inv_panel_cg<T, 8, 4, 32>), dim3((unsigned)batch), dim3(512), 0, s, pa);
+      else hipLaunchKernelGGL((inv_panel_cg<T, 8, 8, 32>), dim3((unsigned)batch), dim3(256), 0, s, pa);
+    }
   }
 }
 
@@ -881,12 +896,16 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
     const int k0 = p * NB, nbw = min(NB, n - k0), nxt = 1 - cur;
     PanelArgs pa{n, k0, nbw, B[cur].p, B[cur].stride, B[cur].ld,
                  B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, info, mask};
-    if (NB == 32 && kind == 0)
-      launch_panel_cg<T>(s, batch, n, pa);
-    else if (NB == 32 && RPT == 1 && kind == 4)
-      hipLaunchKernelGGL((inv_panel4<T>), dim3((unsigned)batch), dim3(NT4), 0, s, pa);
-    else
-      hipLaunchKernelGGL((inv_panel<T, NB, RPT>), dim3((unsigned)batch), dim3(NT), 0, s, pa);
+    if constexpr (NB == 64) {
+      launch_panel_cg<T, 64>(s, batch, n, pa);
+    } else {
+      if (NB == 32 && kind == 0)
+        launch_panel_cg<T, 32>(s, batch, n, pa);
+      else if (NB == 32 && RPT == 1 && kind == 4)
+        hipLaunchKernelGGL((inv_panel4<T>), dim3((unsigned)batch), dim3(NT4), 0, s, pa);
+      else
+        hipLaunchKernelGGL((inv_panel<T, NB, RPT>), dim3((unsigned)batch), dim3(NT), 0, s, pa);
+    }
     if (n > nbw) {
       UpdArgs ua{n, k0, nbw, tn * tn, B[cur].p, B[cur].stride, B[cur].ld,
                  B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, mask};
@@ -908,15 +927,16 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
 }  // namespace
 
 // Panel width: 32 columns; with the older panel kernels 16 above n = 256
-// (where two rows per thread double their per-column work).  QX_INV_NB=16|32
-// overrides for n <= 256 (16 runs the one-thread-per-row kernel).
+// (where two rows per thread double their per-column work).  QX_INV_NB=16|32|64
+// overrides for n <= 256 (16 runs the one-thread-per-row kernel; 64 the
+// column-group panel of eight waves of eight columns, whose rank-64 updates
+// pass over the matrix half as often).
 int inv_panel_width(int n) {
-  static const int forced_nb = []() {
-    const char* e = std::getenv("QX_INV_NB");
-    const int v = e ? std::atoi(e) : 0;
-    return (v == 16 || v == 32) ? v : 0;
-  }();
+  const char* e = std::getenv("QX_INV_NB");  // read per call
+  const int v = e ? std::atoi(e) : 0;
+  const int forced_nb = (v == 16 || v == 32 || v == 64) ? v : 0;
   if (n > 256) return inv_panel_kind() == 0 ? 32 : 16;
+  if (forced_nb == 64 && inv_panel_kind() != 0) return 32;
   return forced_nb ? forced_nb : 32;
 }
 
@@ -976,10 +996,12 @@ void zinv(hipStream_t s, int64_t batch, int n, const Mat& A, const Mat& S, const
   if (precision == QX_PREC_C64) {
     if (big) run_inverse<float, 16, 2>(s, batch, n, X, scr, s0, info, mask);
     else if (NB == 16) run_inverse<float, 16, 1>(s, batch, n, X, scr, s0, info, mask);
+    else if (NB == 64) run_inverse<float, 64, 1>(s, batch, n, X, scr, s0, info, mask);
     else run_inverse<float, 32, 1>(s, batch, n, X, scr, s0, info, mask);
   } else {
     if (big) run_inverse<double, 16, 2>(s, batch, n, X, scr, s0, info, mask);
     else if (NB == 16) run_inverse<double, 16, 1>(s, batch, n, X, scr, s0, info, mask);
+    else if (NB == 64) run_inverse<double, 64, 1>(s, batch, n, X, scr, s0, info, mask);
     else run_inverse<double, 32, 1>(s, batch, n, X, scr, s0, info, mask);
   }
   QX_HIP_CHECK(hipGetLastError());

@@ -245,8 +245,38 @@ def test_panel_kernels_agree(device, monkeypatch, n, prec):
             assert rel_err(o[e], outs[("1", "8")][e]) < tol, (v, e)
 
 
+@pytest.mark.parametrize("n", [33, 64, 65, 100, 128, 200, 256])
+@pytest.mark.parametrize("prec", [0, 1], ids=["c128", "c64"])
+@pytest.mark.parametrize("in_place", [False, True])
+def test_panel_width_64(device, monkeypatch, n, prec, in_place):
+    """64-column panels (eight waves of eight columns) with rank-64 updates in
+    two k-chunks, against the 32-column panels and NumPy: the same
+    elimination blocked differently, so equal to rounding; in place and with
+    a separate output (the panel count's parity picks the start buffer)."""
+    rng = np.random.default_rng(7 * n + prec)
+    batch = 3
+    a = _cplx(rng, (batch, n, n))
+    a[:, 0, 0] = 0.0  # a row interchange at the first pivot
+    a[:, n - 1, n - 1] = 0.0
+    outs = {}
+    for nb in ("64", "32"):
+        monkeypatch.setenv("QX_INV_NB", nb)
+        A = torch.as_tensor(a, device=device)
+        X = A if in_place else torch.empty_like(A)
+        _inv(device, A, X, prec)
+        outs[nb] = X.cpu().numpy()
+    cond = max(np.linalg.cond(a[e]) for e in range(batch))
+    tol = (1e-15 if prec == 0 else 1e-7) * cond * n
+    for e in range(batch):
+        ref = np.linalg.inv(a[e])
+        assert rel_err(outs["64"][e], ref) < tol, (e, rel_err(outs["64"][e], ref), tol)
+        assert rel_err(outs["64"][e], outs["32"][e]) < tol
+
+
+@pytest.mark.parametrize("nb", ["32", "64"])
 @pytest.mark.parametrize("n", [64, 300])
-def test_singular_panel_raises(device, n):
+def test_singular_panel_raises(device, monkeypatch, nb, n):
+    monkeypatch.setenv("QX_INV_NB", nb)
     a = np.zeros((2, n, n), complex)
     a[0] = np.eye(n)
     a[1, :, : n // 2] = 1.0  # rank 1
