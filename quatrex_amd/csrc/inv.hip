@@ -695,14 +695,18 @@ struct UpdArgs {
 
 // Xn[r, c] = (r in K ? 0 : Xc[sig r, c]) + sum_k Xn[r, k0+k] Xc[sig(k0+k), c]
 // for the columns c outside the panel; 64 x 64 tiles, 4 waves of 32 x 32.
-// The rank-NB sum runs over k-chunks of KC = 32 staged through LDS one after
-// another (NB = 64: two), so a wider panel costs no more LDS per workgroup.
-template <typename T, int NB>
-__global__ __launch_bounds__(NT) void inv_update(UpdArgs g) {
+// The rank-NB sum runs over k-chunks of KC staged through LDS one after
+// another (KC = 32; NB = 64: two chunks), so a wider panel costs no more LDS
+// per workgroup.  KC = 16 with a register target of WPE = 4 waves per SIMD
+// halves the LDS and registers of a workgroup so that four instead of two
+// share a CU: more loads in flight for large batches, where the update
+// streams every matrix from HBM.  The k order of the sum is the same.
+template <typename T, int NB, int KC, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+void inv_update(UpdArgs g) {
   typedef InvMma<T> MM;
   typedef typename MM::acc_t acc_t;
-  constexpr int KC = 32;
-  static_assert(NB % KC == 0 || NB < KC, "panel width: a multiple of 32, or below 32");
+  static_assert(NB % KC == 0 || NB < KC, "panel width: a multiple of the chunk, or below it");
   constexpr int KW = NB < KC ? NB : KC;  // staged k per chunk
   __shared__ T As[2][KW][64 + 1];
   __shared__ T Bs[2][KW][64 + 1];
@@ -866,6 +870,18 @@ int inv_group_width() {
   return (e && std::atoi(e) == 8) ? 8 : 4;
 }
 
+// Update kernel: 16-deep k-chunks with four workgroups per CU from 256
+// matrices per call (at 1024 matrices of 256: 6.63 -> 6.12 ms per inverse;
+// 128: 1.26 -> 1.19 ms), 32-deep chunks with two below (at 32-128 matrices
+// equal or up to 2 % faster).  Both give the same bits.  QX_INV_UPDATE=16|32
+// (read per call) forces one.
+bool narrow_update(int64_t batch) {
+  const char* e = std::getenv("QX_INV_UPDATE");
+  const int v = e ? std::atoi(e) : 0;
+  if (v == 16 || v == 32) return v == 16;
+  return batch >= 256;
+}
+
 template <typename T, int NB>
 void launch_panel_cg(hipStream_t s, int64_t batch, int n, const PanelArgs& pa) {
   if constexpr (NB == 64) {
@@ -911,7 +927,10 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
                  B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, mask};
       const int64_t blocks = batch * tn * tn;
       QX_REQUIRE(blocks < (int64_t(1) << 31), "inv: grid too large");
-      hipLaunchKernelGGL((inv_update<T, NB>), dim3((unsigned)blocks), dim3(NT), 0, s, ua);
+      if (narrow_update(batch))
+        hipLaunchKernelGGL((inv_update<T, NB, 16, 4>), dim3((unsigned)blocks), dim3(NT), 0, s, ua);
+      else
+        hipLaunchKernelGGL((inv_update<T, NB, 32, 1>), dim3((unsigned)blocks), dim3(NT), 0, s, ua);
     }
     cur = nxt;
   }

@@ -273,6 +273,29 @@ def test_panel_width_64(device, monkeypatch, n, prec, in_place):
         assert rel_err(outs["64"][e], outs["32"][e]) < tol
 
 
+@pytest.mark.parametrize("n", [40, 100, 256])
+@pytest.mark.parametrize("prec", [0, 1], ids=["c128", "c64"])
+def test_update_chunk_depth_bitwise(device, monkeypatch, n, prec):
+    """The trailing update staged in 16-deep k-chunks (four workgroups per
+    CU, the default from 256 matrices per call) or 32-deep ones (two): the
+    same products summed in the same k order, so the same bits."""
+    rng = np.random.default_rng(11 * n + prec)
+    a = _cplx(rng, (5, n, n))
+    a[:, 0, 0] = 0.0
+    outs = {}
+    for depth in ("16", "32"):
+        monkeypatch.setenv("QX_INV_UPDATE", depth)
+        A = torch.as_tensor(a, device=device)
+        X = torch.empty_like(A)
+        _inv(device, A, X, prec)
+        outs[depth] = X.cpu().numpy()
+    assert np.array_equal(outs["16"].view(np.uint64), outs["32"].view(np.uint64))
+    cond = max(np.linalg.cond(a[e]) for e in range(5))
+    tol = (1e-15 if prec == 0 else 1e-7) * cond * n
+    for e in range(5):
+        assert rel_err(outs["16"][e], np.linalg.inv(a[e])) < tol
+
+
 @pytest.mark.parametrize("nb", ["32", "64"])
 @pytest.mark.parametrize("n", [64, 300])
 def test_singular_panel_raises(device, monkeypatch, nb, n):
