@@ -6,7 +6,7 @@
 // Algorithm: blocked in-place Gauss-Jordan elimination split into kernels so
 // that the O(n^3) part runs on every CU of the chip while the sequential
 // pivoting chain stays short:
-//   for each panel of NB = 32 columns
+//   for each panel of NB = 32 columns (QX_INV_NB=64: 64 for n <= 256)
 //     panel kernel  (one workgroup per matrix): by default inv_panel_cg, in
 //        which each wave holds a group of the panel's columns for all rows
 //        and the rows never move (one barrier per column; see the kernel);
@@ -17,7 +17,8 @@
 //        unreduced rows, compared through a 32-bit key (float magnitude with
 //        9 low mantissa bits replaced by the row index).  The reduced panel,
 //        the pivot rows and the composed row gather sig are written out.
-//     update kernel (64 x 64 tiles of every matrix, MFMA): the other columns
+//     update kernel (64 x 64 tiles of every matrix, MFMA; the rank-NB sum
+//        staged through LDS in 16- or 32-deep k-chunks): the other columns
 //        receive the row interchanges and the rank-NB update in one pass,
 //          Xn[r, C] = (r in pivot rows ? 0 : Xc[sig r, C]) + P[r, :] Xc[sig K, C],
 //        reading buffer Xc and writing buffer Xn (ping-pong), so the gather
