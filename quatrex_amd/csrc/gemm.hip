@@ -374,9 +374,18 @@ void zgemm_kernel(GemmArgs g) {
 // The arithmetic per output element (order of the k sum, 4M / 3M split, the
 // epilogue's explicit fma) is the same as zgemm_kernel's, so both kernels
 // give bit-identical results.
-template <int OPA, int OPB, int SPLIT, int TM, int TN, int D, bool CHECK>
+//
+// T = float (complex64 arithmetic on complex128 storage): the fragments are
+// loaded as complex128 and rounded to float right before their k-step, the
+// conjugation is an explicit negation (the f32 instruction's last operand is
+// a lane-broadcast pattern, not a neg modifier), and the 3M sums are formed
+// in float, as zgemm_kernel<float> forms them on its LDS store; the result
+// rows follow the f32 layout.
+template <typename T, int OPA, int OPB, int SPLIT, int TM, int TN, int D, bool CHECK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void zgemm_reg_kernel(GemmArgs g) {
+  constexpr bool F64 = sizeof(T) == 8;
+  typedef typename MfmaTraits<T>::acc_t acc_t;
   constexpr int BM = 32 * TM, BN = 32 * TN;
   // neg-modifier masks: s_a = -1 when op(A) = C (imag part negated), s_b ditto
   constexpr bool CA = OPA == QX_OP_C, CB = OPB == QX_OP_C;
@@ -423,19 +432,62 @@ void zgemm_reg_kernel(GemmArgs g) {
     for (int j = 0; j < TN; ++j) fb[j] = pb[j][kidx * kb_];
   };
 
-  d4 acc1[TM][TN], acc2[TM][TN], acc3[SPLIT == 3 ? TM : 1][TN];
+  acc_t acc1[TM][TN], acc2[TM][TN], acc3[SPLIT == 3 ? TM : 1][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      acc1[i][j] = d4{0, 0, 0, 0};
-      acc2[i][j] = d4{0, 0, 0, 0};
-      if constexpr (SPLIT == 3) acc3[i][j] = d4{0, 0, 0, 0};
+      acc1[i][j] = acc_t{0, 0, 0, 0};
+      acc2[i][j] = acc_t{0, 0, 0, 0};
+      if constexpr (SPLIT == 3) acc3[i][j] = acc_t{0, 0, 0, 0};
     }
 
   // One k-step of MFMAs on fragments fa[TM], fb[TN].
   auto step = [&](const z2* fa, const z2* fb) {
-    if constexpr (SPLIT == 3) {
+    if constexpr (!F64) {
+      typedef MfmaTraits<T> MT;
+      T ar[TM], ai[TM], br[TN], bi[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        ar[i] = (T)fa[i].x;
+        ai[i] = (T)(CA ? -fa[i].y : fa[i].y);
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        br[j] = (T)fb[j].x;
+        bi[j] = (T)(CB ? -fb[j].y : fb[j].y);
+      }
+      if constexpr (SPLIT == 3) {
+        T as[TM], bs[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) as[i] = ar[i] + ai[i];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bs[j] = br[j] + bi[j];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            acc1[i][j] = MT::mma(ar[i], br[j], acc1[i][j]);
+            acc2[i][j] = MT::mma(ai[i], bi[j], acc2[i][j]);
+            acc3[i][j] = MT::mma(as[i], bs[j], acc3[i][j]);
+          }
+      } else {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            acc1[i][j] = MT::mma(ar[i], br[j], acc1[i][j]);
+            acc2[i][j] = MT::mma(ar[i], bi[j], acc2[i][j]);
+          }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            acc1[i][j] = MT::mma(-ai[i], bi[j], acc1[i][j]);
+            acc2[i][j] = MT::mma(ai[i], br[j], acc2[i][j]);
+          }
+      }
+    } else if constexpr (SPLIT == 3) {
       double as[TM], bs[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) as[i] = CA ? fa[i].x - fa[i].y : fa[i].x + fa[i].y;
@@ -531,17 +583,17 @@ void zgemm_reg_kernel(GemmArgs g) {
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int row = m0 + wm * 16 * TM + i * 16 + k4 + 4 * q;
+        const int row = m0 + wm * 16 * TM + i * 16 + (F64 ? k4 + 4 * q : 4 * k4 + q);
         const int col = n0 + wn * 16 * TN + j * 16 + r16;
         if (!CHECK || (row < g.m && col < g.n)) {
           double vr, vi;
           if constexpr (SPLIT == 3) {
-            const double p1 = acc1[i][j][q], p2 = acc2[i][j][q];
+            const double p1 = (double)acc1[i][j][q], p2 = (double)acc2[i][j][q];
             vr = p1 - p2;
-            vi = acc3[i][j][q] - p1 - p2;
+            vi = (double)acc3[i][j][q] - p1 - p2;
           } else {
-            vr = acc1[i][j][q];
-            vi = acc2[i][j][q];
+            vr = (double)acc1[i][j][q];
+            vi = (double)acc2[i][j][q];
           }
           // explicit fma, as in zgemm_kernel
           double outr = fma(g.ar, vr, -(g.ai * vi)), outi = fma(g.ar, vi, g.ai * vr);
@@ -595,7 +647,7 @@ int gemm_tile(int m, int n, int64_t batch) {
   return (m >= 128 && n >= 128 && big >= 256) ? 128 : 64;
 }
 
-template <int SPLIT, int TM, int TN, int D>
+template <typename T, int SPLIT, int TM, int TN, int D>
 void launch_reg(hipStream_t s, int64_t batch, int opA, int opB, GemmArgs g) {
   constexpr int BM = 32 * TM, BN = 32 * TN;
   g.tilesM = (g.m + BM - 1) / BM;
@@ -609,9 +661,9 @@ void launch_reg(hipStream_t s, int64_t batch, int opA, int opB, GemmArgs g) {
   if (timer) timer->begin(s, &ev0);
 #define QX_REG_LAUNCH(OA, OB)                                                                   \
   if (full)                                                                                     \
-    hipLaunchKernelGGL((zgemm_reg_kernel<OA, OB, SPLIT, TM, TN, D, false>), grid, block, 0, s, g); \
+    hipLaunchKernelGGL((zgemm_reg_kernel<T, OA, OB, SPLIT, TM, TN, D, false>), grid, block, 0, s, g); \
   else                                                                                          \
-    hipLaunchKernelGGL((zgemm_reg_kernel<OA, OB, SPLIT, TM, TN, D, true>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((zgemm_reg_kernel<T, OA, OB, SPLIT, TM, TN, D, true>), grid, block, 0, s, g);
   if (opA == QX_OP_N && opB == QX_OP_N) {
     QX_REG_LAUNCH(QX_OP_N, QX_OP_N)
   } else if (opA == QX_OP_N && opB == QX_OP_C) {
@@ -673,24 +725,36 @@ int reg_shape(int split, int opA, int opB, int m, int n) {
   return split == 3 ? REG_2x2x4 : shape;
 }
 
-template <int SPLIT>
+template <typename T, int SPLIT>
 void launch_reg_shape(hipStream_t s, int64_t batch, int opA, int opB, const GemmArgs& g, int shape) {
   if constexpr (SPLIT == 4) {
     switch (shape) {
-      case REG_2x4x2: launch_reg<SPLIT, 2, 4, 2>(s, batch, opA, opB, g); return;
-      case REG_4x2x2: launch_reg<SPLIT, 4, 2, 2>(s, batch, opA, opB, g); return;
+      case REG_2x4x2: launch_reg<T, SPLIT, 2, 4, 2>(s, batch, opA, opB, g); return;
+      case REG_4x2x2: launch_reg<T, SPLIT, 4, 2, 2>(s, batch, opA, opB, g); return;
       default: break;
     }
   }
-  launch_reg<SPLIT, 2, 2, 4>(s, batch, opA, opB, g);
+  launch_reg<T, SPLIT, 2, 2, 4>(s, batch, opA, opB, g);
+}
+
+// complex64 products on the register-direct kernel only with QX_ZGEMM_F32=reg
+// (read per call).  Same bits as the LDS-staged kernel, but slower: each
+// fragment is still 16 bytes of complex128 storage per lane and per wave
+// while the f32 matrix core runs twice as fast, so the register-direct
+// loads bind (32 x 256^3, A.B: 54.0 vs 45.7 us; mixed RGF at 256 energies:
+// 405 vs 458 energies/s); the LDS kernel loads each element once per
+// workgroup and keeps it as float.
+bool reg_f32() {
+  const char* e = std::getenv("QX_ZGEMM_F32");
+  return e && std::string(e) == "reg";
 }
 
 template <typename T, int SPLIT>
 void launch_typed(hipStream_t s, int64_t batch, int opA, int opB, const GemmArgs& g) {
-  if constexpr (sizeof(T) == 8) {
+  if (sizeof(T) == 8 || reg_f32()) {
     const int shape = reg_shape(SPLIT, opA, opB, g.m, g.n);
     if (shape >= 0) {
-      launch_reg_shape<SPLIT>(s, batch, opA, opB, g, shape);
+      launch_reg_shape<T, SPLIT>(s, batch, opA, opB, g, shape);
       return;
     }
   }

@@ -109,11 +109,14 @@ REG_CASES = [(256, 256, 256, 8), (128, 128, 64, 4), (200, 136, 72, 3), (5, 7, 3,
 
 @pytest.mark.parametrize("mnkb", REG_CASES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("ops", [(0, 0), (0, 2), (2, 0), (2, 2)], ids=["NN", "NC", "CN", "CC"])
-@pytest.mark.parametrize("prec", [0, 3], ids=["c128", "c128_3m"])
-def test_zgemm_register_kernels_match_lds(device, mnkb, ops, prec):
+@pytest.mark.parametrize("prec", [0, 3, 1], ids=["c128", "c128_3m", "c64"])
+def test_zgemm_register_kernels_match_lds(device, monkeypatch, mnkb, ops, prec):
     """Every register-direct GEMM shape gives the bits of the LDS-staged
     kernel: both sum k in the same order with the same 4M / 3M split (full
-    tiles, ragged tiles and k not a multiple of 4)."""
+    tiles, ragged tiles and k not a multiple of 4).  complex64 arithmetic
+    (3M, register-direct on request: QX_ZGEMM_F32=reg): both round the
+    operands to float and form the 3M sums in float."""
+    monkeypatch.setenv("QX_ZGEMM_F32", "reg")
     L = _lib()
     m, n, k, batch = mnkb
     rng = np.random.default_rng(m + 2 * n + 3 * k)
