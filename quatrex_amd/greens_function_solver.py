@@ -72,6 +72,83 @@ class _Bound:
         inner = int(np.prod(mat.stack_shape[1:], dtype=np.int64))
         self.S = (s1 - s0) * inner
         bs = mat.block_sizes
+        # Which blocks are read or written, and where: a function of the
+        # pattern, the symmetry and the flags only, kept in the pattern cache
+        # that the container shares with its views and zeros_like copies
+        # (with the layouts built from it), so repeated calls on one pattern
+        # skip the per-block walk.
+        plan_key = ("_qx_plan", bool(output), bool(write_lower), bool(write_offdiag),
+                    bool(mat.symmetry))
+        plan = mat._slices.get(plan_key)
+        if plan is None:
+            plan = self._plan(mat, write_lower, write_offdiag)
+            mat._slices[plan_key] = plan
+        want, direct, staged = plan["want"], plan["direct"], plan["staged"]
+        self.staging = None
+
+        def offsets():
+            diag = np.full(n, -1, np.int64)
+            upper = np.full(max(n - 1, 0), -1, np.int64)
+            lower = np.full(max(n - 1, 0), -1, np.int64)
+            return diag, upper, lower, {"d": diag, "u": upper, "l": lower}
+
+        # A read-only stack broadcast over its only stack axis (stride 0, e.g.
+        # an energy-independent self-energy expanded to the batch) is passed
+        # with stack stride 0: every matrix of the batch reads the same data.
+        # Any other non-contiguous input (a broadcast over the first of
+        # several stack axes repeats with the period of the inner axes, which
+        # one stack stride cannot express) is copied contiguously for this
+        # batch; outputs must be contiguous.
+        data = mat.data
+        bcast = (not output and data.dim() == 2 and data.shape[0] > 1 and data.stride(0) == 0
+                 and data.stride(1) == 1)
+        self._copy = None
+        if not staged:
+            if bcast:
+                self.base, stride = data.data_ptr(), 0
+            elif data.is_contiguous():
+                self.base = data.data_ptr() + s0 * inner * mat.nnz * data.element_size()
+                stride = mat.nnz
+            elif output:
+                raise ValueError("an output DSDBCOO.data must be contiguous")
+            else:
+                self._copy = data[s0:s1].reshape(self.S, mat.nnz).contiguous()
+                self.base, stride = self._copy.data_ptr(), mat.nnz
+            lay = plan["layouts"].get((self.S, stride))
+            if lay is None:
+                diag, upper, lower, arrs = offsets()
+                for (kind, i), off in direct.items():
+                    arrs[kind][i] = off
+                lay = plan["layouts"][(self.S, stride)] = _lib.make_layout(
+                    bs, self.S, stride, diag, upper, lower)
+            self.layout, self._keep = lay
+        else:
+            # all blocks of this container go through one dense staging buffer
+            diag, upper, lower, arrs = offsets()
+            keys = sorted(set(direct) | set(staged), key=lambda k: (k[1], k[0]))
+            off, offs = 0, {}
+            for key in keys:
+                i, j = want[key]
+                offs[key] = off
+                off += int(bs[i]) * int(bs[j])
+            self.staging = torch.zeros((self.S, max(off, 1)), dtype=torch.complex128,
+                                       device=mat.device)
+            self._staged_blocks = {}
+            view = mat.stack[s0:s1]
+            for key in keys:
+                i, j = want[key]
+                arrs[key[0]][key[1]] = offs[key]
+                seg = self.staging[:, offs[key]:offs[key] + int(bs[i]) * int(bs[j])]
+                self._staged_blocks[key] = (i, j, seg)
+                if not output:
+                    seg.copy_(view.blocks[i, j].reshape(self.S, -1))
+            self.base = self.staging.data_ptr()
+            stride = self.staging.shape[1]
+            self.layout, self._keep = _lib.make_layout(bs, self.S, stride, diag, upper, lower)
+
+    @staticmethod
+    def _plan(mat, write_lower, write_offdiag):
+        n = mat.num_blocks
         want = {}
         for i in range(n):
             want[("d", i)] = (i, i)
@@ -95,57 +172,7 @@ class _Bound:
                 direct[key] = sl.start
             else:
                 staged.append(key)
-        self.staging = None
-        diag = np.full(n, -1, np.int64)
-        upper = np.full(max(n - 1, 0), -1, np.int64)
-        lower = np.full(max(n - 1, 0), -1, np.int64)
-        arrs = {"d": diag, "u": upper, "l": lower}
-        # A read-only stack broadcast over its only stack axis (stride 0, e.g.
-        # an energy-independent self-energy expanded to the batch) is passed
-        # with stack stride 0: every matrix of the batch reads the same data.
-        # Any other non-contiguous input (a broadcast over the first of
-        # several stack axes repeats with the period of the inner axes, which
-        # one stack stride cannot express) is copied contiguously for this
-        # batch; outputs must be contiguous.
-        data = mat.data
-        bcast = (not output and data.dim() == 2 and data.shape[0] > 1 and data.stride(0) == 0
-                 and data.stride(1) == 1)
-        self._copy = None
-        if not staged:
-            for (kind, i), off in direct.items():
-                arrs[kind][i] = off
-            if bcast:
-                self.base, stride = data.data_ptr(), 0
-            elif data.is_contiguous():
-                self.base = data.data_ptr() + s0 * inner * mat.nnz * data.element_size()
-                stride = mat.nnz
-            elif output:
-                raise ValueError("an output DSDBCOO.data must be contiguous")
-            else:
-                self._copy = data[s0:s1].reshape(self.S, mat.nnz).contiguous()
-                self.base, stride = self._copy.data_ptr(), mat.nnz
-        else:
-            # all blocks of this container go through one dense staging buffer
-            keys = sorted(set(direct) | set(staged), key=lambda k: (k[1], k[0]))
-            off, offs = 0, {}
-            for key in keys:
-                i, j = want[key]
-                offs[key] = off
-                off += int(bs[i]) * int(bs[j])
-            self.staging = torch.zeros((self.S, max(off, 1)), dtype=torch.complex128,
-                                       device=mat.device)
-            self._staged_blocks = {}
-            view = mat.stack[s0:s1]
-            for key in keys:
-                i, j = want[key]
-                arrs[key[0]][key[1]] = offs[key]
-                seg = self.staging[:, offs[key]:offs[key] + int(bs[i]) * int(bs[j])]
-                self._staged_blocks[key] = (i, j, seg)
-                if not output:
-                    seg.copy_(view.blocks[i, j].reshape(self.S, -1))
-            self.base = self.staging.data_ptr()
-            stride = self.staging.shape[1]
-        self.layout, self._keep = _lib.make_layout(bs, self.S, stride, diag, upper, lower)
+        return {"want": want, "direct": direct, "staged": staged, "layouts": {}}
 
     def ptr(self):
         return ctypes.c_void_p(self.base)
