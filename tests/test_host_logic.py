@@ -109,3 +109,48 @@ def test_bound_strides_for_contiguous_and_broadcast_stacks(monkeypatch, s0, s1):
     assert torch.equal(b._copy, exp[s0:s1].reshape(-1, nnz))
     with pytest.raises(ValueError, match="contiguous"):
         _bound_case(monkeypatch, exp, (E, K), s0, s1, output=True)
+
+
+def test_bound_plan_cache_matches_fresh_layouts(monkeypatch):
+    """The block plan and layout cached in the pattern cache (shared by views
+    and zeros_like copies) give the offsets a fresh container computes, per
+    flag combination and batch size; the base pointer follows each call."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from quatrex_amd import _lib
+    from quatrex_amd.datastructures import DSDBCOO, btd_pattern
+    from quatrex_amd.greens_function_solver import _Bound
+
+    monkeypatch.setattr(_lib, "require_device_tensor", lambda t, what: None)
+    bs = [2, 3, 2]
+    rows, cols, _ = btd_pattern(bs, offdiag=True)
+    nnz = len(rows)
+
+    def offsets(b):
+        n = len(bs)
+        lay = b.layout
+        get = lambda p, k: np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_int64)),
+                                                 shape=(k,)).copy()
+        return (lay.n_stack, lay.stack_stride, tuple(get(lay.diag, n)),
+                tuple(get(lay.upper, n - 1)), tuple(get(lay.lower, n - 1)))
+
+    data = torch.arange(5 * nnz, dtype=torch.float64).to(torch.complex128).reshape(5, nnz)
+    m = DSDBCOO(data, rows, cols, bs, (5,))
+    z = DSDBCOO.zeros_like(m)  # shares the pattern cache
+    cases = [dict(output=False), dict(output=True), dict(output=True, write_lower=False),
+             dict(output=True, write_offdiag=False)]
+    for kw in cases:
+        for s0, s1 in ((0, 5), (1, 3), (0, 5)):
+            fresh = _Bound(DSDBCOO(data.clone(), rows, cols, bs, (5,)), s0, s1, **kw)
+            first = _Bound(m, s0, s1, **kw)
+            again = _Bound(z, s0, s1, **kw)
+            assert offsets(first) == offsets(fresh) == offsets(again)
+            assert first.base == data.data_ptr() + s0 * nnz * 16
+            assert again.base == z.data.data_ptr() + s0 * nnz * 16
+    lw = _Bound(m, 0, 5, output=True, write_lower=False)
+    assert offsets(lw)[4] == (-1, -1)  # lower blocks not written
+    off = _Bound(m, 0, 5, output=True, write_offdiag=False)
+    assert offsets(off)[3] == (-1, -1) and offsets(off)[4] == (-1, -1)
