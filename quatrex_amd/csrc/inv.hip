@@ -694,6 +694,15 @@ struct UpdArgs {
   const int* mask;  // nullable, as PanelArgs::mask
 };
 
+// Workgroup ids are dealt round-robin over the 8 XCDs; this maps them so
+// that each XCD receives one contiguous range of logical ids.
+__device__ inline int update_xcd_remap(int bid, int nblocks) {
+  const int nx = 8;
+  const int per = nblocks / nx, rem = nblocks % nx;
+  const int x = bid % nx, j = bid / nx;
+  return x * per + min(x, rem) + j;
+}
+
 // Xn[r, c] = (r in K ? 0 : Xc[sig r, c]) + sum_k Xn[r, k0+k] Xc[sig(k0+k), c]
 // for the columns c outside the panel; 64 x 64 tiles, 4 waves of 32 x 32.
 // The rank-NB sum runs over k-chunks of KC staged through LDS one after
@@ -714,8 +723,11 @@ void inv_update(UpdArgs g) {
   __shared__ int sig_r[64], sig_k[NB];
   const int tiles = g.tiles;  // per matrix, row-major over (row tile, col tile)
   const int tn = (g.n + 63) / 64;
-  const int64_t e = blockIdx.x / tiles;
-  const int t = blockIdx.x - (int)(e * tiles);
+  // the tiles of one matrix on one XCD, so that the panel columns and pivot
+  // rows every tile reads again come from that XCD's L2
+  const int id = update_xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t e = id / tiles;
+  const int t = id - (int)(e * tiles);
   const int r0 = (t / tn) * 64, c0 = (t % tn) * 64;
   const int n = g.n, k0 = g.k0, nbw = g.nbw;
   // a column tile that lies inside the panel has nothing to do
