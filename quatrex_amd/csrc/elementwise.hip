@@ -96,11 +96,24 @@ struct HermArgs {
   int ldy2;
 };
 
+// Workgroup ids are dealt round-robin over the 8 XCDs; this maps them so
+// that each XCD receives one contiguous range of logical ids (the tiles of
+// one matrix together).
+__device__ inline int tile_xcd_remap(int bid, int nblocks) {
+  const int nx = 8;
+  const int per = nblocks / nx, rem = nblocks % nx;
+  const int x = bid % nx, j = bid / nx;
+  return x * per + min(x, rem) + j;
+}
+
+// Tiles (r, c) and (c, r) read each other's data: with the tiles of a matrix
+// on one XCD the second read comes from its L2.
 __global__ __launch_bounds__(NT) void herm_kernel(HermArgs g) {
   __shared__ z2 tile[TS][TS + 1];
   const int tiles = g.tm * g.tn;
-  const int64_t eg = blockIdx.x / tiles;
-  const int t = blockIdx.x - (int)(eg * tiles);
+  const int id = tile_xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t eg = id / tiles;
+  const int t = id - (int)(eg * tiles);
   const bool second = eg >= g.half;
   const int64_t e = second ? eg - g.half : eg;
   const z2* A = second ? g.A2 + e * g.sA2 : (g.A ? g.A + e * g.sA : nullptr);
@@ -182,8 +195,9 @@ __global__ __launch_bounds__(NT) void contact_sigma_kernel(const z2* S, int64_t 
                                                            z2* G, int64_t sG, int tn) {
   __shared__ z2 tile[TS][TS + 1];
   const int tiles = tn * tn;
-  const int64_t e = blockIdx.x / tiles;
-  const int t = blockIdx.x - (int)(e * tiles);
+  const int id = tile_xcd_remap(blockIdx.x, gridDim.x);  // as herm_kernel
+  const int64_t e = id / tiles;
+  const int t = id - (int)(e * tiles);
   const int r0 = (t / tn) * TS, c0 = (t % tn) * TS;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   const z2* Se = S + e * sS;
