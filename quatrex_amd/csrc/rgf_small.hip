@@ -16,9 +16,9 @@
 // - The two teams run independent products of one step side by side: the
 //   lesser and greater chains of a layer, pairs of retarded products
 //   (rgf_small_kernel describes the schedule).  The sweep is a chain of
-//   dependent steps, each a fraction of a microsecond of matrix-core work
-//   behind a memory round trip and a barrier, so halving the number of
-//   steps is what shortens it.
+//   dependent steps, each about a microsecond of one SIMD's matrix-core
+//   work behind a memory round trip and a barrier, so fewer steps make it
+//   shorter.
 // - Block inverse (sinv_wg): the four waves of team 0, eight lanes per row
 //   (four columns each), Gauss-Jordan with partial pivoting and one barrier
 //   per column: the pivot is a DPP wave maximum of 32-bit keys (float
@@ -518,7 +518,7 @@ __device__ __forceinline__ void sinv_wg(int n, const z2* A, int lda, const z2* S
 #pragma unroll
           for (int c = 0; c < 4; ++c) sh.cand[b][w][4 * q + c] = make_double2(xr[c], xi[c]);
         }
-        if (lane == 0 && w < kThreads / 64) sh.keys[b][w] = wkey;  // rows >= 32: none
+        if (lane == 0 && w < kThreads / 64) sh.keys[b][w] = wkey;  // waves 4..7 hold no row
         if (r == k) {
 #pragma unroll
           for (int c = 0; c < 4; ++c) sh.krow[b][4 * q + c] = make_double2(xr[c], xi[c]);
