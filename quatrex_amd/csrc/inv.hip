@@ -23,6 +23,9 @@
 //          Xn[r, C] = (r in pivot rows ? 0 : Xc[sig r, C]) + P[r, :] Xc[sig K, C],
 //        reading buffer Xc and writing buffer Xn (ping-pong), so the gather
 //        is race free across workgroups.
+//   Below 256 matrices per call the panel of step p + 1 runs beside the
+//   update of step p (look-ahead, inv_step): only the update of the column
+//   tile that holds the next panel precedes it.
 //   A final gather undoes the column interchanges (reverse order of the
 //   pivots) into the output.  The flop count equals getrf + getri
 //   (8 n^3 real flops per complex matrix).
@@ -113,10 +116,11 @@ struct PanelArgs {
   z2* Xn;
   int64_t sXn;
   int ldn;
-  int* ints;  // per matrix: perm[n], sig[n], idx[n]
+  int* ints;  // per matrix: two row-gather buffers sig[2][n], then idx[n]
   int64_t sI;
   int* info;
   const int* mask;  // nullable: matrices with mask[e] == 0 are left alone
+  int sig_sel;  // the panel's row gather goes to ints + n (0) or ints (1)
 };
 
 // Row and column gathers of a finished panel.  Each position is traced
@@ -133,7 +137,7 @@ struct PanelArgs {
 //           complete gather and no panel traces more than NB steps.
 template <int NTH>
 __device__ __forceinline__ void panel_gathers(const int* piv, int n, int k0, int nbw, int tid,
-                                              int* perm, int* sig, int* idx) {
+                                              int* sig, int* idx) {
   for (int u = tid; u < n; u += NTH) {
     int pc = (k0 == 0) ? u : idx[u];
     int pr = u;
@@ -147,7 +151,6 @@ __device__ __forceinline__ void panel_gathers(const int* piv, int n, int k0, int
     sig[u] = pr;
     idx[u] = pc;
   }
-  for (int u = tid; u < nbw; u += NTH) perm[k0 + u] = piv[u];
 }
 
 // Each thread keeps its row(s) of the panel in registers, rotated so that
@@ -165,9 +168,8 @@ __global__ __launch_bounds__(NT) void inv_panel(PanelArgs g) {
   const int n = g.n, k0 = g.k0, nbw = g.nbw;
   const z2* Xc = g.Xc + e * g.sXc;
   z2* Xn = g.Xn + e * g.sXn;
-  int* perm = g.ints + e * g.sI;
-  int* sig = perm + n;
-  int* idx = sig + n;
+  int* sig = g.ints + e * g.sI + (g.sig_sel ? 0 : n);
+  int* idx = g.ints + e * g.sI + 2 * n;
 
   T rr[RPT][NB], ri[RPT][NB];
 #pragma unroll
@@ -299,9 +301,9 @@ __global__ __launch_bounds__(NT) void inv_panel(PanelArgs g) {
       }
     }
   }
-  // pivot record and gathers (this panel's pivots are in perm_l since the
-  // column loop's last barrier)
-  panel_gathers<NT>(perm_l, n, k0, nbw, tid, perm, sig, idx);
+  // gathers (this panel's pivots are in perm_l since the column loop's last
+  // barrier)
+  panel_gathers<NT>(perm_l, n, k0, nbw, tid, sig, idx);
 }
 
 // Panel kernel for n <= 256 with four threads per row (1024 threads): thread
@@ -327,9 +329,8 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
   const int n = g.n, k0 = g.k0, nbw = g.nbw;
   const z2* Xc = g.Xc + e * g.sXc;
   z2* Xn = g.Xn + e * g.sXn;
-  int* perm = g.ints + e * g.sI;
-  int* sig = perm + n;
-  int* idx = sig + n;
+  int* sig = g.ints + e * g.sI + (g.sig_sel ? 0 : n);
+  int* idx = g.ints + e * g.sI + 2 * n;
 
   T rr[CQ], ri[CQ];
 #pragma unroll
@@ -446,9 +447,9 @@ __global__ __launch_bounds__(NT4) void inv_panel4(PanelArgs g) {
         Xn[(int64_t)r * g.ldn + k0 + col] = make_double2((double)rr[c], (double)ri[c]);
     }
   }
-  // pivot record and gathers (this panel's pivots are in perm_l since the
-  // column loop's last barrier)
-  panel_gathers<NT4>(perm_l, n, k0, nbw, tid, perm, sig, idx);
+  // gathers (this panel's pivots are in perm_l since the column loop's last
+  // barrier)
+  panel_gathers<NT4>(perm_l, n, k0, nbw, tid, sig, idx);
 }
 
 // Wave-wide unsigned max with DPP only: quad and row butterflies give every
@@ -492,24 +493,39 @@ __device__ inline unsigned wave_max_bcast(unsigned v) {
 // row), arithmetic and outputs are those of the other panel kernels up to
 // rounding; ties are broken by the row's index at the start of the panel.
 template <typename T, int RPL, int CW, int NB>
-__global__ __launch_bounds__(64 * (NB / CW)) void inv_panel_cg(PanelArgs g) {
-  constexpr int NW = NB / CW, NTH = 64 * NW, NR = 64 * RPL;
-  __shared__ T col_r[2][NR], col_i[2][NR];
-  __shared__ T piv_r[2], piv_i[2];
-  __shared__ unsigned key_b[2];
-  __shared__ T prow_r[NW][CW], prow_i[NW][CW];
-  __shared__ int prow_pos[NW];
-  __shared__ double rcp_r[NB], rcp_i[NB];
-  __shared__ int perm_l[NB];
-  const int64_t e = blockIdx.x;
+struct PanelCgShared {
+  T col_r[2][64 * RPL], col_i[2][64 * RPL];
+  T piv_r[2], piv_i[2];
+  unsigned key_b[2];
+  T prow_r[NB / CW][CW], prow_i[NB / CW][CW];
+  int prow_pos[NB / CW];
+  double rcp_r[NB], rcp_i[NB];
+  int perm_l[NB];
+};
+
+// The panel of matrix e, by a workgroup of 64 NB / CW threads.
+template <typename T, int RPL, int CW, int NB>
+__device__ __forceinline__ void panel_cg(const PanelArgs& g, int64_t e,
+                                         PanelCgShared<T, RPL, CW, NB>& sh) {
+  constexpr int NW = NB / CW, NTH = 64 * NW;
+  auto& col_r = sh.col_r;
+  auto& col_i = sh.col_i;
+  auto& piv_r = sh.piv_r;
+  auto& piv_i = sh.piv_i;
+  auto& key_b = sh.key_b;
+  auto& prow_r = sh.prow_r;
+  auto& prow_i = sh.prow_i;
+  auto& prow_pos = sh.prow_pos;
+  auto& rcp_r = sh.rcp_r;
+  auto& rcp_i = sh.rcp_i;
+  auto& perm_l = sh.perm_l;
   if (g.mask && !g.mask[e]) return;  // uniform over the workgroup
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int n = g.n, k0 = g.k0, nbw = g.nbw;
   const z2* Xc = g.Xc + e * g.sXc;
   z2* Xn = g.Xn + e * g.sXn;
-  int* perm = g.ints + e * g.sI;
-  int* sig = perm + n;
-  int* idx = sig + n;
+  int* sig = g.ints + e * g.sI + (g.sig_sel ? 0 : n);
+  int* idx = g.ints + e * g.sI + 2 * n;
 
   T xr[RPL][CW], xi[RPL][CW];
   int pos[RPL];  // where row lane + 64 j currently stands
@@ -678,11 +694,17 @@ __global__ __launch_bounds__(64 * (NB / CW)) void inv_panel_cg(PanelArgs g) {
       }
     }
   }
-  panel_gathers<NTH>(perm_l, n, k0, nbw, tid, perm, sig, idx);
+  panel_gathers<NTH>(perm_l, n, k0, nbw, tid, sig, idx);
+}
+
+template <typename T, int RPL, int CW, int NB>
+__global__ __launch_bounds__(64 * (NB / CW)) void inv_panel_cg(PanelArgs g) {
+  __shared__ PanelCgShared<T, RPL, CW, NB> sh;
+  panel_cg<T, RPL, CW, NB>(g, blockIdx.x, sh);
 }
 
 struct UpdArgs {
-  int n, k0, nbw, tiles;
+  int n, k0, nbw, tiles;  // tiles per matrix: row tiles x column tiles handled
   const z2* Xc;
   int64_t sXc;
   int ldc;
@@ -692,6 +714,10 @@ struct UpdArgs {
   const int* ints;
   int64_t sI;
   const int* mask;  // nullable, as PanelArgs::mask
+  int sig_sel;      // the panel's row gather: ints + n (0) or ints (1)
+  // Column tiles handled: cfirst + ci for ci < ncols (0: all), stepping over
+  // tile cskip1 - 1 (cskip1 = 0: none).
+  int cfirst, ncols, cskip1;
 };
 
 // Workgroup ids are dealt round-robin over the 8 XCDs; this maps them so
@@ -703,40 +729,68 @@ __device__ inline int update_xcd_remap(int bid, int nblocks) {
   return x * per + min(x, rem) + j;
 }
 
+// Matrix, first row and first column of update tile id.
+struct UpdTile {
+  int64_t e;
+  int r0, c0;
+};
+
+__device__ inline UpdTile update_tile_of(const UpdArgs& g, int id) {
+  const int tn = (g.n + 63) / 64;
+  const int nct = g.ncols ? g.ncols : tn;
+  UpdTile u;
+  u.e = id / g.tiles;
+  const int t = id - (int)(u.e * g.tiles);
+  const int ci = g.cfirst + t % nct;
+  u.r0 = (t / nct) * 64;
+  u.c0 = (ci + (g.cskip1 > 0 && ci >= g.cskip1 - 1 ? 1 : 0)) * 64;
+  return u;
+}
+
+// A column tile that lies inside the panel, or a matrix left alone, has
+// nothing to do.
+__device__ inline bool update_live(const UpdArgs& g, const UpdTile& u) {
+  if (u.c0 >= g.k0 && u.c0 + 64 <= g.k0 + g.nbw) return false;
+  return !(g.mask && !g.mask[u.e]);
+}
+
+template <typename T, int NB, int KC>
+struct UpdShared {
+  static constexpr int KW = NB < KC ? NB : KC;  // staged k per chunk
+  T As[2][KW][64 + 1];
+  T Bs[2][KW][64 + 1];
+  int sig_r[64], sig_k[NB];
+};
+
 // Xn[r, c] = (r in K ? 0 : Xc[sig r, c]) + sum_k Xn[r, k0+k] Xc[sig(k0+k), c]
-// for the columns c outside the panel; 64 x 64 tiles, 4 waves of 32 x 32.
-// The rank-NB sum runs over k-chunks of KC staged through LDS one after
-// another (KC = 32; NB = 64: two chunks), so a wider panel costs no more LDS
-// per workgroup.  KC = 16 with a register target of WPE = 4 waves per SIMD
-// halves the LDS and registers of a workgroup so that four instead of two
-// share a CU: more loads in flight for large batches, where the update
-// streams every matrix from HBM.  The k order of the sum is the same.
-template <typename T, int NB, int KC, int WPE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
-void inv_update(UpdArgs g) {
+// for the columns c outside the panel, on one 64 x 64 tile by 256 threads
+// (tid: the thread's index among them), 4 waves of 32 x 32.  The rank-NB sum
+// runs over k-chunks of KC staged through LDS one after another (KC = 32;
+// NB = 64: two chunks), so a wider panel costs no more LDS per workgroup.
+// KC = 16 with a register target of four waves per SIMD halves the LDS and
+// registers of a workgroup so that four instead of two share a CU: more
+// loads in flight for large batches, where the update streams every matrix
+// from HBM.  The k order of the sum is the same.  A tile that is not live
+// loads only from its matrix's first element and stores nothing, and passes
+// the same workgroup barriers as a live one.
+template <typename T, int NB, int KC>
+__device__ __forceinline__ void update_tile(const UpdArgs& g, const UpdTile& u, bool live,
+                                            int tid, UpdShared<T, NB, KC>& sh) {
   typedef InvMma<T> MM;
   typedef typename MM::acc_t acc_t;
   static_assert(NB % KC == 0 || NB < KC, "panel width: a multiple of the chunk, or below it");
-  constexpr int KW = NB < KC ? NB : KC;  // staged k per chunk
-  __shared__ T As[2][KW][64 + 1];
-  __shared__ T Bs[2][KW][64 + 1];
-  __shared__ int sig_r[64], sig_k[NB];
-  const int tiles = g.tiles;  // per matrix, row-major over (row tile, col tile)
-  const int tn = (g.n + 63) / 64;
-  // the tiles of one matrix on one XCD, so that the panel columns and pivot
-  // rows every tile reads again come from that XCD's L2
-  const int id = update_xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t e = id / tiles;
-  const int t = id - (int)(e * tiles);
-  const int r0 = (t / tn) * 64, c0 = (t % tn) * 64;
+  constexpr int KW = UpdShared<T, NB, KC>::KW;
+  auto& As = sh.As;
+  auto& Bs = sh.Bs;
+  auto& sig_r = sh.sig_r;
+  auto& sig_k = sh.sig_k;
+  const int64_t e = u.e;
+  const int r0 = u.r0, c0 = u.c0;
   const int n = g.n, k0 = g.k0, nbw = g.nbw;
-  // a column tile that lies inside the panel has nothing to do
-  if (c0 >= k0 && c0 + 64 <= k0 + nbw) return;
-  if (g.mask && !g.mask[e]) return;
   const z2* Xc = g.Xc + e * g.sXc;
   z2* Xn = g.Xn + e * g.sXn;
-  const int* sig = g.ints + e * g.sI + n;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int* sig = g.ints + e * g.sI + (g.sig_sel ? 0 : n);
+  const int lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
 
   if (tid < 64) sig_r[tid] = (r0 + tid < n) ? sig[r0 + tid] : 0;
@@ -755,7 +809,7 @@ void inv_update(UpdArgs g) {
         const int r = r0 + rl;
         const int c = c0 + wn * 32 + j * 16 + (lane & 15);
         // unconditional loads from clamped addresses: all in flight together
-        const bool ok = r < n && c < n && !(r >= k0 && r < k0 + nbw);
+        const bool ok = live && r < n && c < n && !(r >= k0 && r < k0 + nbw);
         const z2 v = Xc[ok ? (int64_t)sig_r[rl] * g.ldc + c : 0];
         aR[i][j][q] = ok ? (T)v.x : (T)0;
         aI[i][j][q] = ok ? (T)v.y : (T)0;
@@ -771,11 +825,11 @@ void inv_update(UpdArgs g) {
     for (int it = 0; it < NS; ++it) {
       const int q = tid + it * NT;
       const int r = q / KW, k = q - r * KW;
-      const bool oka = r0 + r < n && kc + k < nbw;
+      const bool oka = live && r0 + r < n && kc + k < nbw;
       va[it] = Xn[oka ? (int64_t)(r0 + r) * g.ldn + k0 + kc + k : 0];
       if (!oka) va[it] = make_double2(0.0, 0.0);
       const int k2 = q >> 6, c = q & 63;
-      const bool okb = kc + k2 < nbw && c0 + c < n;
+      const bool okb = live && kc + k2 < nbw && c0 + c < n;
       vb[it] = Xc[okb ? (int64_t)sig_k[kc + k2] * g.ldc + c0 + c : 0];
       if (!okb) vb[it] = make_double2(0.0, 0.0);
     }
@@ -818,6 +872,7 @@ void inv_update(UpdArgs g) {
         }
     }
   }
+  if (!live) return;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -829,6 +884,47 @@ void inv_update(UpdArgs g) {
         if (r < n && c < n && !(c >= k0 && c < k0 + nbw))
           Xn[(int64_t)r * g.ldn + c] = make_double2((double)aR[i][j][q], (double)aI[i][j][q]);
       }
+}
+
+// The update as its own launch: one tile per workgroup, the tiles of one
+// matrix on one XCD, so that the panel columns and pivot rows every tile
+// reads again come from that XCD's L2.  WPE: register target (waves per SIMD).
+template <typename T, int NB, int KC, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+void inv_update(UpdArgs g) {
+  __shared__ UpdShared<T, NB, KC> sh;
+  const UpdTile u = update_tile_of(g, update_xcd_remap(blockIdx.x, gridDim.x));
+  if (!update_live(g, u)) return;  // uniform over the workgroup
+  update_tile<T, NB, KC>(g, u, true, threadIdx.x, sh);
+}
+
+// Look-ahead step of the 32-column factorisation: the panel of step p + 1 on
+// workgroups [0, np) (np: the batch rounded up to a multiple of 8, so that
+// the update's workgroup ids below keep their XCD), and the update of step p
+// on the others, two tiles per workgroup of 512 threads (one per half).
+// The update passed here covers every column tile except the one holding
+// panel p + 1's columns, which an update launch before this one completed,
+// so the two parts touch disjoint data: the panel reads its columns of the
+// buffer the update writes and writes its columns of the buffer the update
+// reads, and the row gathers of consecutive panels live in different sig
+// buffers.
+template <typename T, int RPL>
+__global__ __launch_bounds__(512) void inv_step(PanelArgs pa, UpdArgs ua, int batch, int np,
+                                                int64_t total) {
+  __shared__ union {
+    PanelCgShared<T, RPL, 4, 32> p;
+    UpdShared<T, 32, 32> u[2];
+  } sh;
+  if ((int)blockIdx.x < np) {
+    if ((int)blockIdx.x < batch) panel_cg<T, RPL, 4, 32>(pa, blockIdx.x, sh.p);
+    return;
+  }
+  const int half = threadIdx.x >> 8;
+  const int64_t id =
+      2 * (int64_t)update_xcd_remap(blockIdx.x - np, gridDim.x - np) + half;
+  const bool in = id < total;
+  const UpdTile u = update_tile_of(ua, in ? (int)id : 0);
+  update_tile<T, 32, 32>(ua, u, in && update_live(ua, u), threadIdx.x & (NT - 1), sh.u[half]);
 }
 
 // X[r, c] = Y[r, idx c]; X may not alias Y.
@@ -913,6 +1009,76 @@ void launch_panel_cg(hipStream_t s, int64_t batch, int n, const PanelArgs& pa) {
   }
 }
 
+// Look-ahead (default below 256 matrices per call; QX_INV_LOOKAHEAD=0, read
+// per call, turns it off): with 32-column panels of eight-wave column
+// groups, the update of step p is split into the column tile that holds
+// panel p + 1 (its own launch) and the rest, which runs in one launch with
+// panel p + 1 (inv_step), so the pivoting chain of the next panel no longer
+// waits for the whole update.  The same kernels' arithmetic on the same
+// data: the same bits.  32 x 256^2: 559 -> 500 us per inverse, 32 x 416^2
+// 1691 -> 1245, 32 x 512^2 2315 -> 1642.  From 256 matrices on, where the
+// update streams every matrix from HBM and runs 16-deep chunks with four
+// workgroups per CU, the combined launch (one workgroup of 512 threads per
+// CU) was slower: 1024 x 256^2 5.77 -> 6.87 ms.
+bool inv_lookahead() {
+  const char* e = std::getenv("QX_INV_LOOKAHEAD");
+  return !(e && std::atoi(e) == 0);
+}
+
+// An update launch of the look-ahead (32-deep chunks, as inv_step).
+template <typename T>
+void launch_update(hipStream_t s, int64_t batch, const UpdArgs& ua) {
+  const int64_t blocks = batch * ua.tiles;
+  QX_REQUIRE(blocks < (int64_t(1) << 31), "inv: grid too large");
+  hipLaunchKernelGGL((inv_update<T, 32, 32, 1>), dim3((unsigned)blocks), dim3(NT), 0, s, ua);
+}
+
+template <typename T>
+void run_inverse_lookahead(hipStream_t s, int64_t batch, int n, const Mat (&B)[2], int s0,
+                           const InvScratch& scr, int* info, const int* mask) {
+  constexpr int NB = 32;
+  const int npan = (n + NB - 1) / NB, tn = (n + 63) / 64;
+  auto panel_args = [&](int p) {
+    const int c = (s0 + p) & 1, k0 = p * NB;
+    return PanelArgs{n, k0, min(NB, n - k0), B[c].p, B[c].stride, B[c].ld,
+                     B[1 - c].p, B[1 - c].stride, B[1 - c].ld, scr.ints, scr.int_stride,
+                     info, mask, p & 1};
+  };
+  launch_panel_cg<T, NB>(s, batch, n, panel_args(0));
+  for (int p = 0; p < npan; ++p) {
+    const PanelArgs pa = panel_args(p);
+    UpdArgs ua{n, pa.k0, pa.nbw, tn * tn, pa.Xc, pa.sXc, pa.ldc, pa.Xn, pa.sXn, pa.ldn,
+               scr.ints, scr.int_stride, mask, p & 1, 0, 0, 0};
+    if (p + 1 == npan) {
+      launch_update<T>(s, batch, ua);
+      break;
+    }
+    // the column tile of panel p + 1 first
+    const int cx = (pa.k0 + NB) / 64;
+    UpdArgs un = ua;
+    un.cfirst = cx;
+    un.ncols = 1;
+    un.tiles = tn;
+    launch_update<T>(s, batch, un);
+    // then panel p + 1 beside the other column tiles
+    UpdArgs ur = ua;
+    ur.ncols = tn - 1;
+    ur.cskip1 = cx + 1;
+    ur.tiles = tn * (tn - 1);
+    const int64_t total = batch * ur.tiles;
+    const int np = (int)((batch + 7) / 8 * 8);
+    const int64_t blocks = np + (total + 1) / 2;
+    QX_REQUIRE(blocks < (int64_t(1) << 31), "inv: grid too large");
+    const PanelArgs pn = panel_args(p + 1);
+    if (n <= 256)
+      hipLaunchKernelGGL((inv_step<T, 4>), dim3((unsigned)blocks), dim3(512), 0, s, pn, ur,
+                         (int)batch, np, total);
+    else
+      hipLaunchKernelGGL((inv_step<T, 8>), dim3((unsigned)blocks), dim3(512), 0, s, pn, ur,
+                         (int)batch, np, total);
+  }
+}
+
 template <typename T, int NB, int RPT>
 void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScratch& scr,
                  int s0, int* info, const int* mask) {
@@ -921,7 +1087,10 @@ void run_inverse(hipStream_t s, int64_t batch, int n, const Mat& X, const InvScr
   const Mat B[2] = {X, Mat(scr.buf, scr.buf_stride, n)};
   int cur = s0;
   const int tn = (n + 63) / 64;
-  for (int p = 0; p < npan; ++p) {
+  const bool la = NB == 32 && kind == 0 && inv_group_width() == 4 && npan > 1 &&
+                  !narrow_update(batch) && inv_lookahead();
+  if (la) run_inverse_lookahead<T>(s, batch, n, B, s0, scr, info, mask);
+  for (int p = 0; !la && p < npan; ++p) {
     const int k0 = p * NB, nbw = min(NB, n - k0), nxt = 1 - cur;
     PanelArgs pa{n, k0, nbw, B[cur].p, B[cur].stride, B[cur].ld,
                  B[nxt].p, B[nxt].stride, B[nxt].ld, scr.ints, scr.int_stride, info, mask};

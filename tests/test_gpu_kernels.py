@@ -299,6 +299,32 @@ def test_update_chunk_depth_bitwise(device, monkeypatch, n, prec):
         assert rel_err(outs["16"][e], np.linalg.inv(a[e])) < tol
 
 
+@pytest.mark.parametrize("n,batch", [(40, 5), (64, 3), (100, 5), (256, 6), (256, 300),
+                                     (300, 4), (512, 2), (130, 260)])
+@pytest.mark.parametrize("prec", [0, 1], ids=["c128", "c64"])
+def test_lookahead_bitwise(device, monkeypatch, n, batch, prec):
+    """Look-ahead (the next panel launched beside the rest of the current
+    update) against the panel / update sequence: the same kernels' arithmetic
+    on the same data, so the same bits, for one and several column tiles and
+    both panel row counts (n > 256).  From 256 matrices on (300, 260) both
+    settings run the sequence."""
+    rng = np.random.default_rng(7 * n + batch + prec)
+    a = _cplx(rng, (batch, n, n))
+    a[:, 0, 0] = 0.0  # the first pivot needs a row interchange
+    outs = {}
+    for la in ("1", "0"):
+        monkeypatch.setenv("QX_INV_LOOKAHEAD", la)
+        A = torch.as_tensor(a, device=device)
+        X = torch.empty_like(A)
+        _inv(device, A, X, prec)
+        outs[la] = X.cpu().numpy()
+    assert np.array_equal(outs["1"].view(np.uint64), outs["0"].view(np.uint64))
+    for e in range(2):
+        cond = np.linalg.cond(a[e])
+        tol = (1e-15 if prec == 0 else 1e-7) * cond * n
+        assert rel_err(outs["1"][e], np.linalg.inv(a[e])) < tol
+
+
 @pytest.mark.parametrize("nb", ["32", "64"])
 @pytest.mark.parametrize("n", [64, 300])
 def test_singular_panel_raises(device, monkeypatch, nb, n):
