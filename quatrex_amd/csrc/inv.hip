@@ -1015,8 +1015,8 @@ void launch_panel_cg(hipStream_t s, int64_t batch, int n, const PanelArgs& pa) {
 // panel p + 1 (its own launch) and the rest, which runs in one launch with
 // panel p + 1 (inv_step), so the pivoting chain of the next panel no longer
 // waits for the whole update.  The same kernels' arithmetic on the same
-// data: the same bits.  32 x 256^2: 559 -> 500 us per inverse, 32 x 416^2
-// 1691 -> 1245, 32 x 512^2 2315 -> 1642.  From 256 matrices on, where the
+// data: the same bits.  32 x 256^2: 557 -> 500 us per inverse, 32 x 416^2
+// 1691 -> 1241, 32 x 512^2 2315 -> 1629.  From 256 matrices on, where the
 // update streams every matrix from HBM and runs 16-deep chunks with four
 // workgroups per CU, the combined launch (one workgroup of 512 threads per
 // CU) was slower: 1024 x 256^2 5.77 -> 6.87 ms.
@@ -1060,7 +1060,8 @@ void run_inverse_lookahead(hipStream_t s, int64_t batch, int n, const Mat (&B)[2
     un.ncols = 1;
     un.tiles = tn;
     launch_update<T>(s, batch, un);
-    // then panel p + 1 beside the other column tiles
+    // then panel p + 1 beside the other column tiles (n <= 64 has none:
+    // total = 0 and the launch is the panel alone)
     UpdArgs ur = ua;
     ur.ncols = tn - 1;
     ur.cskip1 = cx + 1;
