@@ -485,20 +485,29 @@ __device__ inline unsigned wave_max_bcast(unsigned v) {
 //     LDS, double-buffered by column parity so no second barrier guards the
 //     reuse;
 //   - after the barrier every wave reads the pivot key and value and the
-//     multipliers of its rows; the lane holding row p writes this wave's
-//     part of it (and its position) to a per-wave LDS slot, which the wave
-//     reads back; 1 / x_pk, the scaled pivot row and the elimination follow;
+//     multipliers of its rows, and takes its part of row p (and the row's
+//     position) from the lane that holds it (v_readlane, p being uniform);
+//     1 / x_pk, the scaled pivot row and the elimination follow;
 //   - the owner of the next column computes and publishes its pivot data.
 // Pivot choice (largest |re| + |im| through the 32-bit key, ties to the lower
 // row), arithmetic and outputs are those of the other panel kernels up to
 // rounding; ties are broken by the row's index at the start of the panel.
+// Value of v in lane l of the wave (l uniform).
+__device__ inline double lane_read(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline float lane_read(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 template <typename T, int RPL, int CW, int NB>
 struct PanelCgShared {
   T col_r[2][64 * RPL], col_i[2][64 * RPL];
   T piv_r[2], piv_i[2];
   unsigned key_b[2];
-  T prow_r[NB / CW][CW], prow_i[NB / CW][CW];
-  int prow_pos[NB / CW];
   double rcp_r[NB], rcp_i[NB];
   int perm_l[NB];
 };
@@ -513,9 +522,6 @@ __device__ __forceinline__ void panel_cg(const PanelArgs& g, int64_t e,
   auto& piv_r = sh.piv_r;
   auto& piv_i = sh.piv_i;
   auto& key_b = sh.key_b;
-  auto& prow_r = sh.prow_r;
-  auto& prow_i = sh.prow_i;
-  auto& prow_pos = sh.prow_pos;
   auto& rcp_r = sh.rcp_r;
   auto& rcp_i = sh.rcp_i;
   auto& perm_l = sh.perm_l;
@@ -598,34 +604,33 @@ __device__ __forceinline__ void panel_cg(const PanelArgs& g, int64_t e,
           fr[j] = col_r[buf][lane + 64 * j];  // zero for the pivot row
           fi[j] = col_i[buf][lane + 64 * j];
         }
-        // this wave's part of row p and its position, through LDS
+        // this wave's part of row p and its position, read from the lane
+        // that holds them (p is uniform: register row p >> 6 of lane p & 63)
+        const int pl = p & 63, pj = p >> 6;
+        T ar[CW], ai[CW];
+        int q = 0;
+#pragma unroll
+        for (int c = 0; c < CW; ++c) ar[c] = ai[c] = (T)0;
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
-          if (lane + 64 * j == p) {
+          if (j == pj) {  // uniform: a scalar branch
 #pragma unroll
             for (int c = 0; c < CW; ++c) {
-              prow_r[w][c] = xr[j][c];
-              prow_i[w][c] = xi[j][c];
+              ar[c] = lane_read(xr[j][c], pl);
+              ai[c] = lane_read(xi[j][c], pl);
             }
-            prow_pos[w] = pos[j];
+            q = __builtin_amdgcn_readlane(pos[j], pl);
           }
         }
-        // The other lanes read what one lane of this wave wrote: without the
-        // fences the compiler may forward the writing lane's registers and
-        // place the other lanes' reads ahead of the write.
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         double vr, vi;
         complex_recip(pvr, pvi, vr, vi);
         T sr[CW], si[CW];
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
-          const double a = (double)prow_r[w][c], b = (double)prow_i[w][c];
+          const double a = (double)ar[c], b = (double)ai[c];
           sr[c] = (T)(a * vr - b * vi);
           si[c] = (T)(a * vi + b * vr);
         }
-        const int q = prow_pos[w];
         if (w == qa) {
           // active column: -f / x_pk for the other rows, 1 (scaled at the
           // end) for the pivot row
